@@ -1,0 +1,161 @@
+/*
+ * osqp_amd_algebra.h — the algebra/hip backend: OSQP's algebra plugin
+ * interface implemented with hand-written gfx950 HIP kernels.
+ *
+ * These are the symbols OSQP's solver core (src/osqp_api.c, src/auxil.c,
+ * src/scaling.c) calls; a maintainer adds this backend to the reference build
+ * as algebra/hip and links libosqp_amd.so instead of algebra/builtin.
+ * Declarations and their meaning follow:
+ *   include/private/algebra_vector.h:28-290
+ *   include/private/algebra_matrix.h:26-138
+ *   include/private/lin_alg.h:13-53
+ *   include/private/types.h:243-279   (struct linsys_solver)
+ *
+ * All vector and matrix storage lives in device memory (HBM); vectors are
+ * FP64, indices int32.  Functions returning a scalar synchronise the backend
+ * stream; everything else is asynchronous on it.
+ */
+#ifndef OSQP_AMD_ALGEBRA_H
+#define OSQP_AMD_ALGEBRA_H
+
+#include "osqp_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OSQPVectorf_ OSQPVectorf;
+typedef struct OSQPVectori_ OSQPVectori;
+typedef struct OSQPMatrix_  OSQPMatrix;
+typedef struct linsys_solver LinSysSolver;
+
+/* ---- vectors: include/private/algebra_vector.h ------------------------ */
+OSQPInt      OSQPVectorf_is_eq(const OSQPVectorf* A, const OSQPVectorf* B, OSQPFloat tol); /* :28 */
+OSQPVectorf* OSQPVectorf_new(const OSQPFloat* a, OSQPInt length);                          /* :39 */
+OSQPVectori* OSQPVectori_new(const OSQPInt* a, OSQPInt length);                            /* :43 */
+OSQPVectorf* OSQPVectorf_malloc(OSQPInt length);                                           /* :33 */
+OSQPVectori* OSQPVectori_malloc(OSQPInt length);                                           /* :35 */
+OSQPVectorf* OSQPVectorf_calloc(OSQPInt length);                                           /* :34 */
+OSQPVectori* OSQPVectori_calloc(OSQPInt length);                                           /* :36 */
+OSQPVectorf* OSQPVectorf_copy_new(const OSQPVectorf* a);                                   /* :47 */
+void         OSQPVectorf_free(OSQPVectorf* a);                                             /* :50 */
+void         OSQPVectori_free(OSQPVectori* a);                                             /* :53 */
+OSQPVectorf* OSQPVectorf_view(const OSQPVectorf* a, OSQPInt head, OSQPInt length);         /* :81 */
+void         OSQPVectorf_view_free(OSQPVectorf* a);                                        /* :91 */
+OSQPInt      OSQPVectorf_length(const OSQPVectorf* a);                                     /* :97 */
+OSQPInt      OSQPVectori_length(const OSQPVectori* a);                                     /* :100 */
+void OSQPVectorf_copy(OSQPVectorf* b, const OSQPVectorf* a);                               /* :106 */
+void OSQPVectorf_from_raw(OSQPVectorf* b, const OSQPFloat* av);                            /* :110 */
+void OSQPVectori_from_raw(OSQPVectori* b, const OSQPInt* av);                              /* :114 */
+void OSQPVectorf_to_raw(OSQPFloat* bv, const OSQPVectorf* a);                              /* :118 */
+void OSQPVectori_to_raw(OSQPInt* bv, const OSQPVectori* a);                                /* :122 */
+void OSQPVectorf_set_scalar(OSQPVectorf* a, OSQPFloat sc);                                 /* :126 */
+void OSQPVectorf_set_scalar_conditional(OSQPVectorf* a, const OSQPVectori* test,
+                                        OSQPFloat sc_if_neg, OSQPFloat sc_if_zero,
+                                        OSQPFloat sc_if_pos);                              /* :130 */
+void OSQPVectorf_round_to_zero(OSQPVectorf* a, OSQPFloat tol);                             /* :139 */
+void OSQPVectorf_mult_scalar(OSQPVectorf* a, OSQPFloat sc);                                /* :143 */
+void OSQPVectorf_plus(OSQPVectorf* x, const OSQPVectorf* a, const OSQPVectorf* b);         /* :147 */
+void OSQPVectorf_minus(OSQPVectorf* x, const OSQPVectorf* a, const OSQPVectorf* b);        /* :152 */
+void OSQPVectorf_add_scaled(OSQPVectorf* x, OSQPFloat sca, const OSQPVectorf* a,
+                            OSQPFloat scb, const OSQPVectorf* b);                          /* :157 */
+void OSQPVectorf_add_scaled3(OSQPVectorf* x, OSQPFloat sca, const OSQPVectorf* a,
+                             OSQPFloat scb, const OSQPVectorf* b,
+                             OSQPFloat scc, const OSQPVectorf* c);                         /* :164 */
+OSQPFloat OSQPVectorf_norm_inf(const OSQPVectorf* v);                                      /* :173 */
+OSQPFloat OSQPVectorf_scaled_norm_inf(const OSQPVectorf* S, const OSQPVectorf* v);         /* :176 */
+OSQPFloat OSQPVectorf_norm_inf_diff(const OSQPVectorf* a, const OSQPVectorf* b);           /* :180 */
+OSQPFloat OSQPVectorf_norm_1(const OSQPVectorf* a);                                        /* :187 */
+OSQPFloat OSQPVectorf_dot_prod(const OSQPVectorf* a, const OSQPVectorf* b);                /* :190 */
+OSQPFloat OSQPVectorf_dot_prod_signed(const OSQPVectorf* a, const OSQPVectorf* b,
+                                      OSQPInt sign);                                       /* :198 */
+void OSQPVectorf_ew_prod(OSQPVectorf* c, const OSQPVectorf* a, const OSQPVectorf* b);      /* :203 */
+OSQPInt OSQPVectorf_all_leq(const OSQPVectorf* l, const OSQPVectorf* u);                   /* :210 */
+void OSQPVectorf_ew_bound_vec(OSQPVectorf* x, const OSQPVectorf* z,
+                              const OSQPVectorf* l, const OSQPVectorf* u);                 /* :217 */
+void OSQPVectorf_project_polar_reccone(OSQPVectorf* y, const OSQPVectorf* l,
+                                       const OSQPVectorf* u, OSQPFloat infval);            /* :227 */
+OSQPInt OSQPVectorf_in_reccone(const OSQPVectorf* y, const OSQPVectorf* l,
+                               const OSQPVectorf* u, OSQPFloat infval, OSQPFloat tol);     /* :237 */
+void OSQPVectorf_ew_reciprocal(OSQPVectorf* b, const OSQPVectorf* a);                      /* :246 */
+void OSQPVectorf_ew_sqrt(OSQPVectorf* a);                                                  /* :250 */
+void OSQPVectorf_ew_max_vec(OSQPVectorf* c, const OSQPVectorf* a, const OSQPVectorf* b);   /* :253 */
+void OSQPVectorf_ew_min_vec(OSQPVectorf* c, const OSQPVectorf* a, const OSQPVectorf* b);   /* :258 */
+OSQPInt OSQPVectorf_ew_bounds_type(OSQPVectori* iseq, const OSQPVectorf* l,
+                                   const OSQPVectorf* u, OSQPFloat tol, OSQPFloat infval); /* :269 */
+void OSQPVectorf_set_scalar_if_lt(OSQPVectorf* x, const OSQPVectorf* z,
+                                  OSQPFloat testval, OSQPFloat newval);                    /* :279 */
+void OSQPVectorf_set_scalar_if_gt(OSQPVectorf* x, const OSQPVectorf* z,
+                                  OSQPFloat testval, OSQPFloat newval);                    /* :287 */
+
+/* ---- matrices: include/private/algebra_matrix.h ----------------------- */
+OSQPInt     OSQPMatrix_is_eq(const OSQPMatrix* A, const OSQPMatrix* B, OSQPFloat tol);     /* :26 */
+OSQPMatrix* OSQPMatrix_new_from_csc(const OSQPCscMatrix* A, OSQPInt is_triu);              /* :34 */
+void        OSQPMatrix_update_values(OSQPMatrix* M, const OSQPFloat* Mx_new,
+                                     const OSQPInt* Mx_new_idx, OSQPInt M_new_n);          /* :60 */
+OSQPInt     OSQPMatrix_get_m(const OSQPMatrix* M);                                         /* :66 */
+OSQPInt     OSQPMatrix_get_n(const OSQPMatrix* M);                                         /* :69 */
+OSQPInt     OSQPMatrix_get_nz(const OSQPMatrix* M);                                        /* :81 */
+void        OSQPMatrix_mult_scalar(OSQPMatrix* A, OSQPFloat sc);                           /* :86 */
+void        OSQPMatrix_lmult_diag(OSQPMatrix* A, const OSQPVectorf* L);                    /* :90 */
+void        OSQPMatrix_rmult_diag(OSQPMatrix* A, const OSQPVectorf* R);                    /* :94 */
+void        OSQPMatrix_Axpy(const OSQPMatrix* A, const OSQPVectorf* x, OSQPVectorf* y,
+                            OSQPFloat alpha, OSQPFloat beta);                              /* :107 */
+void        OSQPMatrix_Atxpy(const OSQPMatrix* A, const OSQPVectorf* x, OSQPVectorf* y,
+                             OSQPFloat alpha, OSQPFloat beta);                             /* :114 */
+void        OSQPMatrix_col_norm_inf(const OSQPMatrix* M, OSQPVectorf* E);                  /* :125 */
+void        OSQPMatrix_row_norm_inf(const OSQPMatrix* M, OSQPVectorf* E);                  /* :128 */
+void        OSQPMatrix_free(OSQPMatrix* M);                                                /* :135 */
+OSQPMatrix* OSQPMatrix_submatrix_byrows(const OSQPMatrix* A, const OSQPVectori* rows);     /* :137 */
+/* d = diag(A' diag(D) A)  (algebra_matrix.h:98) */
+void        OSQPMatrix_AtDA_extract_diag(const OSQPMatrix* A, const OSQPVectorf* D,
+                                         OSQPVectorf* d);
+/* d = diag(A)  (algebra_matrix.h:103) */
+void        OSQPMatrix_extract_diag(const OSQPMatrix* A, OSQPVectorf* d);
+
+/* ---- libraries and linear system: include/private/lin_alg.h ---------- */
+OSQPInt osqp_algebra_linsys_supported(void);                                               /* :13 */
+enum osqp_linsys_solver_type osqp_algebra_default_linsys(void);                            /* :16 */
+OSQPInt osqp_algebra_init_libs(OSQPInt device);                                            /* :19 */
+void    osqp_algebra_free_libs(void);                                                      /* :22 */
+OSQPInt osqp_algebra_name(char* name, OSQPInt nameLen);                                    /* :25 */
+OSQPInt osqp_algebra_device_name(char* name, OSQPInt nameLen);                             /* :28 */
+OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** s, const OSQPMatrix* P,
+                                        const OSQPMatrix* A, const OSQPVectorf* rho_vec,
+                                        const OSQPSettings* settings,
+                                        OSQPFloat* scaled_prim_res,
+                                        OSQPFloat* scaled_dual_res,
+                                        OSQPInt polishing);                                /* :46 */
+
+/* include/private/types.h:243-279 — the linear-system plugin vtable */
+struct linsys_solver {
+  enum osqp_linsys_solver_type type;
+  const char* (*name)(LinSysSolver* self);
+  OSQPInt (*solve)(LinSysSolver* self, OSQPVectorf* b, OSQPInt admm_iter);
+  void    (*update_settings)(LinSysSolver* self, const OSQPSettings* settings);
+  void    (*warm_start)(LinSysSolver* self, const OSQPVectorf* x);
+  OSQPInt (*adjoint_derivative)(LinSysSolver* self);
+  void    (*free)(LinSysSolver* self);
+  OSQPInt (*update_matrices)(LinSysSolver* self,
+                             const OSQPMatrix* P, const OSQPInt* Px_new_idx, OSQPInt P_new_n,
+                             const OSQPMatrix* A, const OSQPInt* Ax_new_idx, OSQPInt A_new_n);
+  OSQPInt (*update_rho_vec)(LinSysSolver* self, const OSQPVectorf* rho_vec, OSQPFloat rho_sc);
+  OSQPInt nthreads;
+};
+
+/* ---- MI355X extensions (no counterpart in the reference) -------------- */
+/* Device pointer of a vector's storage (for zero-copy hand-off between
+ * the host core and the fused kernels of this backend). */
+OSQPFloat* OSQPVectorf_device_ptr(const OSQPVectorf* a);
+/* Blocks until all queued work of the backend stream has finished. */
+void       osqp_amd_algebra_sync(void);
+/* Last-solve counters of a PCG linsys: total CG iterations and solve calls. */
+void       osqp_amd_linsys_counters(LinSysSolver* s, OSQPInt* cg_iters, OSQPInt* solves);
+/* Last HIP error seen by the backend (0 if none); cleared on read. */
+OSQPInt    osqp_amd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OSQP_AMD_ALGEBRA_H */

@@ -1,0 +1,437 @@
+// vector.hip — OSQPVectorf/OSQPVectori on HBM.
+//
+// Semantics follow the builtin backend (algebra/builtin/vector.c:9-953),
+// including its operation order, so that every elementwise result is the
+// same correctly rounded IEEE value (compiled with -ffp-contract=off).
+// Reductions are deterministic two-pass trees (per-block partials, then one
+// block), never atomics; max-type reductions are therefore bit-exact against
+// the CPU and sums differ only by summation order.
+#include <math.h>
+
+#include "hip_internal.h"
+
+using namespace osqp_hip;
+
+namespace {
+
+// ---- elementwise kernels ---------------------------------------------------
+// c_max / c_min of the reference (glob_opts.h): ternaries, not fmax/fmin,
+// so signed zeros and ties resolve exactly as on the CPU.
+__device__ inline double cmax(double a, double b) { return a > b ? a : b; }
+__device__ inline double cmin(double a, double b) { return a < b ? a : b; }
+
+#define GRID_STRIDE(i, n) \
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+__global__ void k_set(double* a, double sc, int n) { GRID_STRIDE(i, n) a[i] = sc; }
+__global__ void k_set_int(int* a, int sc, int n) { GRID_STRIDE(i, n) a[i] = sc; }
+
+__global__ void k_set_cond(double* a, const int* t, double neg, double zero, double pos, int n) {
+  GRID_STRIDE(i, n) {
+    int v = t[i];
+    a[i] = (v == 0) ? zero : ((v > 0) ? pos : neg);
+  }
+}
+
+__global__ void k_round_to_zero(double* a, double tol, int n) {
+  GRID_STRIDE(i, n) if (fabs(a[i]) < tol) a[i] = 0.0;
+}
+
+__global__ void k_mult_sc(double* a, double sc, int n) { GRID_STRIDE(i, n) a[i] *= sc; }
+
+__global__ void k_plus(double* x, const double* a, const double* b, int n) { GRID_STRIDE(i, n) x[i] = a[i] + b[i]; }
+__global__ void k_minus(double* x, const double* a, const double* b, int n) { GRID_STRIDE(i, n) x[i] = a[i] - b[i]; }
+
+__global__ void k_add_scaled(double* x, double sa, const double* a, double sb, const double* b, int n) {
+  GRID_STRIDE(i, n) x[i] = sa * a[i] + sb * b[i];
+}
+
+// vector.c:439-466: the increment form x += (sb*b + sc*c) rounds differently
+// from sa*a + sb*b + sc*c, so both forms are kept.
+__global__ void k_add_scaled3(double* x, double sa, const double* a, double sb, const double* b, double sc,
+                              const double* c, int incr, int n) {
+  if (incr) {
+    GRID_STRIDE(i, n) x[i] += sb * b[i] + sc * c[i];
+  } else {
+    GRID_STRIDE(i, n) x[i] = sa * a[i] + sb * b[i] + sc * c[i];
+  }
+}
+
+__global__ void k_ew_prod(double* c, const double* a, const double* b, int n) { GRID_STRIDE(i, n) c[i] = a[i] * b[i]; }
+
+__global__ void k_bound(double* x, const double* z, const double* l, const double* u, int n) {
+  GRID_STRIDE(i, n) x[i] = cmin(cmax(z[i], l[i]), u[i]);
+}
+
+__global__ void k_polar_reccone(double* y, const double* l, const double* u, double infval, int n) {
+  GRID_STRIDE(i, n) {
+    if (u[i] > +infval) {
+      if (l[i] < -infval) y[i] = 0.0;
+      else y[i] = cmin(y[i], 0.0);
+    } else if (l[i] < -infval) {
+      y[i] = cmax(y[i], 0.0);
+    }
+  }
+}
+
+__global__ void k_reciprocal(double* b, const double* a, int n) { GRID_STRIDE(i, n) b[i] = 1.0 / a[i]; }
+__global__ void k_sqrt(double* a, int n) { GRID_STRIDE(i, n) a[i] = sqrt(a[i]); }
+__global__ void k_max_vec(double* c, const double* a, const double* b, int n) {
+  GRID_STRIDE(i, n) c[i] = a[i] > b[i] ? a[i] : b[i];
+}
+__global__ void k_min_vec(double* c, const double* a, const double* b, int n) {
+  GRID_STRIDE(i, n) c[i] = a[i] < b[i] ? a[i] : b[i];
+}
+__global__ void k_if_lt(double* x, const double* z, double t, double v, int n) {
+  GRID_STRIDE(i, n) x[i] = z[i] < t ? v : z[i];
+}
+__global__ void k_if_gt(double* x, const double* z, double t, double v, int n) {
+  GRID_STRIDE(i, n) x[i] = z[i] > t ? v : z[i];
+}
+
+// ---- reductions --------------------------------------------------------------
+enum RedOp : int {
+  R_NORM_INF = 0,
+  R_SCALED_NORM_INF,
+  R_NORM_INF_DIFF,
+  R_NORM_1,
+  R_DOT,
+  R_DOT_POS,
+  R_DOT_NEG,
+  R_ANY_GT,        // max over i of (a[i] > b[i])       -> all_leq = !result
+  R_RECCONE_BAD,   // max over i of reccone violation  -> in_reccone = !result
+  R_EQ_BAD,        // max over i of |a-b| > tol        -> is_eq = !result
+};
+
+struct RedArgs {
+  const double* a;
+  const double* b;
+  const double* c;
+  const double* d;
+  double p0, p1;
+};
+
+__device__ inline bool is_max_op(int op) { return op != R_NORM_1 && op != R_DOT && op != R_DOT_POS && op != R_DOT_NEG; }
+
+__device__ inline double red_elem(int op, const RedArgs& r, long long i) {
+  switch (op) {
+    case R_NORM_INF: return fabs(r.a[i]);
+    case R_SCALED_NORM_INF: return fabs(r.a[i] * r.b[i]);
+    case R_NORM_INF_DIFF: return fabs(r.a[i] - r.b[i]);
+    case R_NORM_1: return fabs(r.a[i]);
+    case R_DOT: return r.a[i] * r.b[i];
+    case R_DOT_POS: return r.a[i] * cmax(r.b[i], 0.0);
+    case R_DOT_NEG: return r.a[i] * cmin(r.b[i], 0.0);
+    case R_ANY_GT: return r.a[i] > r.b[i] ? 1.0 : 0.0;
+    case R_RECCONE_BAD: {
+      // vector.c:704-727 with y=a, l=b, u=c, infval=p0, tol=p1
+      double y = r.a[i], l = r.b[i], u = r.c[i];
+      bool bad = ((u < +r.p0) && (y > +r.p1)) || ((l > -r.p0) && (y < -r.p1));
+      return bad ? 1.0 : 0.0;
+    }
+    case R_EQ_BAD: return fabs(r.a[i] - r.b[i]) > r.p0 ? 1.0 : 0.0;
+  }
+  return 0.0;
+}
+
+__global__ void k_reduce_partials(int op, RedArgs r, int n, double* partials) {
+  __shared__ double lds[kBlock / kWave];
+  const bool mx = is_max_op(op);
+  double acc = 0.0;
+  GRID_STRIDE(i, n) {
+    double v = red_elem(op, r, i);
+    if (mx) acc = v > acc ? v : acc;
+    else acc += v;
+  }
+  double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void k_reduce_final(int op, const double* partials, int nparts, double* out) {
+  __shared__ double lds[kBlock / kWave];
+  const bool mx = is_max_op(op);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    double v = partials[i];
+    if (mx) acc = v > acc ? v : acc;
+    else acc += v;
+  }
+  double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
+  if (threadIdx.x == 0) *out = t;
+}
+
+double reduce(int op, const RedArgs& r, int n) {
+  if (n <= 0) return 0.0;
+  Runtime& R = rt();
+  int g = reduce_grid(n);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(g), dim3(kBlock), 0, R.stream, op, r, n, R.d_partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, R.stream, op, R.d_partials, g, R.d_scalars);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(R.h_scalars, R.d_scalars, sizeof(double), hipMemcpyDeviceToHost, R.stream));
+  HIPCHK(hipStreamSynchronize(R.stream));
+  return R.h_scalars[0];
+}
+
+// ew_bounds_type: classify constraints (vector.c:885-921; loose checked
+// before equality, as the builtin does) and report whether any type changed.
+__global__ void k_bounds_type(int* iseq, const double* l, const double* u, double tol, double infval, int n,
+                              double* partials) {
+  __shared__ double lds[kBlock / kWave];
+  double changed = 0.0;
+  GRID_STRIDE(i, n) {
+    int old = iseq[i], t;
+    if ((l[i] < -infval) && (u[i] > infval)) t = -1;
+    else if (u[i] - l[i] < tol) t = 1;
+    else t = 0;
+    iseq[i] = t;
+    if (t != old) changed = 1.0;
+  }
+  double tt = block_max(changed, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tt;
+}
+
+inline int egrid(int n) { return grid_for(n, kBlock, 8192); }
+
+inline hipStream_t S() { return rt().stream; }
+
+}  // namespace
+
+extern "C" {
+
+// ---- creation / views / copies ------------------------------------------------
+OSQPVectorf* OSQPVectorf_malloc(OSQPInt length) {
+  OSQPVectorf* v = (OSQPVectorf*)calloc(1, sizeof(OSQPVectorf));
+  if (!v) return nullptr;
+  v->length = length;
+  v->is_view = 0;
+  v->d = length > 0 ? (double*)dev_alloc(sizeof(double) * (size_t)length) : nullptr;
+  if (length > 0 && !v->d) {
+    free(v);
+    return nullptr;
+  }
+  return v;
+}
+
+OSQPVectori* OSQPVectori_malloc(OSQPInt length) {
+  OSQPVectori* v = (OSQPVectori*)calloc(1, sizeof(OSQPVectori));
+  if (!v) return nullptr;
+  v->length = length;
+  v->d = length > 0 ? (int*)dev_alloc(sizeof(int) * (size_t)length) : nullptr;
+  if (length > 0 && !v->d) {
+    free(v);
+    return nullptr;
+  }
+  return v;
+}
+
+OSQPVectorf* OSQPVectorf_calloc(OSQPInt length) {
+  OSQPVectorf* v = OSQPVectorf_malloc(length);
+  if (v && length > 0) hipLaunchKernelGGL(k_set, dim3(egrid(length)), dim3(kBlock), 0, S(), v->d, 0.0, length);
+  return v;
+}
+
+OSQPVectori* OSQPVectori_calloc(OSQPInt length) {
+  OSQPVectori* v = OSQPVectori_malloc(length);
+  if (v && length > 0) hipLaunchKernelGGL(k_set_int, dim3(egrid(length)), dim3(kBlock), 0, S(), v->d, 0, length);
+  return v;
+}
+
+OSQPVectorf* OSQPVectorf_new(const OSQPFloat* a, OSQPInt length) {
+  OSQPVectorf* v = OSQPVectorf_malloc(length);
+  if (v && length > 0) OSQPVectorf_from_raw(v, a);
+  return v;
+}
+
+OSQPVectori* OSQPVectori_new(const OSQPInt* a, OSQPInt length) {
+  OSQPVectori* v = OSQPVectori_malloc(length);
+  if (v && length > 0) OSQPVectori_from_raw(v, a);
+  return v;
+}
+
+OSQPVectorf* OSQPVectorf_copy_new(const OSQPVectorf* a) {
+  OSQPVectorf* v = OSQPVectorf_malloc(a->length);
+  if (v) OSQPVectorf_copy(v, a);
+  return v;
+}
+
+void OSQPVectorf_free(OSQPVectorf* a) {
+  if (!a) return;
+  if (!a->is_view) dev_free(a->d);
+  free(a);
+}
+
+void OSQPVectori_free(OSQPVectori* a) {
+  if (!a) return;
+  dev_free(a->d);
+  free(a);
+}
+
+OSQPVectorf* OSQPVectorf_view(const OSQPVectorf* a, OSQPInt head, OSQPInt length) {
+  OSQPVectorf* v = (OSQPVectorf*)calloc(1, sizeof(OSQPVectorf));
+  if (!v) return nullptr;
+  v->d = a->d ? a->d + head : nullptr;
+  v->length = length;
+  v->is_view = 1;
+  return v;
+}
+
+void OSQPVectorf_view_free(OSQPVectorf* a) { free(a); }
+
+OSQPInt OSQPVectorf_length(const OSQPVectorf* a) { return a->length; }
+OSQPInt OSQPVectori_length(const OSQPVectori* a) { return a->length; }
+OSQPFloat* OSQPVectorf_device_ptr(const OSQPVectorf* a) { return a->d; }
+
+void OSQPVectorf_copy(OSQPVectorf* b, const OSQPVectorf* a) {
+  if (!a || !b || a->length <= 0) return;
+  HIPCHK(hipMemcpyAsync(b->d, a->d, sizeof(double) * (size_t)a->length, hipMemcpyDeviceToDevice, S()));
+}
+
+void OSQPVectorf_from_raw(OSQPVectorf* b, const OSQPFloat* av) {
+  if (b->length > 0) upload(b->d, av, sizeof(double) * (size_t)b->length);
+}
+
+void OSQPVectori_from_raw(OSQPVectori* b, const OSQPInt* av) {
+  if (b->length > 0) upload(b->d, av, sizeof(int) * (size_t)b->length);
+}
+
+void OSQPVectorf_to_raw(OSQPFloat* bv, const OSQPVectorf* a) {
+  if (a->length > 0) download(bv, a->d, sizeof(double) * (size_t)a->length);
+}
+
+void OSQPVectori_to_raw(OSQPInt* bv, const OSQPVectori* a) {
+  if (a->length > 0) download(bv, a->d, sizeof(int) * (size_t)a->length);
+}
+
+// ---- elementwise math ------------------------------------------------------------
+#define EW_LAUNCH(kern, n, ...) \
+  do { if ((n) > 0) hipLaunchKernelGGL(kern, dim3(egrid(n)), dim3(kBlock), 0, S(), __VA_ARGS__); } while (0)
+
+void OSQPVectorf_set_scalar(OSQPVectorf* a, OSQPFloat sc) { EW_LAUNCH(k_set, a->length, a->d, sc, a->length); }
+
+void OSQPVectorf_set_scalar_conditional(OSQPVectorf* a, const OSQPVectori* test, OSQPFloat sc_if_neg,
+                                        OSQPFloat sc_if_zero, OSQPFloat sc_if_pos) {
+  EW_LAUNCH(k_set_cond, a->length, a->d, test->d, sc_if_neg, sc_if_zero, sc_if_pos, a->length);
+}
+
+void OSQPVectorf_round_to_zero(OSQPVectorf* a, OSQPFloat tol) { EW_LAUNCH(k_round_to_zero, a->length, a->d, tol, a->length); }
+
+void OSQPVectorf_mult_scalar(OSQPVectorf* a, OSQPFloat sc) { EW_LAUNCH(k_mult_sc, a->length, a->d, sc, a->length); }
+
+void OSQPVectorf_plus(OSQPVectorf* x, const OSQPVectorf* a, const OSQPVectorf* b) {
+  EW_LAUNCH(k_plus, a->length, x->d, a->d, b->d, a->length);
+}
+
+void OSQPVectorf_minus(OSQPVectorf* x, const OSQPVectorf* a, const OSQPVectorf* b) {
+  EW_LAUNCH(k_minus, a->length, x->d, a->d, b->d, a->length);
+}
+
+void OSQPVectorf_add_scaled(OSQPVectorf* x, OSQPFloat sca, const OSQPVectorf* a, OSQPFloat scb,
+                            const OSQPVectorf* b) {
+  EW_LAUNCH(k_add_scaled, x->length, x->d, sca, a->d, scb, b->d, x->length);
+}
+
+void OSQPVectorf_add_scaled3(OSQPVectorf* x, OSQPFloat sca, const OSQPVectorf* a, OSQPFloat scb,
+                             const OSQPVectorf* b, OSQPFloat scc, const OSQPVectorf* c) {
+  int incr = (x == a && sca == 1.0) ? 1 : 0;
+  EW_LAUNCH(k_add_scaled3, x->length, x->d, sca, a->d, scb, b->d, scc, c->d, incr, x->length);
+}
+
+void OSQPVectorf_ew_prod(OSQPVectorf* c, const OSQPVectorf* a, const OSQPVectorf* b) {
+  EW_LAUNCH(k_ew_prod, a->length, c->d, a->d, b->d, a->length);
+}
+
+void OSQPVectorf_ew_bound_vec(OSQPVectorf* x, const OSQPVectorf* z, const OSQPVectorf* l, const OSQPVectorf* u) {
+  EW_LAUNCH(k_bound, x->length, x->d, z->d, l->d, u->d, x->length);
+}
+
+void OSQPVectorf_project_polar_reccone(OSQPVectorf* y, const OSQPVectorf* l, const OSQPVectorf* u, OSQPFloat infval) {
+  EW_LAUNCH(k_polar_reccone, y->length, y->d, l->d, u->d, infval, y->length);
+}
+
+void OSQPVectorf_ew_reciprocal(OSQPVectorf* b, const OSQPVectorf* a) { EW_LAUNCH(k_reciprocal, a->length, b->d, a->d, a->length); }
+
+void OSQPVectorf_ew_sqrt(OSQPVectorf* a) { EW_LAUNCH(k_sqrt, a->length, a->d, a->length); }
+
+void OSQPVectorf_ew_max_vec(OSQPVectorf* c, const OSQPVectorf* a, const OSQPVectorf* b) {
+  EW_LAUNCH(k_max_vec, a->length, c->d, a->d, b->d, a->length);
+}
+
+void OSQPVectorf_ew_min_vec(OSQPVectorf* c, const OSQPVectorf* a, const OSQPVectorf* b) {
+  EW_LAUNCH(k_min_vec, a->length, c->d, a->d, b->d, a->length);
+}
+
+void OSQPVectorf_set_scalar_if_lt(OSQPVectorf* x, const OSQPVectorf* z, OSQPFloat testval, OSQPFloat newval) {
+  EW_LAUNCH(k_if_lt, x->length, x->d, z->d, testval, newval, x->length);
+}
+
+void OSQPVectorf_set_scalar_if_gt(OSQPVectorf* x, const OSQPVectorf* z, OSQPFloat testval, OSQPFloat newval) {
+  EW_LAUNCH(k_if_gt, x->length, x->d, z->d, testval, newval, x->length);
+}
+
+// ---- reductions returning host scalars ---------------------------------------------
+OSQPFloat OSQPVectorf_norm_inf(const OSQPVectorf* v) {
+  RedArgs r{v->d, nullptr, nullptr, nullptr, 0, 0};
+  return reduce(R_NORM_INF, r, v->length);
+}
+
+OSQPFloat OSQPVectorf_scaled_norm_inf(const OSQPVectorf* Sv, const OSQPVectorf* v) {
+  RedArgs r{Sv->d, v->d, nullptr, nullptr, 0, 0};
+  return reduce(R_SCALED_NORM_INF, r, v->length);
+}
+
+OSQPFloat OSQPVectorf_norm_inf_diff(const OSQPVectorf* a, const OSQPVectorf* b) {
+  RedArgs r{a->d, b->d, nullptr, nullptr, 0, 0};
+  return reduce(R_NORM_INF_DIFF, r, a->length);
+}
+
+OSQPFloat OSQPVectorf_norm_1(const OSQPVectorf* a) {
+  RedArgs r{a->d, nullptr, nullptr, nullptr, 0, 0};
+  return reduce(R_NORM_1, r, a->length);
+}
+
+OSQPFloat OSQPVectorf_dot_prod(const OSQPVectorf* a, const OSQPVectorf* b) {
+  RedArgs r{a->d, b->d, nullptr, nullptr, 0, 0};
+  return reduce(R_DOT, r, a->length);
+}
+
+OSQPFloat OSQPVectorf_dot_prod_signed(const OSQPVectorf* a, const OSQPVectorf* b, OSQPInt sign) {
+  RedArgs r{a->d, b->d, nullptr, nullptr, 0, 0};
+  if (sign == 1) return reduce(R_DOT_POS, r, a->length);
+  if (sign == -1) return reduce(R_DOT_NEG, r, a->length);
+  return reduce(R_DOT, r, a->length);
+}
+
+OSQPInt OSQPVectorf_all_leq(const OSQPVectorf* l, const OSQPVectorf* u) {
+  RedArgs r{l->d, u->d, nullptr, nullptr, 0, 0};
+  return reduce(R_ANY_GT, r, l->length) > 0.0 ? 0 : 1;
+}
+
+OSQPInt OSQPVectorf_in_reccone(const OSQPVectorf* y, const OSQPVectorf* l, const OSQPVectorf* u, OSQPFloat infval,
+                               OSQPFloat tol) {
+  RedArgs r{y->d, l->d, u->d, nullptr, infval, tol};
+  return reduce(R_RECCONE_BAD, r, y->length) > 0.0 ? 0 : 1;
+}
+
+OSQPInt OSQPVectorf_is_eq(const OSQPVectorf* A, const OSQPVectorf* B, OSQPFloat tol) {
+  if (A->length != B->length) return 0;
+  RedArgs r{A->d, B->d, nullptr, nullptr, tol, 0};
+  return reduce(R_EQ_BAD, r, A->length) > 0.0 ? 0 : 1;
+}
+
+OSQPInt OSQPVectorf_ew_bounds_type(OSQPVectori* iseq, const OSQPVectorf* l, const OSQPVectorf* u, OSQPFloat tol,
+                                   OSQPFloat infval) {
+  int n = iseq->length;
+  if (n <= 0) return 0;
+  Runtime& R = rt();
+  int g = reduce_grid(n);
+  hipLaunchKernelGGL(k_bounds_type, dim3(g), dim3(kBlock), 0, R.stream, iseq->d, l->d, u->d, tol, infval, n,
+                     R.d_partials);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, R.stream, (int)R_NORM_INF, R.d_partials, g,
+                     R.d_scalars);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(R.h_scalars, R.d_scalars, sizeof(double), hipMemcpyDeviceToHost, R.stream));
+  HIPCHK(hipStreamSynchronize(R.stream));
+  return R.h_scalars[0] > 0.0 ? 1 : 0;
+}
+
+}  // extern "C"

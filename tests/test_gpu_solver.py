@@ -1,0 +1,290 @@
+"""Solver-level parity of the HIP engine (osqp_setup/osqp_solve through the C ABI).
+
+The reference's solver tests (tests/basic_qp*, basic_lp, unconstrained,
+no_active_set, primal_dual_infeasibility, primal_infeasibility, non_cvx,
+large_qp, update_matrices) are restated here against:
+  * the closed-form answers and statuses those tests hold, and
+  * the CPU oracle (builtin algorithm + exact LDL^T, FP64) on the same data:
+    same status; objective within 1e-4 relative (TESTS_TOL, osqp_tester.h);
+    the GPU x, y also meet the eps_abs/eps_rel residual test on their own
+    (FP64 KKT certificate computed here with scipy).
+Both arithmetics are covered: precision=64 (plain FP64 backend) and
+precision=32 (FP32-PCG, FP64 residuals); the FP32 closed-form tolerance is the
+reference's float-build TESTS_TOL of 1e-3.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import OracleSolver
+from osqp_amd import OSQP, OSQP_NAN
+from osqp_amd import problems as pr
+
+from .conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = dict(rho=0.1, alpha=1.6, max_iter=2000, scaling=1, eps_abs=1e-5, eps_rel=1e-5)
+PRECISIONS = [64, 32]
+
+
+def tol_for(precision):
+    return 1e-4 if precision == 64 else 1e-3
+
+
+def gpu_solve(prob, precision=64, **kw):
+    s = OSQP(precision=precision)
+    st = dict(FIXTURE)
+    st.update(kw)
+    rc = s.setup(prob["P"], prob["q"], prob["A"], prob["l"], prob["u"], **st)
+    assert rc == 0, f"setup failed rc={rc}"
+    res = s.solve()
+    return s, res
+
+
+def oracle_solve(prob, **kw):
+    o = OracleSolver()
+    st = dict(FIXTURE)
+    st.update(kw)
+    assert o.setup(prob["P"], prob["q"], prob["A"], prob["l"], prob["u"], linsys_solver=1, **st) == 0
+    o.solve()
+    return o
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_basic_qp(precision):
+    p = pr.basic_qp()
+    s, r = gpu_solve(p, precision)
+    T = tol_for(precision)
+    assert r.info.status_val == 1, r.info.status
+    assert np.max(np.abs(r.x - p["x"])) < T
+    assert np.max(np.abs(r.y - p["y"])) < T
+    assert abs(r.info.obj_val - p["obj"]) < T
+    # prim/dual infeasibility certificates are OSQP_NAN for a solved problem
+    assert np.all(r.prim_inf_cert == OSQP_NAN) and np.all(r.dual_inf_cert == OSQP_NAN)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_basic_qp_update_vectors(precision):
+    """test_basic_qp.cpp:535-716: update q, then l/u, re-solve."""
+    p = pr.basic_qp()
+    s, r = gpu_solve(p, precision)
+    o = OracleSolver()
+    o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=1, **FIXTURE)
+    o.solve()
+    for upd in (dict(q=p["q_new"]), dict(l=p["l_new"]), dict(u=p["u_new"]), dict(l=p["l_new"], u=p["u_new"])):
+        assert s.update(**upd) == 0
+        r = s.solve()
+        o.update_data_vec(**upd)
+        o.solve()
+        assert r.info.status_val == o.info.status_val
+        if o.info.status_val == 1:
+            assert abs(r.info.obj_val - o.info.obj_val) <= tol_for(precision) * max(1.0, abs(o.info.obj_val))
+    # l > u is rejected and leaves the data unchanged
+    assert s.update(l=np.array([2.0, 0, 0, 0]), u=np.array([1.0, 1, 1, 1])) == 1
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_basic_qp2_update(precision):
+    p = pr.basic_qp2()
+    s, r = gpu_solve(p, precision, eps_abs=1e-6, eps_rel=1e-6)
+    T = tol_for(precision)
+    assert r.info.status_val == 1
+    assert np.max(np.abs(r.x - p["x"])) / np.max(np.abs(p["x"])) < T
+    assert abs(r.info.obj_val - p["obj"]) / abs(p["obj"]) < T
+    assert s.update(q=p["q_new"], u=p["u_new"]) == 0
+    r = s.solve()
+    assert r.info.status_val == 1
+    assert np.max(np.abs(r.x - p["x_new"])) / np.max(np.abs(p["x_new"])) < T
+    assert abs(r.info.obj_val - p["obj_new"]) / abs(p["obj_new"]) < T
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("name", ["basic_lp", "no_active_set", "unconstrained"])
+def test_closed_forms(name, precision):
+    p = getattr(pr, name)()
+    s, r = gpu_solve(p, precision)
+    assert r.info.status_val == 1, r.info.status
+    assert np.max(np.abs(r.x - p["x"])) < 1e-3
+    if len(p.get("y", [])):
+        assert np.max(np.abs(r.y - p["y"])) < 1e-3
+    assert abs(r.info.obj_val - p["obj"]) < 1e-3
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("case", [1, 2, 3, 4])
+def test_primal_dual_infeasibility(case, precision):
+    p = pr.primal_dual_infeasibility(case)
+    s, r = gpu_solve(p, precision)
+    assert r.info.status == p["status"]
+    if case == 1:
+        assert np.max(np.abs(r.x - p["x"])) < tol_for(precision)
+    else:
+        assert np.all(r.x == OSQP_NAN)
+        cert = r.prim_inf_cert if "primal" in p["status"] else r.dual_inf_cert
+        assert abs(np.max(np.abs(cert)) - 1.0) < 1e-12
+        if "primal" in p["status"]:
+            # certificate: A' y = 0 and u'y+ + l'y- < 0 (auxil.c:460-512)
+            y = cert
+            assert np.max(np.abs(p["A"].T @ y)) < 1e-3
+            lc, uc = np.clip(p["l"], -1e30, 1e30), np.clip(p["u"], -1e30, 1e30)
+            assert uc @ np.maximum(y, 0) + lc @ np.minimum(y, 0) < 0
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_primal_infeasibility_random(precision):
+    p = pr.primal_infeasibility()
+    s, r = gpu_solve(p, precision)
+    assert r.info.status == "primal infeasible"
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_non_cvx_solve(precision):
+    p = pr.non_cvx()
+    s, r = gpu_solve(p, precision, adaptive_rho=0, sigma=5.0)
+    assert r.info.status_val == 9
+    assert r.info.obj_val == OSQP_NAN
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_large_qp_fixture(precision):
+    p = pr.load_fixture(os.path.join(GOLDEN, "large_qp.json"))
+    s, r = gpu_solve(p, precision)
+    assert r.info.status_val == 1
+    assert abs(r.info.obj_val - p["obj"]) / abs(p["obj"]) < tol_for(precision)
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("name", ["hs21", "hs35", "qptest", "cvxqp2_s", "qpcblend", "largeqp", "primalc1"])
+def test_reference_problem_sets(name, precision):
+    """examples/problems/*.c data sets: status and objective equal the oracle's."""
+    p = pr.load_fixture(os.path.join(GOLDEN, f"{name}.json"))
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    o = oracle_solve(p, **st)
+    s, r = gpu_solve(p, precision, **st)
+    assert r.info.status_val == o.info.status_val, (r.info.status, o.info.status)
+    if o.info.status_val == 1:
+        assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * max(1.0, abs(o.info.obj_val))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_qp_vs_oracle(seed, precision):
+    """Random QPs of the random-QP configuration's shape (m = 10 n), eps 1e-4."""
+    p = pr.random_qp(120, 1200, 0.15, 0.15, seed)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    o = oracle_solve(p, **st)
+    s, r = gpu_solve(p, precision, **st)
+    assert r.info.status_val == o.info.status_val == 1
+    assert abs(r.info.obj_val - o.info.obj_val) / abs(o.info.obj_val) < 1e-3
+    c = pr.kkt_certificate(p, r.x, r.y)
+    # the returned point passes the eps test OSQP used to stop (10x margin for
+    # the FP64 recomputation here vs. the scaled/unscaled mix inside OSQP)
+    assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+    assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("N", [10, 30])
+def test_mpc_vs_oracle(N, precision):
+    p = pr.mpc(N, seed=N)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    o = oracle_solve(p, **st)
+    s, r = gpu_solve(p, precision, **st)
+    assert r.info.status_val == o.info.status_val
+    if o.info.status_val == 1:
+        assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * max(1.0, abs(o.info.obj_val))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_portfolio_update_q_warm(precision):
+    p = pr.portfolio(500, seed=3)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    o = oracle_solve(p, **st)
+    s, r = gpu_solve(p, precision, **st)
+    assert r.info.status_val == o.info.status_val == 1
+    assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * max(1.0, abs(o.info.obj_val))
+    rng = np.random.default_rng(9)
+    for _ in range(3):
+        q = pr.portfolio_q(rng.standard_normal(500), p["k"])
+        s.update(q=q)
+        r = s.solve()
+        o.update_data_vec(q=q)
+        o.solve()
+        assert r.info.status_val == o.info.status_val == 1
+        assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * max(1.0, abs(o.info.obj_val))
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_termination_fixed_iterations(precision):
+    p = pr.basic_qp()
+    s, r = gpu_solve(p, precision, max_iter=400, scaling=0, check_termination=0, warm_starting=0)
+    assert r.info.iter == 400
+    assert r.info.status_val == 1
+    assert np.max(np.abs(r.x - p["x"])) < tol_for(precision)
+
+
+def test_update_rho_same_iterations():
+    p = pr.basic_qp()
+    kw = dict(adaptive_rho=0, eps_abs=5e-5, eps_rel=5e-5, check_termination=1)
+    a, ra = gpu_solve(p, 64, rho=0.7, **kw)
+    b = OSQP(precision=64)
+    b.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **dict(FIXTURE, rho=0.1, **kw))
+    assert b.update_rho(0.7) == 0
+    rb = b.solve()
+    assert ra.info.status_val == rb.info.status_val == 1
+    assert ra.info.iter == rb.info.iter
+
+
+def test_warm_start_from_solution():
+    p = pr.basic_qp()
+    s, r = gpu_solve(p, 64, check_termination=1, adaptive_rho=0)
+    s.warm_start(x=p["x"])
+    s.warm_start(y=p["y"])
+    r = s.solve()
+    assert r.info.iter == 1
+    assert r.info.status_val == 1
+
+
+def test_time_limit():
+    p = pr.basic_qp()
+    s, r = gpu_solve(p, 64, rho=20, adaptive_rho=0)
+    assert r.info.status_val == 1
+    s.update_settings(time_limit=1e-7, eps_abs=1e-12, eps_rel=1e-12, max_iter=2_000_000_000, check_termination=0)
+    s.cold_start()
+    r = s.solve()
+    assert r.info.status_val == 8
+
+
+def test_settings_validation():
+    """src/auxil.c:1040-1223: invalid settings are rejected at setup."""
+    p = pr.basic_qp()
+    for bad in (dict(alpha=2.0), dict(rho=0.0), dict(sigma=-1.0), dict(max_iter=0), dict(eps_abs=-1.0),
+                dict(eps_abs=0.0, eps_rel=0.0), dict(cg_tol_fraction=1.0), dict(linsys_solver=1),
+                dict(scaled_termination=2), dict(time_limit=0.0)):
+        s = OSQP()
+        assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **dict(FIXTURE, **bad)) == 2, bad
+    s = OSQP()
+    bad_l = p["u"] + 1.0
+    assert s.setup(p["P"], p["q"], p["A"], bad_l, p["u"], **FIXTURE) == 1
+
+
+def test_update_matrices():
+    """tests/update_matrices: full P/A value update equals a fresh setup."""
+    p = pr.random_qp(40, 200, 0.2, 0.2, 7)
+    q = pr.random_qp(40, 200, 0.2, 0.2, 8)
+    # same sparsity pattern, new values
+    rng = np.random.default_rng(1)
+    P2 = p["P"].copy()
+    P2.data = P2.data * (1 + 0.1 * rng.random(P2.nnz))
+    A2 = p["A"].copy()
+    A2.data = rng.standard_normal(A2.nnz)
+    st = dict(eps_abs=1e-5, eps_rel=1e-5)
+    s, r = gpu_solve(p, 64, **st)
+    assert s.update(Px=P2.data, Ax=A2.data) == 0
+    r = s.solve()
+    f, rf = gpu_solve(dict(p, P=P2, A=A2), 64, **st)
+    assert r.info.status_val == rf.info.status_val == 1
+    assert abs(r.info.obj_val - rf.info.obj_val) / abs(rf.info.obj_val) < 1e-4
+    del q
