@@ -1,0 +1,180 @@
+"""Benchmark: OSQP ADMM solves on one MI355X.
+
+Default workload: the large random QP (n=50,000, m=500,000, A ~5e7 nonzeros,
+P = MM' + 1e-2 I with M of density 1e-4), eps_abs = eps_rel = 1e-4, solved by
+the FP32-PCG engine (FP32 CG with FP64 defect correction; FP64 residuals and
+termination).  A "step" is one full cold-started osqp_solve of that QP with its
+data already resident in HBM (setup excluded).  The plain FP64 HIP backend is
+run on the same QP as the GPU comparison, and the CPU oracle's FP64 PCG path is
+timed on a bounded prefix of the same solve as the CPU baseline.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--small]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QPs solved/s + p50/p99 latency at eps 1e-4; ADMM-iter HBM GB/s vs peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="n=5000, m=50000 instance (quick check)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (one MI355X per job); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from osqp_amd import OSQP, device_name
+    from osqp_amd import problems as pr
+
+    n, m = (5000, 50000) if args.small else (50000, 500000)
+    dens_M, dens_A = (1e-3, 2e-2) if args.small else (1e-4, 2e-3)
+    t0 = time.time()
+    prob = pr.random_qp(n, m, dens_M, dens_A, seed=args.seed)
+    log(f"[bench] generated n={n} m={m} nnz(A)={prob['A'].nnz} nnz(P triu)={prob['P'].nnz} in {time.time() - t0:.1f}s")
+    settings = dict(eps_abs=1e-4, eps_rel=1e-4, verbose=0)
+
+    # ---- FP32-PCG engine (the measured configuration) -------------------------
+    t0 = time.time()
+    eng = OSQP(precision=32)
+    rc = eng.setup(prob["P"], prob["q"], prob["A"], prob["l"], prob["u"], **settings)
+    assert rc == 0, f"setup failed: {rc}"
+    torch.cuda.synchronize()
+    log(f"[bench] FP32-PCG setup {time.time() - t0:.2f}s on {device_name()}")
+    for w in range(args.warmup):
+        eng.cold_start()
+        r = eng.solve()
+        log(f"[bench] warmup {w}: status={r.info.status} iter={r.info.iter} t={r.info.solve_time:.3f}s")
+    lat, iters, statuses = [], [], []
+    st0 = eng.stats()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        eng.cold_start()
+        r = eng.solve()
+        lat.append(time.perf_counter() - ts)
+        iters.append(r.info.iter)
+        statuses.append(r.info.status)
+        log(f"[bench] step {k}: status={r.info.status} iter={r.info.iter} obj={r.info.obj_val:.8e} "
+            f"lat={lat[-1] * 1e3:.1f}ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    st1 = eng.stats()
+    x32, y32, obj32 = r.x, r.y, r.info.obj_val
+    cert32 = pr.kkt_certificate(prob, x32, y32)
+    qps = args.steps / elapsed
+    admm_iters = int(np.sum(iters))
+    cg_iters = st1.pcg_iters - st0.pcg_iters
+    ms_per_admm = elapsed * 1e3 / max(admm_iters, 1)
+
+    # ---- dominant kernels: CG operator, timed with HIP events ------------------
+    kt = eng.time_kernels(reps=20)
+    dom = "k_Kv" if kt["ms_kv"] >= kt["ms_av"] else "k_Av_rho"
+    dom_ms = max(kt["ms_kv"], kt["ms_av"])
+    dom_bytes = kt["bytes_kv"] if dom == "k_Kv" else kt["bytes_av"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    log(f"[bench] kernels: Av {kt['ms_av']:.4f}ms {kt['bytes_av'] / kt['ms_av'] / 1e6:.0f} GB/s; "
+        f"Kv {kt['ms_kv']:.4f}ms {kt['bytes_kv'] / kt['ms_kv'] / 1e6:.0f} GB/s")
+    eng.close()
+
+    # ---- plain FP64 backend on the same QP (GPU comparison) -------------------
+    fp64 = None
+    if not args.no_fp64:
+        e64 = OSQP(precision=64)
+        assert e64.setup(prob["P"], prob["q"], prob["A"], prob["l"], prob["u"], **settings) == 0
+        ts = time.perf_counter()
+        r64 = e64.solve()
+        t64 = time.perf_counter() - ts
+        fp64 = dict(qps=1.0 / t64, status=r64.info.status, iter=r64.info.iter, obj=r64.info.obj_val,
+                    err_x_inf=float(np.max(np.abs(x32 - r64.x))),
+                    err_y_inf=float(np.max(np.abs(y32 - r64.y))),
+                    err_obj_rel=float(abs(obj32 - r64.info.obj_val) / max(1.0, abs(r64.info.obj_val))))
+        log(f"[bench] FP64 plain: status={r64.info.status} iter={r64.info.iter} t={t64:.3f}s "
+            f"|dx|={fp64['err_x_inf']:.2e} |dobj|={fp64['err_obj_rel']:.2e}")
+        e64.close()
+
+    # ---- CPU baseline: oracle FP64 PCG, bounded prefix of the same solve -------
+    cpu = None
+    if not args.no_cpu:
+        from oracle import OracleSolver
+        k_cpu = 3
+        o = OracleSolver()
+        t0 = time.time()
+        assert o.setup(prob["P"], prob["q"], prob["A"], prob["l"], prob["u"], linsys_solver=2,
+                       max_iter=k_cpu, check_termination=0, **settings) == 0
+        log(f"[bench] oracle setup {time.time() - t0:.1f}s")
+        ts = time.perf_counter()
+        o.solve()
+        tc = time.perf_counter() - ts
+        it_per_s = k_cpu / tc
+        mean_iters = admm_iters / args.steps
+        cpu = dict(value=it_per_s / mean_iters, unit="QPs/s", cores=1, kind="port",
+                   sample=f"oracle FP64 PCG (oracle/osqp_oracle.c), first {k_cpu} ADMM iterations of the same "
+                          f"n={n} m={m} QP ({tc:.1f}s, single thread); QPs/s = ADMM iter/s / "
+                          f"{mean_iters:.0f} iterations per GPU solve")
+        log(f"[bench] cpu oracle: {k_cpu} ADMM iterations in {tc:.1f}s")
+
+    out = {
+        "metric": METRIC,
+        "value": qps,
+        "unit": "QPs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 CG values / fp64 residuals",
+        "data": "synthetic (seeded numpy generator of the configuration)",
+        "config": {"workload": f"large random QP n={n} m={m} nnz(A)={prob['A'].nnz} P=MM'+1e-2I, eps 1e-4, FP32-PCG",
+                   "n": n, "m": m, "nnz_A": int(prob["A"].nnz), "nnz_P_triu": int(prob["P"].nnz),
+                   "eps_abs": 1e-4, "eps_rel": 1e-4, "seed": args.seed},
+        "latency_ms": {"p50": percentile(lat, 50) * 1e3, "p99": percentile(lat, 99) * 1e3},
+        "status": statuses,
+        "admm_iters_per_solve": admm_iters / args.steps,
+        "cg_iters_per_admm_iter": cg_iters / max(admm_iters, 1),
+        "ms_per_admm_iter": ms_per_admm,
+        "kkt_certificate": {"prim": cert32["prim"], "dual": cert32["dual"], "obj": cert32["obj"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "ms_per_launch": dom_ms, "bytes_per_launch": dom_bytes,
+                     "other_kernel": {"name": "k_Av_rho" if dom == "k_Kv" else "k_Kv",
+                                      "ms_per_launch": min(kt["ms_kv"], kt["ms_av"]),
+                                      "bytes_per_launch": kt["bytes_av"] if dom == "k_Kv" else kt["bytes_kv"]}},
+        "gpu_fp64_plain": fp64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

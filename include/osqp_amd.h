@@ -283,6 +283,11 @@ OSQPInt osqp_amd_get_pcg_precision(void);
 OSQPInt osqp_amd_solver_stats(OSQPSolver* solver, OSQPInt* pcg_iters_total,
                               OSQPInt* linsys_solves, double* gpu_ms);
 
+/* Times the CG operator kernels of this solver's linear system (see
+ * osqp_amd_linsys_time_kernels in osqp_amd_algebra.h). */
+OSQPInt osqp_amd_time_pcg_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_av, double* ms_kv,
+                                  double* bytes_av, double* bytes_kv);
+
 #ifdef __cplusplus
 }
 #endif

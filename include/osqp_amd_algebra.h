@@ -151,6 +151,11 @@ OSQPFloat* OSQPVectorf_device_ptr(const OSQPVectorf* a);
 void       osqp_amd_algebra_sync(void);
 /* Last-solve counters of a PCG linsys: total CG iterations and solve calls. */
 void       osqp_amd_linsys_counters(LinSysSolver* s, OSQPInt* cg_iters, OSQPInt* solves);
+/* Mean duration (ms) of one launch of each CG operator kernel on this
+ * linsys' data (w = rho o A p; K p = sigma p + P p + A' w) over `reps`
+ * launches timed with HIP events, and the algorithmic bytes of one launch. */
+OSQPInt    osqp_amd_linsys_time_kernels(LinSysSolver* s, OSQPInt reps, double* ms_av, double* ms_kv,
+                                        double* bytes_av, double* bytes_kv);
 /* Last HIP error seen by the backend (0 if none); cleared on read. */
 OSQPInt    osqp_amd_last_error(void);
 

@@ -666,3 +666,9 @@ OSQPInt osqp_amd_solver_stats(OSQPSolver* solver, OSQPInt* pcg_iters_total, OSQP
   if (gpu_ms) *gpu_ms = solver->info->solve_time * 1e3;
   return 0;
 }
+
+OSQPInt osqp_amd_time_pcg_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_av, double* ms_kv, double* bytes_av,
+                                  double* bytes_kv) {
+  if (!solver || !solver->work || !solver->work->linsys_solver) return 1;
+  return osqp_amd_linsys_time_kernels(solver->work->linsys_solver, reps, ms_av, ms_kv, bytes_av, bytes_kv);
+}

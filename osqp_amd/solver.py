@@ -62,6 +62,8 @@ def load_library():
         lib.osqp_amd_solver_stats.argtypes = [C.POINTER(T.OSQPSolver), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                               C.POINTER(C.c_double)]
         lib.osqp_amd_solver_stats.restype = C.c_int
+        lib.osqp_amd_time_pcg_kernels.argtypes = [C.POINTER(T.OSQPSolver), C.c_int] + [C.POINTER(C.c_double)] * 4
+        lib.osqp_amd_time_pcg_kernels.restype = C.c_int
         lib.osqp_algebra_device_name.argtypes = [C.c_char_p, C.c_int]
         lib.osqp_amd_last_error.restype = C.c_int
         _lib = lib
@@ -185,6 +187,14 @@ class OSQP:
         it, sv, ms = C.c_int(), C.c_int(), C.c_double()
         self.lib.osqp_amd_solver_stats(self._solver, C.byref(it), C.byref(sv), C.byref(ms))
         return SimpleNamespace(pcg_iters=it.value, linsys_solves=sv.value, solve_ms=ms.value)
+
+    def time_kernels(self, reps=20):
+        """Mean ms and algorithmic bytes of one launch of each CG operator kernel."""
+        a, k, ba, bk = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        rc = self.lib.osqp_amd_time_pcg_kernels(self._solver, reps, C.byref(a), C.byref(k), C.byref(ba), C.byref(bk))
+        if rc:
+            raise RuntimeError("osqp_amd_time_pcg_kernels failed")
+        return dict(ms_av=a.value, ms_kv=k.value, bytes_av=ba.value, bytes_kv=bk.value)
 
     def close(self):
         if self._solver:
