@@ -15,7 +15,7 @@
 namespace osqp_hip {
 
 constexpr int kBlock = 256;          // threads per block for streaming kernels
-constexpr int kMaxReduceBlocks = 1024;  // cap on per-block partials of a reduction
+constexpr int kMaxReduceBlocks = 2048;  // cap on per-block partials of a reduction
 constexpr int kWave = 64;            // CDNA wavefront width
 
 struct Runtime {
@@ -128,15 +128,64 @@ __device__ inline T group_max(T v) {
   return v;
 }
 
-// Chooses lanes-per-row for a CSR row kernel from the mean row length.
+// Unroll depth of the CSR row loops: each lane keeps kUnroll independent
+// (index, value) loads in flight before it gathers.
+constexpr int kUnroll = 4;
+
+// Chooses lanes-per-row G for a CSR row kernel from the mean row length, so
+// that one unrolled trip (G * kUnroll elements) covers a typical row.
 inline int lanes_for_rows(long long nnz, long long rows) {
   double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
-  if (avg <= 4) return 4;
-  if (avg <= 12) return 8;
-  if (avg <= 40) return 16;
-  if (avg <= 96) return 32;
+  double per_lane_trip = avg / kUnroll;
+  if (per_lane_trip <= 4) return 4;
+  if (per_lane_trip <= 8) return 8;
+  if (per_lane_trip <= 16) return 16;
+  if (per_lane_trip <= 32) return 32;
   return 64;
 }
+
+// Streaming loads of CSR index/value arrays: each element is read once per
+// SpMV, so they bypass cache retention (keeping the gathered vector in L2).
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// Partial dot product of one CSR row [beg, end) for one lane of a G-lane
+// group: kUnroll predicated loads of (column, value) are issued before the
+// gathers so that every lane has kUnroll memory requests in flight.  Lanes
+// past the row end load nothing and gather element 0 with weight 0.
+template <int G, typename TV, typename TX, typename Gather>
+__device__ __forceinline__ TX csr_row_dot(const int* __restrict__ ci, const TV* __restrict__ val, int beg, int end,
+                                         int lane, Gather gather) {
+  TX acc = 0;
+  for (int k = beg + lane; k < end; k += kUnroll * G) {
+    int c[kUnroll];
+    TV v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int idx = k + u * G;
+      const bool ok = idx < end;
+      c[u] = ok ? ld_stream(ci + idx) : 0;
+      v[u] = ok ? ld_stream(val + idx) : (TV)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) acc += (TX)v[u] * gather(c[u]);
+  }
+  return acc;
+}
+
+// Grid of a G-lanes-per-row kernel: one group per row up to `cap` blocks,
+// then the kernel strides over rows.
+inline int rows_grid(long long rows, int G, int cap) {
+  const int rpb = kBlock / G;
+  long long g = (rows + rpb - 1) / rpb;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+constexpr int kRowGridCap = 2048;  // 8192 wavefronts: every SIMD of 256 CUs 8-deep
 
 }  // namespace osqp_hip
 
@@ -178,4 +227,7 @@ struct OSQPMatrix_ {
   int fp32_dirty;
   int lanes, tlanes; // lanes per row for A and A' kernels
   long long version; // bumped on every value change
+  // row-block partitions (nonzero-balanced) of the CSR copy and of A'
+  int rb_n, trb_n;
+  int *rb_rows, *trb_rows;
 };

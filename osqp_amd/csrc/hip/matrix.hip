@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "hip_internal.h"
+#include "spmv_rb.h"
 
 using namespace osqp_hip;
 
@@ -64,23 +65,19 @@ __global__ void k_rmult_sym(double* val, const int* rp, const int* ci, const dou
   }
 }
 
-// y = alpha * M x + beta * y, G lanes per row
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_spmv(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                const double* __restrict__ val, const double* __restrict__ x,
-                                                double* __restrict__ y, double alpha, double beta) {
-  const int lane = threadIdx.x & (G - 1);
-  const int rows_per_block = kBlock / G;
-  for (int row = blockIdx.x * rows_per_block + threadIdx.x / G; row < rows; row += gridDim.x * rows_per_block) {
-    double acc = 0.0;
-    const int e = rp[row + 1];
-    for (int k = rp[row] + lane; k < e; k += G) acc += val[k] * x[ci[k]];
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      double v = alpha * acc;
-      if (beta != 0.0) v += beta * y[row];
-      y[row] = v;
-    }
+// y = alpha * M x + beta * y over nonzero-balanced row blocks
+__global__ __launch_bounds__(kBlock) void k_rb_spmv(const int* __restrict__ rows, int nb, const int* __restrict__ rp,
+                                                   const int* __restrict__ ci, const double* __restrict__ val,
+                                                   const double* __restrict__ x, double* __restrict__ y, double alpha,
+                                                   double beta) {
+  __shared__ double prod[kChunk];
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    rb_spmv_block<double, double>(rows[b], rows[b + 1], rp, ci, val, prod, [&](int c) { return x[c]; },
+                                  [&](int row, double sum) {
+                                    double v = alpha * sum;
+                                    if (beta != 0.0) v += beta * y[row];
+                                    y[row] = v;
+                                  });
   }
 }
 
@@ -143,7 +140,7 @@ void launch_rows(int G, int rows, Args... args) {
   if (rows <= 0) return;
   const int rpb = kBlock / G;
   int grid = (int)((rows + rpb - 1) / rpb);
-  if (grid > 65535 * 4) grid = 65535 * 4;
+  if (grid > kRowGridCap) grid = kRowGridCap;
   switch (G) {
     case 4: hipLaunchKernelGGL(K<4>::fn, dim3(grid), dim3(kBlock), 0, S(), args...); break;
     case 8: hipLaunchKernelGGL(K<8>::fn, dim3(grid), dim3(kBlock), 0, S(), args...); break;
@@ -154,7 +151,6 @@ void launch_rows(int G, int rows, Args... args) {
   LAUNCHCHK();
 }
 
-template <int G> struct SpmvK { static constexpr auto fn = k_spmv<G>; };
 template <int G> struct RowMaxK { static constexpr auto fn = k_row_max<G>; };
 template <int G> struct RowSqK { static constexpr auto fn = k_row_sq_weighted<G>; };
 
@@ -165,20 +161,49 @@ T* dev_copy(const T* h, size_t n) {
   return d;
 }
 
-void spmv(const int* rp, const int* ci, const double* val, int rows, long long nnz, int G, const double* x,
-          double* y, double alpha, double beta) {
+void spmv(const int* rp, const int* ci, const double* val, int rows, long long nnz, const int* rb_rows, int rb_n,
+          const double* x, double* y, double alpha, double beta) {
   if (rows <= 0) return;
   if (nnz == 0 || alpha == 0.0) {
     hipLaunchKernelGGL(k_scale_y, dim3(egrid(rows)), dim3(kBlock), 0, S(), y, beta, rows);
     return;
   }
-  launch_rows<SpmvK>(G, rows, rows, rp, ci, val, x, y, alpha, beta);
+  int grid = rb_n < kRowGridCap ? rb_n : kRowGridCap;
+  hipLaunchKernelGGL(k_rb_spmv, dim3(grid), dim3(kBlock), 0, S(), rb_rows, rb_n, rp, ci, val, x, y, alpha, beta);
+  LAUNCHCHK();
 }
 
 }  // namespace
 
 // ---- host-side structure builders (also used by the batched engine) ----------
 namespace osqp_hip {
+
+int build_row_blocks(const int* rp, int rows, int* out) {
+  int nb = 0, start = 0;
+  out[0] = 0;
+  for (int r = 0; r < rows; ++r) {
+    const int len = rp[r + 1] - rp[r];
+    if (len > kChunk) {
+      if (start < r) out[++nb] = r;      // close the open block
+      out[++nb] = r + 1;                 // the long row alone
+      start = r + 1;
+    } else if (rp[r + 1] - rp[start] > kChunk || r - start >= kChunk) {
+      out[++nb] = r;                     // block [start, r)
+      start = r;
+    }
+  }
+  if (start < rows) out[++nb] = rows;
+  return nb;
+}
+
+static int* upload_row_blocks(const int* rp_host, int rows, int* nb_out) {
+  std::vector<int> b((size_t)rows + 2);
+  int nb = build_row_blocks(rp_host, rows, b.data());
+  *nb_out = nb;
+  int* d = (int*)dev_alloc(sizeof(int) * (size_t)(nb + 1));
+  upload(d, b.data(), sizeof(int) * (size_t)(nb + 1));
+  return d;
+}
 
 // CSR of an m x n CSC matrix by a stable counting sort; csc2csr[k] = CSR slot
 void build_csr_from_csc(int m, int n, const int* p, const int* i, std::vector<int>& rp, std::vector<int>& ci,
@@ -266,6 +291,8 @@ OSQPMatrix* OSQPMatrix_new_from_csc(const OSQPCscMatrix* A, OSQPInt is_triu) {
     M->diag_pos = nullptr;
     M->lanes = lanes_for_rows(nnz, m);
     M->tlanes = lanes_for_rows(nnz, n);
+    M->rb_rows = upload_row_blocks(rp.data(), m, &M->rb_n);
+    M->trb_rows = upload_row_blocks(A->p, n, &M->trb_n);
   } else {
     build_full_from_triu(n, A->p, A->i, rp, ci, pos, mirror, diag);
     const int full = rp[(size_t)n];
@@ -282,6 +309,9 @@ OSQPMatrix* OSQPMatrix_new_from_csc(const OSQPCscMatrix* A, OSQPInt is_triu) {
     M->csc2mirror = dev_copy(mirror.data(), mirror.size());
     M->diag_pos = dev_copy(diag.data(), diag.size());
     M->lanes = M->tlanes = lanes_for_rows(full, n);
+    M->rb_rows = upload_row_blocks(rp.data(), n, &M->rb_n);
+    M->trb_rows = nullptr;
+    M->trb_n = 0;
   }
   M->fp32_dirty = 1;
   M->version = 1;
@@ -294,6 +324,7 @@ void OSQPMatrix_free(OSQPMatrix* M) {
   dev_free(M->trp); dev_free(M->tci); dev_free(M->tval);
   dev_free(M->csc2csr); dev_free(M->csc2mirror); dev_free(M->diag_pos);
   dev_free(M->val32); dev_free(M->tval32);
+  dev_free(M->rb_rows); dev_free(M->trb_rows);
   free(M);
 }
 
@@ -361,14 +392,14 @@ void OSQPMatrix_rmult_diag(OSQPMatrix* A, const OSQPVectorf* R) {
 
 void OSQPMatrix_Axpy(const OSQPMatrix* A, const OSQPVectorf* x, OSQPVectorf* y, OSQPFloat alpha, OSQPFloat beta) {
   int rows = A->is_triu ? A->n : A->m;
-  spmv(A->rp, A->ci, A->val, rows, A->nnz, A->lanes, x ? x->d : nullptr, y->d, alpha, beta);
+  spmv(A->rp, A->ci, A->val, rows, A->nnz, A->rb_rows, A->rb_n, x ? x->d : nullptr, y->d, alpha, beta);
 }
 
 void OSQPMatrix_Atxpy(const OSQPMatrix* A, const OSQPVectorf* x, OSQPVectorf* y, OSQPFloat alpha, OSQPFloat beta) {
   if (A->is_triu) {
-    spmv(A->rp, A->ci, A->val, A->n, A->nnz, A->lanes, x->d, y->d, alpha, beta);
+    spmv(A->rp, A->ci, A->val, A->n, A->nnz, A->rb_rows, A->rb_n, x->d, y->d, alpha, beta);
   } else {
-    spmv(A->trp, A->tci, A->tval, A->n, A->nnz_in, A->tlanes, x ? x->d : nullptr, y->d, alpha, beta);
+    spmv(A->trp, A->tci, A->tval, A->n, A->nnz_in, A->trb_rows, A->trb_n, x ? x->d : nullptr, y->d, alpha, beta);
   }
 }
 

@@ -6,30 +6,42 @@
 // cuda_pcg.cu:50-285; CPU statement of K in algebra/_common/reduced_kkt.c.
 //
 // MI355X design:
-//   * No host synchronisation inside a solve.  The loop state (r'y, ||r||,
-//     alpha, beta, eps, iteration count, active flag) lives in a device
-//     struct; one-block "scalar" kernels finish each reduction; every CG
-//     kernel returns at once when `active` is cleared, so the host queues
-//     cg_max_iter iterations and never waits.  The reference host-syncs every
+//   * Device-resident control.  The loop state (r'y, ||r||, alpha, beta, eps,
+//     iteration count, active flag) lives in a device struct; one-block
+//     "scalar" kernels finish each reduction; every CG kernel returns at once
+//     when `active` is cleared.  The host never waits on the GPU inside a
+//     solve: it queues CG iterations ahead and reads a per-iteration flag the
+//     loop-head kernel writes to host-coherent memory, so it stops queueing a
+//     couple of iterations after convergence.  The reference host-syncs every
 //     CG iteration (cuda_pcg.cu:188,199).
 //   * The tolerance schedule (cuda_pcg_interface.cu:32-64) runs on the device,
 //     including the zero-iteration counter that halves the reduction factor.
-//   * Two arithmetics:
-//       FP64 ("plain") — the reference algorithm in double throughout.
-//       FP32 ("mixed") — defect correction: the iterate x, the right-hand side
-//         and the residual r = K x - rhs are FP64 (FP32-stored matrix values,
-//         FP64 accumulation); the CG itself runs in FP32 on the correction
-//         system K d = -r / ||r||, with FP32 lane accumulation and
-//         Kahan-compensated dot products; x += ||r|| d in FP64.  Every CG
-//         iteration streams 8 bytes per nonzero (FP32 value + int32 index)
-//         and the solution is not limited by FP32 round-off.
-//   * CSR row kernels with G lanes per row (G from the mean row length);
-//     K p fuses sigma p + P p + A' w into one pass over the n rows.
+//   * Defect-correction form for both arithmetics: the iterate x (warm start),
+//     the right-hand side and the residual r = K x - rhs are FP64 with the
+//     FP64 matrix values (the exact K); CG runs on the correction system
+//     K d = -r/||r|| and x += ||r|| d.  In exact arithmetic this is CG started
+//     from x, i.e. the reference iteration.
+//       FP64 ("plain"): CG values and vectors in double.
+//       FP32 ("mixed"): CG on FP32-stored values (8 bytes per nonzero with the
+//         int32 index), FP32 vectors and lane accumulation, Kahan-compensated
+//         dot products.  Because the residual uses the exact K, neither the
+//         FP32 round-off nor the FP32 rounding of the matrix limits the
+//         accuracy of x: it is classic mixed-precision iterative refinement.
+//   * Traffic per ADMM iteration: after the first solve the residual reuses
+//     A x from the previous solve's z = A x (r = sigma x + P x - b1 +
+//     A' (rho o (A x - b2))), so one solve costs (CG iterations + 1) passes
+//     over A and A' instead of (CG iterations + 2).
+//   * The operator is two row-block SpMVs (spmv_rb.h): w = rho o (A p) over
+//     A, and K p = sigma p + [P A'] [p; w] over one combined CSR of P and A'
+//     built at setup, so each CG iteration is two streaming passes.
 #include <math.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "hip_internal.h"
+#include "spmv_rb.h"
 
 using namespace osqp_hip;
 
@@ -43,7 +55,7 @@ inline hipStream_t S() { return rt().stream; }
 struct PcgState {
   double eps;          // tolerance of the current solve (absolute, on K x = rhs)
   double eps_cmp;      // tolerance compared with the CG residual (eps / scale)
-  double scale;        // defect-correction scale ||r0||_inf (1 for FP64)
+  double scale;        // defect-correction scale ||r0||_inf
   double eps_prev;     // tolerance schedule state
   double red_factor;   // tolerance schedule state
   double rhs_norm;
@@ -80,92 +92,130 @@ struct Acc<float> {
 
 __device__ inline double rho_at(const double* rho, double rho_sc, int i) { return rho ? rho[i] : rho_sc; }
 
-// ---- w = rho o (A v), m rows.  TV: stored value type, TX: vector type -------
-template <typename TV, typename TX, int G>
-__global__ __launch_bounds__(kBlock) void k_Av_rho(const PcgState* st, int m, const int* __restrict__ rp,
-                                                   const int* __restrict__ ci, const TV* __restrict__ val,
-                                                   const TX* __restrict__ v, const double* __restrict__ rho,
-                                                   double rho_sc, TX* __restrict__ w) {
+// ---- row-block (nonzero-balanced) versions of the operator kernels -----------
+// w = rho o (A v)
+template <typename TV, typename TX>
+__global__ __launch_bounds__(kBlock) void k_rb_Av_rho(const PcgState* st, const int* __restrict__ rows, int nb,
+                                                      const int* __restrict__ rp, const int* __restrict__ ci,
+                                                      const TV* __restrict__ val, const TX* __restrict__ v,
+                                                      const double* __restrict__ rho, double rho_sc,
+                                                      TX* __restrict__ w) {
   if (!st->active) return;
-  const int lane = threadIdx.x & (G - 1);
-  const int rpb = kBlock / G;
-  for (int row = blockIdx.x * rpb + threadIdx.x / G; row < m; row += gridDim.x * rpb) {
-    TX acc = 0;
-    const int e = rp[row + 1];
-    for (int k = rp[row] + lane; k < e; k += G) acc += (TX)val[k] * v[ci[k]];
-    acc = group_sum<G>(acc);
-    if (lane == 0) w[row] = (TX)rho_at(rho, rho_sc, row) * acc;
-  }
+  __shared__ TX prod[kChunk];
+  for (int b = blockIdx.x; b < nb; b += gridDim.x)
+    rb_spmv_block<TV, TX>(rows[b], rows[b + 1], rp, ci, val, prod, [&](int c) { return v[c]; },
+                          [&](int row, TX sum) { w[row] = (TX)rho_at(rho, rho_sc, row) * sum; });
 }
 
-// ---- Kv = sigma v + P v + A' w, n rows; partial sums of v'Kv ----------------
-template <typename TV, typename TX, int G>
-__global__ __launch_bounds__(kBlock) void k_Kv(const PcgState* st, int n, double sigma, const int* __restrict__ prp,
-                                               const int* __restrict__ pci, const TV* __restrict__ pval,
-                                               const int* __restrict__ trp, const int* __restrict__ tci,
-                                               const TV* __restrict__ tval, const TX* __restrict__ v,
-                                               const TX* __restrict__ w, TX* __restrict__ Kv, double* partials) {
+// Kv = sigma v + [P A'] [v; w] (- sub) over the combined K-operator CSR.
+// sub == nullptr: partials of v'Kv; otherwise partials of max|Kv - sub|.
+template <typename TV, typename TX>
+__global__ __launch_bounds__(kBlock) void k_rb_Kv(const PcgState* st, const int* __restrict__ rows, int nb,
+                                                  const int* __restrict__ krp, const int* __restrict__ kci,
+                                                  const TV* __restrict__ kval, int n, double sigma,
+                                                  const TX* __restrict__ v, const TX* __restrict__ w,
+                                                  const double* __restrict__ sub, TX* __restrict__ Kv,
+                                                  double* partials) {
   if (!st->active) return;
+  __shared__ TX prod[kChunk];
   __shared__ double lds[kBlock / kWave];
-  const int lane = threadIdx.x & (G - 1);
-  const int rpb = kBlock / G;
   const TX sg = (TX)sigma;
   Acc<TX> dot;
-  for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
-    TX acc = 0;
-    int e = prp[row + 1];
-    for (int k = prp[row] + lane; k < e; k += G) acc += (TX)pval[k] * v[pci[k]];
-    if (trp) {
-      e = trp[row + 1];
-      for (int k = trp[row] + lane; k < e; k += G) acc += (TX)tval[k] * w[tci[k]];
-    }
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      TX vi = v[row];
-      TX kv = sg * vi + acc;
-      Kv[row] = kv;
-      dot.add(vi * kv);
-    }
-  }
-  if (partials) {
-    double t = block_sum(dot.value(), lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = t;
-  }
+  double mx = 0.0;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x)
+    rb_spmv_block<TV, TX>(rows[b], rows[b + 1], krp, kci, kval, prod,
+                          [&](int c) { return c < n ? v[c] : w[c - n]; },
+                          [&](int row, TX sum) {
+                            TX vi = v[row];
+                            TX kv = sg * vi + sum;
+                            if (sub) {
+                              kv -= (TX)sub[row];
+                              double a = fabs((double)kv);
+                              mx = a > mx ? a : mx;
+                            } else {
+                              dot.add(vi * kv);
+                            }
+                            Kv[row] = kv;
+                          });
+  double t = sub ? block_max(mx, lds) : block_sum(dot.value(), lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// ---- rhs = b1 + A' (rho o b2), n rows, FP64 out; partial max|rhs| ------------
-template <typename TV, int G>
-__global__ __launch_bounds__(kBlock) void k_rhs(int n, const double* __restrict__ b, const int* __restrict__ trp,
-                                                const int* __restrict__ tci, const TV* __restrict__ tval,
-                                                const double* __restrict__ rho, double rho_sc,
-                                                double* __restrict__ rhs, double* partials) {
+// rhs = b1 + A' (rho o b2) over the K-operator CSR (P columns weigh zero)
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_rb_rhs(const int* __restrict__ rows, int nb, const int* __restrict__ krp,
+                                                   const int* __restrict__ kci, const TV* __restrict__ kval, int n,
+                                                   const double* __restrict__ b, const double* __restrict__ rho,
+                                                   double rho_sc, double* __restrict__ rhs, double* partials) {
+  __shared__ double prod[kChunk];
   __shared__ double lds[kBlock / kWave];
-  const int lane = threadIdx.x & (G - 1);
-  const int rpb = kBlock / G;
   const double* b2 = b + n;
   double mx = 0.0;
-  for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
-    double acc = 0.0;
-    if (trp) {
-      const int e = trp[row + 1];
-      for (int k = trp[row] + lane; k < e; k += G) {
-        int c = tci[k];
-        acc += (double)tval[k] * (rho_at(rho, rho_sc, c) * b2[c]);
-      }
-    }
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      double r = b[row] + acc;
-      rhs[row] = r;
-      double a = fabs(r);
-      mx = a > mx ? a : mx;
-    }
-  }
+  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x)
+    rb_spmv_block<TV, double>(rows[blk], rows[blk + 1], krp, kci, kval, prod,
+                              [&](int c) { return c < n ? 0.0 : rho_at(rho, rho_sc, c - n) * b2[c - n]; },
+                              [&](int row, double sum) {
+                                double r = b[row] + sum;
+                                rhs[row] = r;
+                                double a = fabs(r);
+                                mx = a > mx ? a : mx;
+                              });
   double t = block_max(mx, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// b[n:n+m] = A x (FP64 accumulation); ax = A x; polishing: rho_pol (A x - b2)
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_rb_finish_z(const int* __restrict__ rows, int nb,
+                                                        const int* __restrict__ rp, const int* __restrict__ ci,
+                                                        const TV* __restrict__ val, const double* __restrict__ x,
+                                                        double* __restrict__ b, double* __restrict__ ax, int n,
+                                                        int polishing, double rho_pol) {
+  __shared__ double prod[kChunk];
+  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x)
+    rb_spmv_block<TV, double>(rows[blk], rows[blk + 1], rp, ci, val, prod, [&](int c) { return x[c]; },
+                              [&](int row, double sum) {
+                                if (ax) ax[row] = sum;
+                                b[n + row] = polishing ? (sum - b[n + row]) * rho_pol : sum;
+                              });
+}
+
+// combined K-operator CSR [P A'] (n rows; A' columns shifted by n)
+__global__ void k_krp(int n, const int* prp, const int* trp, int* krp) {
+  GRID_STRIDE(r, n + 1) krp[r] = prp[r] + (trp ? trp[r] : 0);
+}
+
+template <typename TV>
+__global__ void k_kfill(int n, const int* prp, const int* pci, const double* pval, const int* trp, const int* tci,
+                        const double* tval, const int* krp, int* kci, TV* kval, int cols_too) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (long long r = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n;
+       r += (long long)gridDim.x * blockDim.x / kWave) {
+    const int o = krp[r], p0 = prp[r], p1 = prp[r + 1];
+    for (int k = p0 + lane; k < p1; k += kWave) {
+      kval[o + k - p0] = (TV)pval[k];
+      if (cols_too) kci[o + k - p0] = pci[k];
+    }
+    if (trp) {
+      const int o2 = o + (p1 - p0), t0 = trp[r], t1 = trp[r + 1];
+      for (int k = t0 + lane; k < t1; k += kWave) {
+        kval[o2 + k - t0] = (TV)tval[k];
+        if (cols_too) kci[o2 + k - t0] = tci[k] + n;
+      }
+    }
+  }
+}
+
+// w = rho o (Ax - b2) from the cached A x of the previous solve (m rows)
+__global__ void k_w_from_ax(const PcgState* st, int m, const double* ax, const double* b2, const double* rho,
+                            double rho_sc, double* w) {
+  if (!st->active) return;
+  GRID_STRIDE(i, m) w[i] = rho_at(rho, rho_sc, (int)i) * (ax[i] - b2[i]);
+}
+
 // ---- tolerance (cuda_pcg_interface.cu:32-64) + loop reset, one block ---------
+// rhs_norm is needed only for the first ADMM iteration and in polishing; the
+// cached path passes nparts = 0.
 __global__ void k_tolerance(PcgState* st, const double* partials, int nparts, int admm_iter, double spr, double sdr,
                             int polishing, double tol_fraction, int reduction_threshold) {
   __shared__ double lds[kBlock / kWave];
@@ -204,45 +254,7 @@ __global__ void k_tolerance(PcgState* st, const double* partials, int nparts, in
   st->solves++;
 }
 
-// ---- FP64: r = Kx - rhs; y = M^-1 r; p = -y; partials r'y and max|r| ---------
-__global__ void k_init_r(int n, const double* Kx, const double* rhs, const double* dinv, double* r, double* y,
-                         double* p, double* part_rty, double* part_rn) {
-  __shared__ double lds[kBlock / kWave];
-  double dot = 0.0, mx = 0.0;
-  GRID_STRIDE(i, n) {
-    double ri = Kx[i] - rhs[i];
-    double yi = dinv[i] * ri;
-    r[i] = ri;
-    y[i] = yi;
-    p[i] = -yi;
-    dot += yi * ri;
-    double a = fabs(ri);
-    mx = a > mx ? a : mx;
-  }
-  double t1 = block_sum(dot, lds);
-  double t2 = block_max(mx, lds);
-  if (threadIdx.x == 0) {
-    part_rty[blockIdx.x] = t1;
-    part_rn[blockIdx.x] = t2;
-  }
-}
-
-// ---- FP32 defect correction ---------------------------------------------------
-// r64 = K x64 - rhs64 (in place in Kx); partial max|r64|
-__global__ void k_dc_residual(int n, double* Kx, const double* rhs, double* part_rn) {
-  __shared__ double lds[kBlock / kWave];
-  double mx = 0.0;
-  GRID_STRIDE(i, n) {
-    double ri = Kx[i] - rhs[i];
-    Kx[i] = ri;
-    double a = fabs(ri);
-    mx = a > mx ? a : mx;
-  }
-  double t = block_max(mx, lds);
-  if (threadIdx.x == 0) part_rn[blockIdx.x] = t;
-}
-
-// one block: scale = ||r64||; tolerance on the scaled system
+// one block: scale = ||r64||; tolerance on the scaled correction system
 __global__ void k_dc_scale(PcgState* st, const double* part_rn, int nparts) {
   __shared__ double lds[kBlock / kWave];
   double mx = 0.0;
@@ -255,19 +267,20 @@ __global__ void k_dc_scale(PcgState* st, const double* part_rn, int nparts) {
 }
 
 // r = r64 / scale; y = M^-1 r; p = -y; d = 0; partials r'y, max|r|
-__global__ void k_dc_init_cg(const PcgState* st, int n, const double* r64, const float* dinv, float* r, float* y,
-                             float* p, float* d, double* part_rty, double* part_rn) {
+template <typename V>
+__global__ void k_dc_init_cg(const PcgState* st, int n, const double* r64, const V* dinv, V* r, V* y, V* p, V* d,
+                             double* part_rty, double* part_rn) {
   __shared__ double lds[kBlock / kWave];
   const double inv = 1.0 / st->scale;
-  Acc<float> dot;
+  Acc<V> dot;
   double mx = 0.0;
   GRID_STRIDE(i, n) {
-    float ri = (float)(r64[i] * inv);
-    float yi = dinv[i] * ri;
+    V ri = (V)(r64[i] * inv);
+    V yi = dinv[i] * ri;
     r[i] = ri;
     y[i] = yi;
     p[i] = -yi;
-    d[i] = 0.0f;
+    d[i] = (V)0;
     dot.add(yi * ri);
     double a = fabs((double)ri);
     mx = a > mx ? a : mx;
@@ -281,7 +294,8 @@ __global__ void k_dc_init_cg(const PcgState* st, int n, const double* r64, const
 }
 
 // x64 += scale * d ; b[0:n] = x64
-__global__ void k_dc_apply(const PcgState* st, int n, const float* d, double* x64, double* b) {
+template <typename V>
+__global__ void k_dc_apply(const PcgState* st, int n, const V* d, double* x64, double* b) {
   const double s = st->scale;
   const int ran = st->iter > 0;
   GRID_STRIDE(i, n) {
@@ -293,10 +307,15 @@ __global__ void k_dc_apply(const PcgState* st, int n, const float* d, double* x6
 }
 
 // ---- loop head / alpha: one block -----------------------------------------------
+// `flags` (host-coherent, may be null): flags[it] = 2 * solve + active, read by
+// the host to stop queueing iterations.
 __global__ void k_loop_head(PcgState* st, const double* part_rty, const double* part_rn, int nparts, int max_iter,
-                            int first, int guard) {
+                            int first, int guard, int* flags, int it) {
   __shared__ double lds[kBlock / kWave];
-  if (!st->active) return;
+  if (!st->active) {
+    if (flags && threadIdx.x == 0) flags[it] = 2 * st->solves;
+    return;
+  }
   double s = 0.0, mx = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
     s += part_rty[i];
@@ -316,6 +335,7 @@ __global__ void k_loop_head(PcgState* st, const double* part_rty, const double* 
   // the mixed path stops on a non-finite or non-positive r'y (FP32 breakdown)
   if (guard && !(rty > 0.0 && isfinite(rty))) go = 0;
   st->active = go;
+  if (flags) flags[it] = 2 * st->solves + go;
 }
 
 __global__ void k_alpha(PcgState* st, const double* part, int nparts, int guard) {
@@ -331,7 +351,7 @@ __global__ void k_alpha(PcgState* st, const double* part, int nparts, int guard)
   }
 }
 
-// x += alpha p; r += alpha Kp; y = M^-1 r; partials r'y, max|r|
+// d += alpha p; r += alpha Kp; y = M^-1 r; partials r'y, max|r|
 template <typename V>
 __global__ void k_update(const PcgState* st, int n, V* x, V* r, V* y, const V* p, const V* Kp, const V* dinv,
                          double* part_rty, double* part_rn) {
@@ -366,32 +386,7 @@ __global__ void k_pupdate(const PcgState* st, int n, V* p, const V* y) {
   GRID_STRIDE(i, n) p[i] = beta * p[i] - y[i];
 }
 
-__global__ void k_copy_x(int n, const double* x, double* b) {
-  GRID_STRIDE(i, n) b[i] = x[i];
-}
-
-// b[n:n+m] = A x  (or rho_pol (A x - b2) when polishing); FP64 accumulation
-template <typename TV, int G>
-__global__ __launch_bounds__(kBlock) void k_finish_z(int m, int n, const int* __restrict__ rp,
-                                                     const int* __restrict__ ci, const TV* __restrict__ val,
-                                                     const double* __restrict__ x, double* __restrict__ b,
-                                                     int polishing, double rho_pol) {
-  const int lane = threadIdx.x & (G - 1);
-  const int rpb = kBlock / G;
-  for (int row = blockIdx.x * rpb + threadIdx.x / G; row < m; row += gridDim.x * rpb) {
-    double acc = 0.0;
-    const int e = rp[row + 1];
-    for (int k = rp[row] + lane; k < e; k += G) acc += (double)val[k] * x[ci[k]];
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      double z = acc;
-      if (polishing) z = (z - b[n + row]) * rho_pol;
-      b[n + row] = z;
-    }
-  }
-}
-
-// diag preconditioner: dinv = 1 / (sigma + diag(P) + sum_j rho_j A_ji^2)
+// diag preconditioner: d = sigma + diag(P) + sum_j rho_j A_ji^2
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_precond(int n, double sigma, const double* pval, const int* diag_pos,
                                                     const int* trp, const int* tci, const double* tval,
@@ -434,6 +429,10 @@ __global__ void k_state_init(PcgState* st, double tol_fraction) {
   }
 }
 
+__global__ void k_set_active(PcgState* st, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st->active = v;
+}
+
 // ---- the solver object ----------------------------------------------------------
 struct HipPcg {
   LinSysSolver base;  // must stay first: LinSysSolver* <-> HipPcg*
@@ -443,78 +442,29 @@ struct HipPcg {
   const OSQPMatrix *P, *A;
   const OSQPVectorf* rho_vec;
   OSQPFloat *scaled_prim_res, *scaled_dual_res;
-  // FP64 vectors: x (warm start), rhs, Kx / r64, w64
-  double *x64, *rhs64, *r64, *w64;
-  // CG vectors in the CG precision (double for FP64, float for FP32)
+  // FP64 vectors: x (warm start), rhs, residual, w, cached A x
+  double *x64, *rhs64, *r64, *w64, *ax64;
+  int ax_valid;       // ax64 == A x64 for the current x64 and A values
+  long long ax_Aver;  // A version ax64 was computed with
+  // CG vectors in the CG precision (double or float)
   void *d, *p, *Kp, *r, *y, *w, *dinv;
   double* dtmp;       // FP64 n-vector (preconditioner diagonal)
   PcgState* st;
   double* parts;      // 3 * kMaxReduceBlocks partial slots
   PcgState* h_st;     // pinned mirror for counters
+  int* h_flags;       // host-coherent per-iteration loop flags
+  int* d_flags;       // device address of h_flags
+  int flags_len;
+  int host_solves;    // solves queued (matches st->solves once they ran)
   int gA, gT, gK;     // lanes per row: A rows, A' rows, K rows
+  int lookahead;      // CG iterations queued ahead of the last observed flag
+  // combined K-operator CSR [P A'] + its row blocks: FP64 values for the
+  // defect-correction residual, FP32 values for the mixed CG
+  int *krp, *kci, *k_rows, k_nb;
+  double* kval64;
+  float* kval32;
+  long long kver_P, kver_A;
 };
-
-template <template <typename, typename, int> class K, typename TV, typename TX, typename... Args>
-void launch_g2(int G, int grid, Args... args) {
-  switch (G) {
-    case 4: hipLaunchKernelGGL((K<TV, TX, 4>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    case 8: hipLaunchKernelGGL((K<TV, TX, 8>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    case 16: hipLaunchKernelGGL((K<TV, TX, 16>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    case 32: hipLaunchKernelGGL((K<TV, TX, 32>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    default: hipLaunchKernelGGL((K<TV, TX, 64>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-  }
-}
-
-template <template <typename, int> class K, typename TV, typename... Args>
-void launch_g1(int G, int grid, Args... args) {
-  switch (G) {
-    case 4: hipLaunchKernelGGL((K<TV, 4>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    case 8: hipLaunchKernelGGL((K<TV, 8>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    case 16: hipLaunchKernelGGL((K<TV, 16>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    case 32: hipLaunchKernelGGL((K<TV, 32>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-    default: hipLaunchKernelGGL((K<TV, 64>::fn), dim3(grid), dim3(kBlock), 0, S(), args...); break;
-  }
-}
-
-template <typename TV, typename TX, int G> struct AvRhoK { static constexpr auto fn = k_Av_rho<TV, TX, G>; };
-template <typename TV, typename TX, int G> struct KvK { static constexpr auto fn = k_Kv<TV, TX, G>; };
-template <typename TV, int G> struct RhsK { static constexpr auto fn = k_rhs<TV, G>; };
-template <typename TV, int G> struct FinishK { static constexpr auto fn = k_finish_z<TV, G>; };
-
-inline int rows_grid(int rows, int G, int cap) {
-  const int rpb = kBlock / G;
-  long long g = (rows + rpb - 1) / rpb;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return (int)g;
-}
-
-template <typename TV>
-struct Views {
-  const int *arp, *aci, *trp, *tci, *prp, *pci;
-  const TV *aval, *tval, *pval;
-};
-
-template <typename TV>
-Views<TV> views(const HipPcg* s);
-
-template <>
-Views<double> views<double>(const HipPcg* s) {
-  Views<double> v;
-  v.arp = s->A->rp; v.aci = s->A->ci; v.aval = s->A->val;
-  v.trp = s->m ? s->A->trp : nullptr; v.tci = s->A->tci; v.tval = s->A->tval;
-  v.prp = s->P->rp; v.pci = s->P->ci; v.pval = s->P->val;
-  return v;
-}
-
-template <>
-Views<float> views<float>(const HipPcg* s) {
-  Views<float> v;
-  v.arp = s->A->rp; v.aci = s->A->ci; v.aval = s->A->val32;
-  v.trp = s->m ? s->A->trp : nullptr; v.tci = s->A->tci; v.tval = s->A->tval32;
-  v.prp = s->P->rp; v.pci = s->P->ci; v.pval = s->P->val32;
-  return v;
-}
 
 void refresh_fp32(const OSQPMatrix* Mc) {
   OSQPMatrix* M = (OSQPMatrix*)Mc;
@@ -523,22 +473,61 @@ void refresh_fp32(const OSQPMatrix* Mc) {
   if (M->nnz > 0)
     hipLaunchKernelGGL(k_cast<float>, dim3(grid_for(M->nnz, kBlock, 16384)), dim3(kBlock), 0, S(), (long long)M->nnz,
                        (const double*)M->val, M->val32);
-  if (!M->is_triu) {
-    if (!M->tval32) M->tval32 = (float*)dev_alloc(sizeof(float) * (size_t)(M->nnz_in > 0 ? M->nnz_in : 1));
-    if (M->nnz_in > 0)
-      hipLaunchKernelGGL(k_cast<float>, dim3(grid_for(M->nnz_in, kBlock, 16384)), dim3(kBlock), 0, S(),
-                         (long long)M->nnz_in, (const double*)M->tval, M->tval32);
-  }
   M->fp32_dirty = 0;
 }
 
-template <int G> struct PrecK { static constexpr auto fn = k_precond<G>; };
+void fill_kvals(HipPcg* s, int cols_too) {
+  const int n = s->n;
+  const int* trp = s->m ? s->A->trp : nullptr;
+  int grid = grid_for((long long)n * kWave, kBlock, 16384);
+  hipLaunchKernelGGL(k_kfill<double>, dim3(grid), dim3(kBlock), 0, S(), n, (const int*)s->P->rp,
+                     (const int*)s->P->ci, (const double*)s->P->val, trp, (const int*)s->A->tci,
+                     (const double*)s->A->tval, (const int*)s->krp, s->kci, s->kval64, cols_too);
+  if (s->kval32)
+    hipLaunchKernelGGL(k_kfill<float>, dim3(grid), dim3(kBlock), 0, S(), n, (const int*)s->P->rp,
+                       (const int*)s->P->ci, (const double*)s->P->val, trp, (const int*)s->A->tci,
+                       (const double*)s->A->tval, (const int*)s->krp, s->kci, s->kval32, 0);
+  s->kver_P = s->P->version;
+  s->kver_A = s->A->version;
+}
+
+void refresh_kvals(HipPcg* s) {
+  if (s->kver_P == s->P->version && s->kver_A == s->A->version) return;
+  fill_kvals(s, 0);
+}
+
+int build_kcsr(HipPcg* s) {
+  const int n = s->n;
+  const int* trp = s->m ? s->A->trp : nullptr;
+  const long long knnz = (long long)s->P->nnz + (s->m ? (long long)s->A->nnz_in : 0);
+  s->krp = (int*)dev_alloc(sizeof(int) * (size_t)(n + 1));
+  s->kci = (int*)dev_alloc(sizeof(int) * (size_t)(knnz > 0 ? knnz : 1));
+  s->kval64 = (double*)dev_alloc(sizeof(double) * (size_t)(knnz > 0 ? knnz : 1));
+  s->kval32 = s->bits == 32 ? (float*)dev_alloc(sizeof(float) * (size_t)(knnz > 0 ? knnz : 1)) : nullptr;
+  if (!s->krp || !s->kci || !s->kval64 || (s->bits == 32 && !s->kval32)) return 1;
+  hipLaunchKernelGGL(k_krp, dim3(grid_for(n + 1, kBlock, 4096)), dim3(kBlock), 0, S(), n, (const int*)s->P->rp,
+                     trp, s->krp);
+  fill_kvals(s, 1);
+  // row blocks of K (host partition of the downloaded row pointer)
+  int* hk = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+  int* hb = (int*)malloc(sizeof(int) * (size_t)(n + 2));
+  if (!hk || !hb) {
+    free(hk);
+    free(hb);
+    return 1;
+  }
+  download(hk, s->krp, sizeof(int) * (size_t)(n + 1));
+  s->k_nb = build_row_blocks(hk, n, hb);
+  s->k_rows = (int*)dev_alloc(sizeof(int) * (size_t)(s->k_nb + 1));
+  upload(s->k_rows, hb, sizeof(int) * (size_t)(s->k_nb + 1));
+  free(hk);
+  free(hb);
+  return s->k_rows ? 0 : 1;
+}
 
 void update_precond(HipPcg* s) {
-  if (s->bits == 32) {
-    refresh_fp32(s->P);
-    refresh_fp32(s->A);
-  }
+  if (s->bits == 32) refresh_fp32(s->A);
+  refresh_kvals(s);
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
   const int G = s->gT;
   int grid = rows_grid(s->n, G, 16384);
@@ -564,107 +553,112 @@ void update_precond(HipPcg* s) {
   LAUNCHCHK();
 }
 
-// CG loop shared by both arithmetics: x is the CG iterate (x64 or d)
+// Host-side stop rule: before queueing iteration `it`, look at the flag of
+// iteration it - lookahead.  The loop-head kernel of that iteration writes
+// 2 * solve + active; while it still holds an older solve's value the GPU has
+// not got there yet and the host waits (yielding), bounded by a deadline after
+// which it simply keeps queueing (the kernels exit on the device flag anyway).
+bool host_should_stop(HipPcg* s, int it) {
+  if (!s->h_flags || it < s->lookahead) return false;
+  volatile int* f = (volatile int*)s->h_flags;
+  const int want = 2 * s->host_solves;
+  struct timespec t0, t;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (;;) {
+    int v = f[it - s->lookahead];
+    if (v >= want) return (v & 1) == 0;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    if ((t.tv_sec - t0.tv_sec) > 5) return false;
+    sched_yield();
+  }
+}
+
 template <typename V>
-void cg_loop(HipPcg* s, V* x, int gA, int gK, int gE, int guard) {
+void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
+  typedef V TV;  // stored value type equals the CG precision
   const int n = s->n, m = s->m;
-  Views<V> mv = views<V>(s);
+  // CG operator in the CG precision; defect-correction residual and z = A x
+  // with the FP64 values (the exact K)
+  const TV* aval = (const TV*)(s->bits == 32 ? (const void*)s->A->val32 : (const void*)s->A->val);
+  const TV* kval = (const TV*)(s->bits == 32 ? (const void*)s->kval32 : (const void*)s->kval64);
+  const double* aval64 = s->A->val;
+  const double* kval64 = s->kval64;
+  const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
+  double* part0 = s->parts;
+  double* part1 = s->parts + kMaxReduceBlocks;
+  double* part2 = s->parts + 2 * kMaxReduceBlocks;
+  const int gK = s->k_nb < kMaxReduceBlocks ? s->k_nb : kMaxReduceBlocks;
+  const int gA = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
+  const int gE = grid_for(n, kBlock, kMaxReduceBlocks);
+  const int gM = grid_for(m, kBlock, 4096);
+  const int guard = s->bits == 32 ? 1 : 0;
   V* p = (V*)s->p;
   V* Kp = (V*)s->Kp;
   V* r = (V*)s->r;
   V* y = (V*)s->y;
   V* w = (V*)s->w;
+  V* d = (V*)s->d;
   const V* dinv = (const V*)s->dinv;
-  const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
-  double* part0 = s->parts;
-  double* part1 = s->parts + kMaxReduceBlocks;
-  double* part2 = s->parts + 2 * kMaxReduceBlocks;
-  for (int it = 0; it < s->max_iter; ++it) {
+
+  s->host_solves++;
+  const bool cached = m > 0 && s->ax_valid && s->ax_Aver == s->A->version && admm_iter > 1 && !s->polishing;
+  if (!cached) {
+    // explicit rhs: needed for ||rhs|| at the first ADMM iteration and in polishing
+    hipLaunchKernelGGL(k_rb_rhs<double>, dim3(gK), dim3(kBlock), 0, S(), (const int*)s->k_rows, s->k_nb,
+                       (const int*)s->krp, (const int*)s->kci, kval64, n, (const double*)b->d, rho, s->rho_sc,
+                       s->rhs64, part0);
+    hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, gK, admm_iter,
+                       *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
+                       s->reduction_threshold);
     if (m)
-      launch_g2<AvRhoK, V, V>(s->gA, gA, (const PcgState*)s->st, m, mv.arp, mv.aci, mv.aval, (const V*)p, rho,
-                              s->rho_sc, w);
-    launch_g2<KvK, V, V>(s->gK, gK, (const PcgState*)s->st, n, s->sigma, mv.prp, mv.pci, mv.pval,
-                         m ? mv.trp : nullptr, mv.tci, mv.tval, (const V*)p, (const V*)w, Kp, part2);
+      hipLaunchKernelGGL((k_rb_Av_rho<double, double>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci, aval64,
+                         (const double*)s->x64, rho, s->rho_sc, s->w64);
+    // r64 = K x64 - rhs64
+    hipLaunchKernelGGL((k_rb_Kv<double, double>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                       (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci, kval64, n, s->sigma,
+                       (const double*)s->x64, (const double*)s->w64, (const double*)s->rhs64, s->r64, part1);
+  } else {
+    hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)nullptr, 0, admm_iter,
+                       *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
+                       s->reduction_threshold);
+    // w64 = rho o (A x64 - b2);  r64 = sigma x64 + P x64 + A' w64 - b1
+    hipLaunchKernelGGL(k_w_from_ax, dim3(gM), dim3(kBlock), 0, S(), (const PcgState*)s->st, m,
+                       (const double*)s->ax64, (const double*)(b->d + n), rho, s->rho_sc, s->w64);
+    hipLaunchKernelGGL((k_rb_Kv<double, double>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                       (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci, kval64, n, s->sigma,
+                       (const double*)s->x64, (const double*)s->w64, (const double*)b->d, s->r64, part1);
+  }
+  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part1, gK);
+  hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
+                     (const double*)s->r64, dinv, r, y, p, d, part0, part1);
+  hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
+                     gE, s->max_iter, 1, guard, s->d_flags, 0);
+  for (int it = 0; it < s->max_iter; ++it) {
+    if (host_should_stop(s, it)) break;
+    if (m)
+      hipLaunchKernelGGL((k_rb_Av_rho<TV, V>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci, aval,
+                         (const V*)p, rho, s->rho_sc, w);
+    hipLaunchKernelGGL((k_rb_Kv<TV, V>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                       (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci, kval, n, s->sigma,
+                       (const V*)p, (const V*)w, (const double*)nullptr, Kp, part2);
     hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part2, gK, guard);
-    hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, x, r, y,
+    hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, d, r, y,
                        (const V*)p, (const V*)Kp, dinv, part0, part1);
     hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0,
-                       (const double*)part1, gE, s->max_iter, 0, guard);
+                       (const double*)part1, gE, s->max_iter, 0, guard, s->d_flags, it + 1);
     hipLaunchKernelGGL(k_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, p, (const V*)y);
   }
-}
-
-void solve_fp64(HipPcg* s, OSQPVectorf* b, int admm_iter) {
-  const int n = s->n, m = s->m;
-  Views<double> mv = views<double>(s);
-  const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
-  double* part0 = s->parts;
-  double* part1 = s->parts + kMaxReduceBlocks;
-  double* part2 = s->parts + 2 * kMaxReduceBlocks;
-  const int gK = rows_grid(n, s->gK, kMaxReduceBlocks);
-  const int gA = rows_grid(m, s->gA, 65535 * 4);
-  const int gE = grid_for(n, kBlock, kMaxReduceBlocks);
-  double* r = (double*)s->r;
-  double* y = (double*)s->y;
-  double* p = (double*)s->p;
-  launch_g1<RhsK, double>(s->gK, gK, n, (const double*)b->d, mv.trp, mv.tci, mv.tval, rho, s->rho_sc, s->rhs64,
-                          part0);
-  hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, gK, admm_iter,
-                     *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
-                     s->reduction_threshold);
-  // r = K x - rhs (x warm-started from the previous solve)
-  if (m)
-    launch_g2<AvRhoK, double, double>(s->gA, gA, (const PcgState*)s->st, m, mv.arp, mv.aci, mv.aval,
-                                      (const double*)s->x64, rho, s->rho_sc, s->w64);
-  launch_g2<KvK, double, double>(s->gK, gK, (const PcgState*)s->st, n, s->sigma, mv.prp, mv.pci, mv.pval,
-                                 m ? mv.trp : nullptr, mv.tci, mv.tval, (const double*)s->x64,
-                                 (const double*)s->w64, s->r64, (double*)nullptr);
-  hipLaunchKernelGGL(k_init_r, dim3(gE), dim3(kBlock), 0, S(), n, (const double*)s->r64, (const double*)s->rhs64,
-                     (const double*)s->dinv, r, y, p, part0, part1);
-  hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
-                     gE, s->max_iter, 1, 0);
-  cg_loop<double>(s, s->x64, gA, gK, gE, 0);
-  hipLaunchKernelGGL(k_copy_x, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), n, (const double*)s->x64, b->d);
-  if (m)
-    launch_g1<FinishK, double>(s->gA, gA, m, n, mv.arp, mv.aci, mv.aval, (const double*)s->x64, b->d, s->polishing,
-                               s->rho_sc);
-  LAUNCHCHK();
-  (void)part2;
-}
-
-void solve_fp32(HipPcg* s, OSQPVectorf* b, int admm_iter) {
-  const int n = s->n, m = s->m;
-  Views<float> mv = views<float>(s);
-  const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
-  double* part0 = s->parts;
-  double* part1 = s->parts + kMaxReduceBlocks;
-  const int gK = rows_grid(n, s->gK, kMaxReduceBlocks);
-  const int gA = rows_grid(m, s->gA, 65535 * 4);
-  const int gE = grid_for(n, kBlock, kMaxReduceBlocks);
-  launch_g1<RhsK, float>(s->gK, gK, n, (const double*)b->d, mv.trp, mv.tci, mv.tval, rho, s->rho_sc, s->rhs64,
-                         part0);
-  hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, gK, admm_iter,
-                     *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
-                     s->reduction_threshold);
-  // r64 = K x64 - rhs64 with FP32 values and FP64 accumulation
-  if (m)
-    launch_g2<AvRhoK, float, double>(s->gA, gA, (const PcgState*)s->st, m, mv.arp, mv.aci, mv.aval,
-                                     (const double*)s->x64, rho, s->rho_sc, s->w64);
-  launch_g2<KvK, float, double>(s->gK, gK, (const PcgState*)s->st, n, s->sigma, mv.prp, mv.pci, mv.pval,
-                                m ? mv.trp : nullptr, mv.tci, mv.tval, (const double*)s->x64,
-                                (const double*)s->w64, s->r64, (double*)nullptr);
-  hipLaunchKernelGGL(k_dc_residual, dim3(gE), dim3(kBlock), 0, S(), n, s->r64, (const double*)s->rhs64, part1);
-  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part1, gE);
-  hipLaunchKernelGGL(k_dc_init_cg, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, (const double*)s->r64,
-                     (const float*)s->dinv, (float*)s->r, (float*)s->y, (float*)s->p, (float*)s->d, part0, part1);
-  hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
-                     gE, s->max_iter, 1, 1);
-  cg_loop<float>(s, (float*)s->d, gA, gK, gE, 1);
-  hipLaunchKernelGGL(k_dc_apply, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
-                     (const float*)s->d, s->x64, b->d);
-  if (m)
-    launch_g1<FinishK, float>(s->gA, gA, m, n, mv.arp, mv.aci, mv.aval, (const double*)s->x64, b->d, s->polishing,
-                              s->rho_sc);
+  hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
+                     (const V*)d, s->x64, b->d);
+  if (m) {
+    hipLaunchKernelGGL(k_rb_finish_z<double>, dim3(gA), dim3(kBlock), 0, S(), (const int*)s->A->rb_rows,
+                       s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci, aval64, (const double*)s->x64, b->d,
+                       s->ax64, n, s->polishing, s->rho_sc);
+    s->ax_valid = 1;
+    s->ax_Aver = s->A->version;
+  }
   LAUNCHCHK();
 }
 
@@ -677,11 +671,32 @@ const char* pcg_name(LinSysSolver* self) {
 
 OSQPInt pcg_solve(LinSysSolver* self, OSQPVectorf* b, OSQPInt admm_iter) {
   HipPcg* s = (HipPcg*)self;
+  if (s->flags_len < s->max_iter + 1) {
+    if (s->h_flags) {
+      HIPCHK(hipStreamSynchronize(S()));
+      (void)hipHostFree(s->h_flags);
+    }
+    s->h_flags = nullptr;
+    s->d_flags = nullptr;
+    s->flags_len = 0;
+    if (hipHostMalloc((void**)&s->h_flags, sizeof(int) * (size_t)(s->max_iter + 1),
+                      hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+        hipHostGetDevicePointer((void**)&s->d_flags, s->h_flags, 0) == hipSuccess) {
+      memset(s->h_flags, 0, sizeof(int) * (size_t)(s->max_iter + 1));
+      s->flags_len = s->max_iter + 1;
+    } else {
+      (void)hipGetLastError();
+      if (s->h_flags) (void)hipHostFree(s->h_flags);
+      s->h_flags = nullptr;
+      s->d_flags = nullptr;
+    }
+  }
+  if (s->kver_P != s->P->version || s->kver_A != s->A->version || (s->bits == 32 && s->A->fp32_dirty))
+    update_precond(s);
   if (s->bits == 32) {
-    if (s->A->fp32_dirty || s->P->fp32_dirty) update_precond(s);
-    solve_fp32(s, b, admm_iter);
+    solve_impl<float>(s, b, admm_iter);
   } else {
-    solve_fp64(s, b, admm_iter);
+    solve_impl<double>(s, b, admm_iter);
   }
   return 0;
 }
@@ -702,23 +717,28 @@ void pcg_warm_start(LinSysSolver* self, const OSQPVectorf* x) {
   HipPcg* s = (HipPcg*)self;
   if (s->n > 0)
     HIPCHK(hipMemcpyAsync(s->x64, x->d, sizeof(double) * (size_t)s->n, hipMemcpyDeviceToDevice, S()));
+  s->ax_valid = 0;
 }
 
 void pcg_free(LinSysSolver* self) {
   HipPcg* s = (HipPcg*)self;
   if (!s) return;
   HIPCHK(hipStreamSynchronize(S()));
-  dev_free(s->x64); dev_free(s->rhs64); dev_free(s->r64); dev_free(s->w64);
+  dev_free(s->x64); dev_free(s->rhs64); dev_free(s->r64); dev_free(s->w64); dev_free(s->ax64);
   dev_free(s->d); dev_free(s->p); dev_free(s->Kp); dev_free(s->r); dev_free(s->y);
   dev_free(s->w); dev_free(s->dinv); dev_free(s->dtmp);
   dev_free(s->st); dev_free(s->parts);
+  dev_free(s->krp); dev_free(s->kci); dev_free(s->kval64); dev_free(s->kval32); dev_free(s->k_rows);
   if (s->h_st) (void)hipHostFree(s->h_st);
+  if (s->h_flags) (void)hipHostFree(s->h_flags);
   free(s);
 }
 
 OSQPInt pcg_update_matrices(LinSysSolver* self, const OSQPMatrix*, const OSQPInt*, OSQPInt, const OSQPMatrix*,
                             const OSQPInt*, OSQPInt) {
-  update_precond((HipPcg*)self);
+  HipPcg* s = (HipPcg*)self;
+  s->ax_valid = 0;
+  update_precond(s);
   return 0;
 }
 
@@ -774,6 +794,7 @@ OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** sp, const OSQPMatrix* P, 
   s->rho_vec = rho_vec;
   s->scaled_prim_res = scaled_prim_res;
   s->scaled_dual_res = scaled_dual_res;
+  s->lookahead = 2;
   if (polishing) {
     s->sigma = settings->delta;
     s->rho_sc = 1.0 / settings->delta;
@@ -787,6 +808,7 @@ OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** sp, const OSQPMatrix* P, 
   s->rhs64 = (double*)dev_alloc(sizeof(double) * n);
   s->r64 = (double*)dev_alloc(sizeof(double) * n);
   s->w64 = (double*)dev_alloc(sizeof(double) * m);
+  s->ax64 = (double*)dev_alloc(sizeof(double) * m);
   s->d = dev_alloc(vb * n);
   s->p = dev_alloc(vb * n);
   s->Kp = dev_alloc(vb * n);
@@ -797,8 +819,8 @@ OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** sp, const OSQPMatrix* P, 
   s->dtmp = (double*)dev_alloc(sizeof(double) * n);
   s->st = (PcgState*)dev_alloc(sizeof(PcgState));
   s->parts = (double*)dev_alloc(sizeof(double) * 3 * kMaxReduceBlocks);
-  if (!s->x64 || !s->rhs64 || !s->r64 || !s->w64 || !s->d || !s->p || !s->Kp || !s->r || !s->y || !s->w ||
-      !s->dinv || !s->dtmp || !s->st || !s->parts) {
+  if (!s->x64 || !s->rhs64 || !s->r64 || !s->w64 || !s->ax64 || !s->d || !s->p || !s->Kp || !s->r || !s->y ||
+      !s->w || !s->dinv || !s->dtmp || !s->st || !s->parts) {
     pcg_free(&s->base);
     *sp = nullptr;
     return OSQP_LINSYS_SOLVER_INIT_ERROR;
@@ -810,6 +832,11 @@ OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** sp, const OSQPMatrix* P, 
   s->gT = A->tlanes;
   // K rows hold a P row and an A' row: size lanes for their sum
   s->gK = lanes_for_rows((long long)P->nnz + (long long)A->nnz_in, s->n);
+  if (build_kcsr(s)) {
+    pcg_free(&s->base);
+    *sp = nullptr;
+    return OSQP_LINSYS_SOLVER_INIT_ERROR;
+  }
   update_precond(s);
   return 0;
 }
@@ -829,45 +856,48 @@ OSQPInt osqp_amd_linsys_time_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventCreate(&e2));
-  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, S(), s->st, s->tol_fraction);
-  // activate the loop flag so the kernels run their full body
-  PcgState h;
-  memset(&h, 0, sizeof(h));
-  h.active = 1;
-  h.red_factor = s->tol_fraction;
-  h.eps_prev = 1.0;
-  h.scale = 1.0;
-  HIPCHK(hipMemcpyAsync(s->st, &h, sizeof(h), hipMemcpyHostToDevice, S()));
+  update_precond(s);
+  // inputs: p = 1, w = 1 (finite values; the timing does not depend on them)
+  if (s->bits == 32) {
+    hipLaunchKernelGGL(k_recip_to<float>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), n,
+                       (const double*)s->dtmp, (float*)s->p, 1);
+    if (m) hipLaunchKernelGGL(k_recip_to<float>, dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, S(), m,
+                              (const double*)s->w64, (float*)s->w, 1);
+  } else {
+    hipLaunchKernelGGL(k_recip_to<double>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), n,
+                       (const double*)s->dtmp, (double*)s->p, 1);
+    if (m) hipLaunchKernelGGL(k_recip_to<double>, dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, S(), m,
+                              (const double*)s->w64, (double*)s->w, 1);
+  }
+  hipLaunchKernelGGL(k_set_active, dim3(1), dim3(64), 0, S(), s->st, 1);
   HIPCHK(hipStreamSynchronize(S()));
-  const int gK = rows_grid(n, s->gK, kMaxReduceBlocks);
-  const int gA = rows_grid(m, s->gA, 65535 * 4);
+  const int gK = s->k_nb < kMaxReduceBlocks ? s->k_nb : kMaxReduceBlocks;
+  const int gA = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
   double* part2 = s->parts + 2 * kMaxReduceBlocks;
   HIPCHK(hipEventRecord(e0, S()));
-  for (int r = 0; r < reps; ++r) {
-    if (s->bits == 32) {
-      Views<float> mv = views<float>(s);
-      if (m) launch_g2<AvRhoK, float, float>(s->gA, gA, (const PcgState*)s->st, m, mv.arp, mv.aci, mv.aval,
-                                             (const float*)s->p, rho, s->rho_sc, (float*)s->w);
-    } else {
-      Views<double> mv = views<double>(s);
-      if (m) launch_g2<AvRhoK, double, double>(s->gA, gA, (const PcgState*)s->st, m, mv.arp, mv.aci, mv.aval,
-                                               (const double*)s->p, rho, s->rho_sc, (double*)s->w);
-    }
+  for (int r = 0; r < reps && m; ++r) {
+    if (s->bits == 32)
+      hipLaunchKernelGGL((k_rb_Av_rho<float, float>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci,
+                         (const float*)s->A->val32, (const float*)s->p, rho, s->rho_sc, (float*)s->w);
+    else
+      hipLaunchKernelGGL((k_rb_Av_rho<double, double>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci,
+                         (const double*)s->A->val, (const double*)s->p, rho, s->rho_sc, (double*)s->w);
   }
   HIPCHK(hipEventRecord(e1, S()));
   for (int r = 0; r < reps; ++r) {
-    if (s->bits == 32) {
-      Views<float> mv = views<float>(s);
-      launch_g2<KvK, float, float>(s->gK, gK, (const PcgState*)s->st, n, s->sigma, mv.prp, mv.pci, mv.pval,
-                                   m ? mv.trp : nullptr, mv.tci, mv.tval, (const float*)s->p, (const float*)s->w,
-                                   (float*)s->Kp, part2);
-    } else {
-      Views<double> mv = views<double>(s);
-      launch_g2<KvK, double, double>(s->gK, gK, (const PcgState*)s->st, n, s->sigma, mv.prp, mv.pci, mv.pval,
-                                     m ? mv.trp : nullptr, mv.tci, mv.tval, (const double*)s->p,
-                                     (const double*)s->w, (double*)s->Kp, part2);
-    }
+    if (s->bits == 32)
+      hipLaunchKernelGGL((k_rb_Kv<float, float>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                         (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci,
+                         (const float*)s->kval32, n, s->sigma, (const float*)s->p, (const float*)s->w,
+                         (const double*)nullptr, (float*)s->Kp, part2);
+    else
+      hipLaunchKernelGGL((k_rb_Kv<double, double>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                         (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci,
+                         (const double*)s->kval64, n, s->sigma, (const double*)s->p, (const double*)s->w,
+                         (const double*)nullptr, (double*)s->Kp, part2);
   }
   HIPCHK(hipEventRecord(e2, S()));
   HIPCHK(hipEventSynchronize(e2));
@@ -877,15 +907,20 @@ OSQPInt osqp_amd_linsys_time_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_
   if (ms_av) *ms_av = (double)t01 / reps;
   if (ms_kv) *ms_kv = (double)t12 / reps;
   const double nnzA = (double)s->A->nnz_in, nnzP = (double)s->P->nnz;
+  // per launch: values + indices once, row pointers once, each gathered
+  // vector element once, rho (FP64) once, the output once
   if (bytes_av) *bytes_av = nnzA * (vb + 4.0) + (m + 1.0) * 4.0 + n * vb + m * 8.0 + m * vb;
   if (bytes_kv)
-    *bytes_kv = nnzP * (vb + 4.0) + (n + 1.0) * 4.0 + nnzA * (vb + 4.0) + (n + 1.0) * 4.0 + m * vb + 2.0 * n * vb;
+    *bytes_kv = (nnzP + nnzA) * (vb + 4.0) + (n + 1.0) * 4.0 + m * vb + 2.0 * n * vb;
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
   HIPCHK(hipEventDestroy(e2));
-  // leave the state as a fresh solver's
+  // the CG state of this solver is reset: the next solve starts fresh
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, S(), s->st, s->tol_fraction);
+  s->ax_valid = 0;
+  s->host_solves = 0;
   HIPCHK(hipStreamSynchronize(S()));
+  if (s->h_flags) memset(s->h_flags, 0, sizeof(int) * (size_t)s->flags_len);
   return 0;
 }
 

@@ -320,3 +320,30 @@ def test_update_values_and_submatrix(algebra):
     L.OSQPMatrix_free(sub)
     for v in (MA, MP, vx, vy, vyv, vo, vp, rows, vs):
         v.free()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_spmv_skewed_rows(algebra, seed):
+    """Row-block partition: rows far longer than one block's chunk (2048
+    nonzeros), empty rows and many tiny rows in one matrix."""
+    rng = np.random.default_rng(seed)
+    m, n = 3000, 20000
+    lens = rng.integers(0, 6, m)
+    lens[[5, 1700]] = [9000, 20000]      # long rows (one is fully dense)
+    lens[100:140] = 0                    # empty rows
+    rows = np.repeat(np.arange(m), lens)
+    cols = np.concatenate([rng.choice(n, k, replace=False) for k in lens])
+    A = sp.csc_matrix((rng.standard_normal(len(rows)), (rows, cols)), shape=(m, n))
+    L = algebra.lib()
+    MA = Mat(A)
+    x = rng.standard_normal(n)
+    vx, vy = Vec(x), Vec(n=m)
+    L.OSQPMatrix_Axpy(MA.h, vx.h, vy.h, 1.0, 0.0)
+    bnd = _bound(abs(A), np.abs(x), np.diff(A.tocsr().indptr))
+    assert np.all(np.abs(vy.get() - A @ x) <= bnd)
+    yv = rng.standard_normal(m)
+    vyv, vo = Vec(yv), Vec(n=n)
+    L.OSQPMatrix_Atxpy(MA.h, vyv.h, vo.h, 1.0, 0.0)
+    assert np.all(np.abs(vo.get() - A.T @ yv) <= _bound(abs(A.T), np.abs(yv), np.diff(A.indptr)))
+    for v in (MA, vx, vy, vyv, vo):
+        v.free()

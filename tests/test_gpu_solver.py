@@ -288,3 +288,15 @@ def test_update_matrices():
     assert r.info.status_val == rf.info.status_val == 1
     assert abs(r.info.obj_val - rf.info.obj_val) / abs(rf.info.obj_val) < 1e-4
     del q
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_long_rows_qp(precision):
+    """A QP whose constraint matrix has rows longer than one SpMV row block
+    (a dense budget row over 6000 variables, as in the portfolio model)."""
+    p = pr.portfolio(6000, seed=4)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    o = oracle_solve(p, **st)
+    s, r = gpu_solve(p, precision, **st)
+    assert r.info.status_val == o.info.status_val == 1
+    assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * max(1.0, abs(o.info.obj_val))
