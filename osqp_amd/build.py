@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libosqp_amd.so")
 
-HIP_SOURCES = ["hip/runtime.hip", "hip/vector.hip", "hip/matrix.hip", "hip/pcg.hip", "serve/batch.hip"]
+HIP_SOURCES = ["hip/runtime.hip", "hip/vector.hip", "hip/matrix.hip", "hip/panel.hip", "hip/pcg.hip", "serve/batch.hip"]
 C_SOURCES = ["host/auxil.c", "host/osqp_api.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -26,6 +26,7 @@ struct Runtime {
   double* d_partials = nullptr;    // per-block partials (2 * kMaxReduceBlocks doubles)
   double* d_scalars = nullptr;     // device result slots
   int last_error = 0;
+  int num_cu = 0;
   char device_name[256] = {0};
 };
 

@@ -1,0 +1,76 @@
+// panel.h — LDS-staged, column-panelled SpMV for large sparse operators.
+//
+// Random gathers of a multiplied vector larger than a CU's L1 go to L2 one
+// request per lane, and that request rate — not HBM bandwidth — bounds a CSR
+// SpMV on gfx950 (a 500000 x 50000 matrix with 100 nonzeros per row streams
+// at 5.4 TB/s without the gather and at 1.4 TB/s with it).  So a CSR matrix
+// (R rows, C columns) is re-stored as np = ceil(C / W) column panels, each a
+// CSR over all R rows with 16-bit panel-local column indices, and the SpMV
+// gathers from a copy of the panel's slice of x in LDS:
+//
+//   * work items ("tasks") are row ranges of one panel holding at most
+//     64 * PER nonzeros and at most 63 rows, listed panel by panel (a panel
+//     row longer than that is a task of its own, reduced in several passes);
+//     the task descriptor carries the nonzero range so a wavefront issues its
+//     stream loads without waiting for a row pointer;
+//   * a workgroup walks a contiguous range of tasks; for every panel in the
+//     range it loads the x slice into LDS once (all loads in flight, then the
+//     LDS stores) and its wavefronts take the panel's tasks independently —
+//     no workgroup barrier inside a panel;
+//   * a wavefront streams the task's (index, value) pairs with branch-free
+//     loads (arrays are padded by 64 * PER entries), multiplies with the LDS
+//     slice into a per-wavefront LDS product buffer, then reduces each row in
+//     column order (groups of L lanes per row, L a power of two chosen from the
+//     task's row count; in-order sums within a lane, then a fixed butterfly);
+//   * row totals go to partial[p * R + row]; combine kernels add the np
+//     partials of a row in panel order.  Results are deterministic and do not
+//     depend on the grid size.
+#pragma once
+
+#include <stdint.h>
+
+#include "hip_internal.h"
+
+namespace osqp_hip {
+
+constexpr int kPanelRowCap = 63;  // rows per task: one row pointer per lane
+
+struct PanelMatrix {
+  int R, C;
+  int W;    // columns per panel
+  int per;  // nonzeros per lane per task (task holds <= 64 * per)
+  int np;
+  long long nnz;
+  int* rp;         // np * (R + 1) row pointers, absolute into ci/val
+  uint16_t* ci;    // nnz + 64 * per entries: column index within the panel
+  int* src;        // nnz: CSR slot each panel entry copies its value from
+  int ntask;
+  int4* task;      // ntask: first row, end row, first nonzero, end nonzero
+  int2* tpanel;    // ntask: panel of each task, end of that panel's tasks
+  float* val32;    // nnz + pad, optional
+  double* val64;   // nnz + pad, optional
+  long long ver;   // source-value version the copies were made from
+  int grid;        // workgroups per launch
+};
+
+// Host builder from a host CSR structure (R x C): allocates and uploads the
+// structure for the FP32 (bits 32) or FP64 (bits 64) kernel; values are
+// filled by panel_refresh_values.  Returns 0 on success.
+int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int* ci);
+void panel_free(PanelMatrix* P);
+// Whether panels pay for an R x C operator with nnz nonzeros: one panel, or
+// at least 4 nonzeros per (panel, row) on average (the row pointers and
+// partial sums cost about as much as 2 nonzeros per (panel, row)).
+int panel_worth(long long nnz, int R, int C, int bits);
+// val64 or val32 <- csr_val[src] (device array, FP64), on the backend stream.
+void panel_refresh_values(PanelMatrix* P, const double* csr_val);
+// partial[p * R + row] = sum over panel p of row `row`, for x = [x0 (n0); x1].
+// When `active` is non-null the launch does nothing while *active == 0.
+void panel_spmv(const PanelMatrix& P, const int* active, const float* x0, int n0, const float* x1, float* partial);
+void panel_spmv(const PanelMatrix& P, const int* active, const double* x0, int n0, const double* x1,
+                double* partial);
+// bytes one panel_spmv launch moves at minimum (values, indices, row
+// pointers, x once, partials once)
+double panel_spmv_bytes(const PanelMatrix& P, int value_bytes);
+
+}  // namespace osqp_hip

@@ -1,0 +1,293 @@
+// panel.hip — column-panelled SpMV: host construction and the gfx950 kernel.
+//
+// Construction is integer-exact: entries keep their row order and, within a
+// row, their column order; panel p holds the columns [p W, (p + 1) W).  `src`
+// records, for every panel entry, the CSR slot it came from, so values are
+// refreshed on the device with one gather whenever the source matrix is
+// rescaled or updated.
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+#include "panel.h"
+
+namespace osqp_hip {
+
+namespace {
+
+constexpr int kPanelWaves = 16;  // wavefronts per workgroup (1024 threads)
+
+__global__ void k_panel_gather(long long nnz, const int* __restrict__ src, const double* __restrict__ csr_val,
+                               double* __restrict__ v64, float* __restrict__ v32) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x) {
+    const double v = csr_val[src[k]];
+    if (v64) v64[k] = v;
+    if (v32) v32[k] = (float)v;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// One task per wavefront at a time; see panel.h for the scheme.
+template <typename TV, typename TX, int W, int NW, int PER>
+__global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int ntask, const int4* __restrict__ task,
+                                                        const int2* __restrict__ tpanel, const int* __restrict__ rp,
+                                                        const uint16_t* __restrict__ ci, const TV* __restrict__ val,
+                                                        int R, int C, const TX* __restrict__ x0, int n0,
+                                                        const TX* __restrict__ x1, TX* __restrict__ partial) {
+  if (active && !*active) return;
+  __shared__ TX xs[W];
+  __shared__ TX prod_all[NW][64 * PER];
+  constexpr int XL = W / (NW * 64);
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  TX* prod = prod_all[wv];
+  const int t_begin = (int)(((long long)blockIdx.x * ntask) / gridDim.x);
+  const int t_end = (int)(((long long)(blockIdx.x + 1) * ntask) / gridDim.x);
+  for (int t = t_begin; t < t_end;) {
+    const int2 pe = tpanel[t];
+    const int p = pe.x;
+    const int te = pe.y < t_end ? pe.y : t_end;
+    // the panel's slice of x into LDS: every load in flight, then the stores
+    const int c0 = p * W;
+    const int w = (C - c0) < W ? (C - c0) : W;
+    TX xr[XL];
+#pragma unroll
+    for (int j = 0; j < XL; ++j) {
+      const int i = j * NW * 64 + threadIdx.x;
+      const int c = c0 + i;
+      xr[j] = i < w ? (c < n0 ? x0[c] : x1[c - n0]) : (TX)0;
+    }
+    __syncthreads();  // the previous panel's readers are done
+#pragma unroll
+    for (int j = 0; j < XL; ++j) xs[j * NW * 64 + threadIdx.x] = xr[j];
+    __syncthreads();
+    const int* rpp = rp + (long long)p * (R + 1);
+    TX* out = partial + (long long)p * R;
+    // this wavefront's tasks of the panel: t + wv + i NW; lane i holds the
+    // descriptor of the i-th one (64 at a time)
+    for (int ub = t + wv; ub < te; ub += 64 * NW) {
+      const int mine = (te - ub + NW - 1) / NW < 64 ? (te - ub + NW - 1) / NW : 64;
+      int4 d = make_int4(0, 0, 0, 0);
+      if (lane < mine) d = task[ub + lane * NW];
+      for (int i = 0; i < mine; ++i) {
+        const int rb = __builtin_amdgcn_readlane(d.x, i);
+        const int re = __builtin_amdgcn_readlane(d.y, i);
+        const int k0 = __builtin_amdgcn_readlane(d.z, i);
+        const int len = __builtin_amdgcn_readlane(d.w, i) - k0;
+        if (len > 64 * PER) {
+          // one panel row longer than a task: lane-strided passes, then a
+          // fixed butterfly
+          TX acc = 0;
+          for (int b = 0; b < len; b += 64 * PER) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+              const int e = b + j * 64 + lane;
+              if (e < len) acc += (TX)ld_nt(val + k0 + e) * xs[ld_nt(ci + k0 + e)];
+            }
+          }
+          acc = wave_sum(acc);
+          if (lane == 0) out[rb] = acc;
+          continue;
+        }
+        const int rpl = lane <= re - rb ? rpp[rb + lane] : 0;
+        int c[PER];
+        TV v[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+          c[j] = (int)ld_nt(ci + k0 + j * 64 + lane);
+          v[j] = ld_nt(val + k0 + j * 64 + lane);
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) prod[j * 64 + lane] = j * 64 + lane < len ? (TX)v[j] * xs[c[j]] : (TX)0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int nr = re - rb;
+        int L = 64;
+        while (L > 1 && L * nr > 64) L >>= 1;
+        const int G = 64 / L, g = lane / L, gl = lane % L;
+        const int bound = ((nr + G - 1) / G) * G;
+        for (int r = g; r < bound; r += G) {
+          const int s = __shfl(rpl, r & 63) - k0;
+          const int e = __shfl(rpl, (r + 1) & 63) - k0;
+          TX acc = 0;
+          if (r < nr) {
+            int k = s + gl;
+            for (; k + 3 * L < e; k += 4 * L) {
+              const TX a0 = prod[k], a1 = prod[k + L], a2 = prod[k + 2 * L], a3 = prod[k + 3 * L];
+              acc += a0;
+              acc += a1;
+              acc += a2;
+              acc += a3;
+            }
+            for (; k < e; k += L) acc += prod[k];
+          }
+          for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_down(acc, off, L);
+          if (r < nr && gl == 0) out[rb + r] = acc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    t = te;
+  }
+}
+
+template <typename TV, typename TX, int W, int PER>
+void launch(const PanelMatrix& P, const int* active, const TV* val, const TX* x0, int n0, const TX* x1,
+            TX* partial) {
+  hipLaunchKernelGGL((k_panel_spmv<TV, TX, W, kPanelWaves, PER>), dim3(P.grid), dim3(kPanelWaves * 64), 0,
+                     rt().stream, active, P.ntask, (const int4*)P.task, (const int2*)P.tpanel, (const int*)P.rp,
+                     (const uint16_t*)P.ci, val, P.R, P.C, x0, n0, x1, partial);
+}
+
+// FP32 operators: 8192-column panels, 12 nonzeros per lane (80 KB of LDS per
+// workgroup); FP64 operators: 4096-column panels, 4 per lane (64 KB)
+constexpr int kW32 = 8192, kPer32 = 12, kW64 = 4096, kPer64 = 4;
+
+}  // namespace
+
+int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int* ci) {
+  memset(P, 0, sizeof(*P));
+  if (bits != 32 && bits != 64) return 1;
+  const int W = bits == 32 ? kW32 : kW64;
+  const int per = bits == 32 ? kPer32 : kPer64;
+  const int want32 = bits == 32, want64 = bits == 64;
+  const long long nnz = rp[R];
+  const int np = C > 0 ? (C + W - 1) / W : 1;
+  const long long pad = 64LL * per;
+  P->R = R;
+  P->C = C;
+  P->W = W;
+  P->per = per;
+  P->np = np;
+  P->nnz = nnz;
+  // per (panel, row) counts -> absolute row pointers, panel-major
+  std::vector<int> rpa((size_t)np * (R + 1), 0);
+  for (int r = 0; r < R; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) rpa[(size_t)(ci[k] / W) * (R + 1) + r + 1]++;
+  long long base = 0;
+  for (int p = 0; p < np; ++p) {
+    int* q = rpa.data() + (size_t)p * (R + 1);
+    q[0] = (int)base;
+    for (int r = 0; r < R; ++r) q[r + 1] += q[r];
+    base = q[R];
+  }
+  std::vector<uint16_t> lci((size_t)(nnz + pad), 0);
+  std::vector<int> src((size_t)(nnz > 0 ? nnz : 1), 0);
+  std::vector<int> next(rpa);
+  for (int r = 0; r < R; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int p = ci[k] / W;
+      const int pos = next[(size_t)p * (R + 1) + r]++;
+      lci[(size_t)pos] = (uint16_t)(ci[k] - p * W);
+      src[(size_t)pos] = k;
+    }
+  // tasks: per panel, greedy row ranges with <= 64 per nonzeros and <= 63
+  // rows; a panel row longer than 64 per nonzeros is a task of its own
+  const int cap = 64 * per;
+  std::vector<int4> tasks;
+  std::vector<int2> tp;
+  for (int p = 0; p < np && R > 0; ++p) {
+    const int* q = rpa.data() + (size_t)p * (R + 1);
+    const size_t first = tasks.size();
+    int cur = 0;
+    for (int i = 0; i < R; ++i) {
+      if (q[i + 1] - q[i] > cap) {
+        if (i > cur) tasks.push_back(make_int4(cur, i, q[cur], q[i]));
+        tasks.push_back(make_int4(i, i + 1, q[i], q[i + 1]));
+        cur = i + 1;
+      } else if (i > cur && (q[i + 1] - q[cur] > cap || i - cur >= kPanelRowCap)) {
+        tasks.push_back(make_int4(cur, i, q[cur], q[i]));
+        cur = i;
+      }
+    }
+    if (cur < R) tasks.push_back(make_int4(cur, R, q[cur], q[R]));
+    for (size_t k = first; k < tasks.size(); ++k) tp.push_back(make_int2(p, (int)tasks.size()));
+  }
+  P->ntask = (int)tasks.size();
+  P->rp = (int*)dev_alloc(sizeof(int) * rpa.size());
+  P->ci = (uint16_t*)dev_alloc(sizeof(uint16_t) * lci.size());
+  P->src = (int*)dev_alloc(sizeof(int) * src.size());
+  P->task = (int4*)dev_alloc(sizeof(int4) * (tasks.empty() ? 1 : tasks.size()));
+  P->tpanel = (int2*)dev_alloc(sizeof(int2) * (tp.empty() ? 1 : tp.size()));
+  P->val64 = want64 ? (double*)dev_alloc(sizeof(double) * (size_t)(nnz + pad)) : nullptr;
+  P->val32 = want32 ? (float*)dev_alloc(sizeof(float) * (size_t)(nnz + pad)) : nullptr;
+  if (!P->rp || !P->ci || !P->src || !P->task || !P->tpanel || (want64 && !P->val64) || (want32 && !P->val32))
+    return 1;
+  upload(P->rp, rpa.data(), sizeof(int) * rpa.size());
+  upload(P->ci, lci.data(), sizeof(uint16_t) * lci.size());
+  upload(P->src, src.data(), sizeof(int) * src.size());
+  if (!tasks.empty()) {
+    upload(P->task, tasks.data(), sizeof(int4) * tasks.size());
+    upload(P->tpanel, tp.data(), sizeof(int2) * tp.size());
+  }
+  // the padding past nnz is read (and multiplied by zero): keep it finite
+  if (want64) HIPCHK(hipMemsetAsync(P->val64 + nnz, 0, sizeof(double) * (size_t)pad, rt().stream));
+  if (want32) HIPCHK(hipMemsetAsync(P->val32 + nnz, 0, sizeof(float) * (size_t)pad, rt().stream));
+  // two resident workgroups per CU, a few rounds of them
+  const int cus = rt().num_cu > 0 ? rt().num_cu : 256;
+  long long g = 8LL * cus;
+  if (g > P->ntask) g = P->ntask > 0 ? P->ntask : 1;
+  P->grid = (int)g;
+  P->ver = -1;
+  return 0;
+}
+
+void panel_free(PanelMatrix* P) {
+  dev_free(P->rp);
+  dev_free(P->ci);
+  dev_free(P->src);
+  dev_free(P->task);
+  dev_free(P->tpanel);
+  dev_free(P->val64);
+  dev_free(P->val32);
+  memset(P, 0, sizeof(*P));
+}
+
+void panel_refresh_values(PanelMatrix* P, const double* csr_val) {
+  if (P->nnz <= 0) return;
+  const int g = grid_for(P->nnz, kBlock, 16384);
+  hipLaunchKernelGGL(k_panel_gather, dim3(g), dim3(kBlock), 0, rt().stream, P->nnz, (const int*)P->src, csr_val,
+                     P->val64, P->val32);
+  LAUNCHCHK();
+}
+
+void panel_spmv(const PanelMatrix& P, const int* active, const float* x0, int n0, const float* x1, float* partial) {
+  if (P.ntask <= 0) return;
+  if (P.W != kW32 || P.per != kPer32 || !P.val32) {
+    check(hipErrorInvalidValue, "panel_spmv: not an FP32 panel matrix", __FILE__, __LINE__);
+    return;
+  }
+  launch<float, float, kW32, kPer32>(P, active, P.val32, x0, n0, x1, partial);
+  LAUNCHCHK();
+}
+
+void panel_spmv(const PanelMatrix& P, const int* active, const double* x0, int n0, const double* x1,
+                double* partial) {
+  if (P.ntask <= 0) return;
+  if (P.W != kW64 || P.per != kPer64 || !P.val64) {
+    check(hipErrorInvalidValue, "panel_spmv: not an FP64 panel matrix", __FILE__, __LINE__);
+    return;
+  }
+  launch<double, double, kW64, kPer64>(P, active, P.val64, x0, n0, x1, partial);
+  LAUNCHCHK();
+}
+
+int panel_worth(long long nnz, int R, int C, int bits) {
+  const int W = bits == 32 ? kW32 : kW64;
+  const long long np = C > 0 ? (C + W - 1) / W : 1;
+  if (R <= 0 || nnz <= 0) return 0;
+  return np == 1 || nnz >= 4 * np * (long long)R;
+}
+
+double panel_spmv_bytes(const PanelMatrix& P, int value_bytes) {
+  return (double)P.nnz * (value_bytes + 2.0) + (double)P.np * (P.R + 1.0) * 4.0 + (double)P.C * value_bytes +
+         (double)P.np * P.R * value_bytes + (double)P.ntask * 20.0;
+}
+
+}  // namespace osqp_hip

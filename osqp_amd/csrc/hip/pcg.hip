@@ -40,8 +40,11 @@
 #include <string.h>
 #include <time.h>
 
+#include <vector>
+
 #include "hip_internal.h"
 #include "spmv_rb.h"
+#include "panel.h"
 
 using namespace osqp_hip;
 
@@ -202,6 +205,90 @@ __global__ void k_kfill(int n, const int* prp, const int* pci, const double* pva
         kval[o2 + k - t0] = (TV)tval[k];
         if (cols_too) kci[o2 + k - t0] = tci[k] + n;
       }
+    }
+  }
+}
+
+// ---- panel combines: sum the per-panel partials of a row ---------------------
+// A group of G consecutive lanes serves one row: lane j adds the partials of
+// panels [j np / G, (j + 1) np / G) in panel order (loads four at a time), and
+// a fixed xor butterfly adds the G sums (every lane of the group ends with the
+// same bits).  Lane 0 of the group runs the epilogue.  Deterministic.
+template <typename TX, int G>
+__device__ __forceinline__ TX panel_sum(const TX* __restrict__ partial, int np, int R, int row, int j) {
+  const int p0 = (np * j) / G, p1 = (np * (j + 1)) / G;
+  TX s = 0;
+  int p = p0;
+  for (; p + 4 <= p1; p += 4) {
+    const TX a0 = partial[(long long)p * R + row], a1 = partial[(long long)(p + 1) * R + row];
+    const TX a2 = partial[(long long)(p + 2) * R + row], a3 = partial[(long long)(p + 3) * R + row];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
+  }
+  for (; p < p1; ++p) s += partial[(long long)p * R + row];
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) s += __shfl_xor(s, off, G);
+  return s;
+}
+
+// lanes per row for np panels
+inline int comb_group(int np) { return np >= 32 ? 8 : (np >= 8 ? 4 : 1); }
+
+#define COMB_ROWS(i, j, R, G)                                                               \
+  for (long long _t = (long long)blockIdx.x * blockDim.x + threadIdx.x, _stride = (long long)gridDim.x * blockDim.x; \
+       _t < (long long)(R) * (G); _t += _stride)                                           \
+    if (const int i = (int)(_t / (G)), j = (int)(_t % (G)); true)
+
+// w = rho o (A v)
+template <typename TX, int G>
+__global__ void k_comb_av(const PcgState* st, int m, int np, const TX* partial, const double* rho, double rho_sc,
+                          TX* w) {
+  if (st && !st->active) return;
+  COMB_ROWS(i, j, m, G) {
+    const TX s = panel_sum<TX, G>(partial, np, m, i, j);
+    if (j == 0) w[i] = (TX)rho_at(rho, rho_sc, i) * s;
+  }
+}
+
+// Kv = sigma v + K-panels (- sub); partials of v'Kv or of max|Kv - sub|
+template <typename TX, int G>
+__global__ void k_comb_kv(const PcgState* st, int n, int np, const TX* partial, double sigma, const TX* v,
+                          const double* sub, TX* Kv, double* partials) {
+  if (!st->active) return;
+  __shared__ double lds[kBlock / kWave];
+  const TX sg = (TX)sigma;
+  Acc<TX> dot;
+  double mx = 0.0;
+  COMB_ROWS(i, j, n, G) {
+    const TX s = panel_sum<TX, G>(partial, np, n, i, j);
+    if (j == 0) {
+      const TX vi = v[i];
+      TX kv = sg * vi + s;
+      if (sub) {
+        kv -= (TX)sub[i];
+        const double a = fabs((double)kv);
+        mx = a > mx ? a : mx;
+      } else {
+        dot.add(vi * kv);
+      }
+      Kv[i] = kv;
+    }
+  }
+  const double t = sub ? block_max(mx, lds) : block_sum(dot.value(), lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// z = A x (FP64): b[n:n+m] (polishing: rho_pol (A x - b2)) and the A x cache
+template <int G>
+__global__ void k_comb_z(int m, int n, int np, const double* partial, double* b, double* ax, int polishing,
+                         double rho_pol) {
+  COMB_ROWS(i, j, m, G) {
+    const double s = panel_sum<double, G>(partial, np, m, i, j);
+    if (j == 0) {
+      if (ax) ax[i] = s;
+      b[n + i] = polishing ? (s - b[n + i]) * rho_pol : s;
     }
   }
 }
@@ -464,7 +551,25 @@ struct HipPcg {
   double* kval64;
   float* kval32;
   long long kver_P, kver_A;
+  // column-panelled copies of A (m x n) and K (n x (n+m)) for the hot SpMVs:
+  // FP32 ones for the mixed CG, FP64 ones for the residual and z = A x (and
+  // for everything in FP64 mode).  An operator whose panels do not pay
+  // (panel_worth) has none (ntask == 0) and uses the row-block CSR kernels.
+  PanelMatrix pA32, pK32, pA64, pK64;
+  void *partA, *partK;  // per-panel partial sums (sized for the largest use)
 };
+
+// the panel copy of A / K for value type V, or null when CSR kernels are used
+template <typename V>
+const PanelMatrix* panels_A(const HipPcg* s) {
+  const PanelMatrix* P = sizeof(V) == 4 ? &s->pA32 : &s->pA64;
+  return P->ntask > 0 ? P : nullptr;
+}
+template <typename V>
+const PanelMatrix* panels_K(const HipPcg* s) {
+  const PanelMatrix* P = sizeof(V) == 4 ? &s->pK32 : &s->pK64;
+  return P->ntask > 0 ? P : nullptr;
+}
 
 void refresh_fp32(const OSQPMatrix* Mc) {
   OSQPMatrix* M = (OSQPMatrix*)Mc;
@@ -494,6 +599,64 @@ void fill_kvals(HipPcg* s, int cols_too) {
 void refresh_kvals(HipPcg* s) {
   if (s->kver_P == s->P->version && s->kver_A == s->A->version) return;
   fill_kvals(s, 0);
+}
+
+void refresh_panels(HipPcg* s) {
+  PanelMatrix* As[2] = {&s->pA32, &s->pA64};
+  PanelMatrix* Ks[2] = {&s->pK32, &s->pK64};
+  for (PanelMatrix* P : As)
+    if (P->ntask > 0 && P->ver != s->A->version) {
+      panel_refresh_values(P, s->A->val);
+      P->ver = s->A->version;
+    }
+  const long long kv = s->P->version * 1000003LL + s->A->version;
+  for (PanelMatrix* P : Ks)
+    if (P->ntask > 0 && P->ver != kv) {
+      panel_refresh_values(P, s->kval64);
+      P->ver = kv;
+    }
+}
+
+int build_panels(HipPcg* s) {
+  const int n = s->n, m = s->m;
+  const long long nnzA = m ? (long long)s->A->nnz_in : 0;
+  const long long knnz = (long long)s->P->nnz + nnzA;
+  const int prec[2] = {32, 64};
+  bool needK = false, needA = false;
+  for (int bits : prec) {
+    if (bits == 32 && s->bits != 32) continue;
+    needK = needK || panel_worth(knnz, n, n + m, bits);
+    needA = needA || (m && panel_worth(nnzA, m, n, bits));
+  }
+  size_t bytesA = 0, bytesK = 0;
+  if (needK) {
+    std::vector<int> hrp((size_t)n + 1), hci((size_t)(knnz > 0 ? knnz : 1));
+    download(hrp.data(), s->krp, sizeof(int) * hrp.size());
+    if (knnz) download(hci.data(), s->kci, sizeof(int) * (size_t)knnz);
+    for (int bits : prec) {
+      if ((bits == 32 && s->bits != 32) || !panel_worth(knnz, n, n + m, bits)) continue;
+      PanelMatrix* P = bits == 32 ? &s->pK32 : &s->pK64;
+      if (panel_build(P, n, n + m, bits, hrp.data(), hci.data())) return 1;
+      const size_t b = (size_t)P->np * (size_t)n * (bits / 8);
+      bytesK = b > bytesK ? b : bytesK;
+    }
+  }
+  if (needA) {
+    std::vector<int> arp((size_t)m + 1), aci((size_t)(nnzA > 0 ? nnzA : 1));
+    download(arp.data(), s->A->rp, sizeof(int) * arp.size());
+    if (nnzA) download(aci.data(), s->A->ci, sizeof(int) * (size_t)nnzA);
+    for (int bits : prec) {
+      if ((bits == 32 && s->bits != 32) || !panel_worth(nnzA, m, n, bits)) continue;
+      PanelMatrix* P = bits == 32 ? &s->pA32 : &s->pA64;
+      if (panel_build(P, m, n, bits, arp.data(), aci.data())) return 1;
+      const size_t b = (size_t)P->np * (size_t)m * (bits / 8);
+      bytesA = b > bytesA ? b : bytesA;
+    }
+  }
+  if (bytesA && !(s->partA = dev_alloc(bytesA))) return 1;
+  if (bytesK && !(s->partK = dev_alloc(bytesK))) return 1;
+  refresh_panels(s);
+  return 0;
 }
 
 int build_kcsr(HipPcg* s) {
@@ -528,6 +691,7 @@ int build_kcsr(HipPcg* s) {
 void update_precond(HipPcg* s) {
   if (s->bits == 32) refresh_fp32(s->A);
   refresh_kvals(s);
+  refresh_panels(s);
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
   const int G = s->gT;
   int grid = rows_grid(s->n, G, 16384);
@@ -573,22 +737,102 @@ bool host_should_stop(HipPcg* s, int it) {
   }
 }
 
+// ---- operators: panel SpMV + combine, or the row-block CSR kernel ----------
+template <typename V>
+const V* csr_vals_A(HipPcg* s) {
+  return (const V*)(sizeof(V) == 4 ? (const void*)s->A->val32 : (const void*)s->A->val);
+}
+template <typename V>
+const V* csr_vals_K(HipPcg* s) {
+  return (const V*)(sizeof(V) == 4 ? (const void*)s->kval32 : (const void*)s->kval64);
+}
+const int* active_flag(HipPcg* s) { return (const int*)((const char*)s->st + offsetof(PcgState, active)); }
+
+// w = rho o (A v)  (skipped while the CG state is inactive)
+template <typename V>
+void op_av_rho(HipPcg* s, const V* v, V* w) {
+  const int m = s->m;
+  const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
+  if (const PanelMatrix* P = panels_A<V>(s)) {
+    panel_spmv(*P, active_flag(s), v, s->n, (const V*)nullptr, (V*)s->partA);
+    const int G = comb_group(P->np);
+    const dim3 g(grid_for((long long)m * G, kBlock, 8192));
+    if (G == 8)
+      hipLaunchKernelGGL((k_comb_av<V, 8>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                         (const V*)s->partA, rho, s->rho_sc, w);
+    else if (G == 4)
+      hipLaunchKernelGGL((k_comb_av<V, 4>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                         (const V*)s->partA, rho, s->rho_sc, w);
+    else
+      hipLaunchKernelGGL((k_comb_av<V, 1>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                         (const V*)s->partA, rho, s->rho_sc, w);
+  } else {
+    const int g = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
+    hipLaunchKernelGGL((k_rb_Av_rho<V, V>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                       (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci,
+                       csr_vals_A<V>(s), v, rho, s->rho_sc, w);
+  }
+}
+
+// Kv = sigma v + [P A'] [v; w] (- sub) and block partials (v'Kv, or
+// max|Kv - sub| when sub is given); returns the number of partials
+template <typename V>
+int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* parts) {
+  const int n = s->n;
+  if (const PanelMatrix* P = panels_K<V>(s)) {
+    panel_spmv(*P, active_flag(s), v, n, w, (V*)s->partK);
+    const int G = comb_group(P->np);
+    const int g = grid_for((long long)n * G, kBlock, kMaxReduceBlocks);
+    if (G == 8)
+      hipLaunchKernelGGL((k_comb_kv<V, 8>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                         (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+    else if (G == 4)
+      hipLaunchKernelGGL((k_comb_kv<V, 4>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                         (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+    else
+      hipLaunchKernelGGL((k_comb_kv<V, 1>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                         (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+    return g;
+  }
+  const int g = s->k_nb < kMaxReduceBlocks ? s->k_nb : kMaxReduceBlocks;
+  hipLaunchKernelGGL((k_rb_Kv<V, V>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, (const int*)s->k_rows,
+                     s->k_nb, (const int*)s->krp, (const int*)s->kci, csr_vals_K<V>(s), n, s->sigma, v, w, sub, Kv,
+                     parts);
+  return g;
+}
+
+// b[n:n+m] = A x64 (FP64 values), the A x cache; polishing: rho_pol (A x - b2)
+void op_finish_z(HipPcg* s, double* b) {
+  const int n = s->n, m = s->m;
+  if (const PanelMatrix* P = panels_A<double>(s)) {
+    panel_spmv(*P, nullptr, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
+    const int G = comb_group(P->np);
+    const dim3 g(grid_for((long long)m * G, kBlock, 8192));
+    if (G == 8)
+      hipLaunchKernelGGL(k_comb_z<8>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
+                         s->polishing, s->rho_sc);
+    else if (G == 4)
+      hipLaunchKernelGGL(k_comb_z<4>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
+                         s->polishing, s->rho_sc);
+    else
+      hipLaunchKernelGGL(k_comb_z<1>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
+                         s->polishing, s->rho_sc);
+  } else {
+    const int g = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
+    hipLaunchKernelGGL(k_rb_finish_z<double>, dim3(g), dim3(kBlock), 0, S(), (const int*)s->A->rb_rows, s->A->rb_n,
+                       (const int*)s->A->rp, (const int*)s->A->ci, (const double*)s->A->val, (const double*)s->x64,
+                       b, s->ax64, n, s->polishing, s->rho_sc);
+  }
+}
+
 template <typename V>
 void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
-  typedef V TV;  // stored value type equals the CG precision
   const int n = s->n, m = s->m;
-  // CG operator in the CG precision; defect-correction residual and z = A x
-  // with the FP64 values (the exact K)
-  const TV* aval = (const TV*)(s->bits == 32 ? (const void*)s->A->val32 : (const void*)s->A->val);
-  const TV* kval = (const TV*)(s->bits == 32 ? (const void*)s->kval32 : (const void*)s->kval64);
-  const double* aval64 = s->A->val;
-  const double* kval64 = s->kval64;
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
   double* part0 = s->parts;
   double* part1 = s->parts + kMaxReduceBlocks;
   double* part2 = s->parts + 2 * kMaxReduceBlocks;
   const int gK = s->k_nb < kMaxReduceBlocks ? s->k_nb : kMaxReduceBlocks;
-  const int gA = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
   const int gE = grid_for(n, kBlock, kMaxReduceBlocks);
   const int gM = grid_for(m, kBlock, 4096);
   const int guard = s->bits == 32 ? 1 : 0;
@@ -602,48 +846,37 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
 
   s->host_solves++;
   const bool cached = m > 0 && s->ax_valid && s->ax_Aver == s->A->version && admm_iter > 1 && !s->polishing;
+  int nres;
   if (!cached) {
     // explicit rhs: needed for ||rhs|| at the first ADMM iteration and in polishing
     hipLaunchKernelGGL(k_rb_rhs<double>, dim3(gK), dim3(kBlock), 0, S(), (const int*)s->k_rows, s->k_nb,
-                       (const int*)s->krp, (const int*)s->kci, kval64, n, (const double*)b->d, rho, s->rho_sc,
-                       s->rhs64, part0);
+                       (const int*)s->krp, (const int*)s->kci, (const double*)s->kval64, n, (const double*)b->d, rho,
+                       s->rho_sc, s->rhs64, part0);
     hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, gK, admm_iter,
                        *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
                        s->reduction_threshold);
-    if (m)
-      hipLaunchKernelGGL((k_rb_Av_rho<double, double>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci, aval64,
-                         (const double*)s->x64, rho, s->rho_sc, s->w64);
-    // r64 = K x64 - rhs64
-    hipLaunchKernelGGL((k_rb_Kv<double, double>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                       (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci, kval64, n, s->sigma,
-                       (const double*)s->x64, (const double*)s->w64, (const double*)s->rhs64, s->r64, part1);
+    // r64 = K x64 - rhs64 with the FP64 values (the exact K)
+    if (m) op_av_rho<double>(s, (const double*)s->x64, s->w64);
+    nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)s->rhs64, s->r64, part1);
   } else {
     hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)nullptr, 0, admm_iter,
                        *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
                        s->reduction_threshold);
-    // w64 = rho o (A x64 - b2);  r64 = sigma x64 + P x64 + A' w64 - b1
+    // w64 = rho o (A x64 - b2) from the cached A x64;  r64 = sigma x64 + P x64 + A' w64 - b1
     hipLaunchKernelGGL(k_w_from_ax, dim3(gM), dim3(kBlock), 0, S(), (const PcgState*)s->st, m,
                        (const double*)s->ax64, (const double*)(b->d + n), rho, s->rho_sc, s->w64);
-    hipLaunchKernelGGL((k_rb_Kv<double, double>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                       (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci, kval64, n, s->sigma,
-                       (const double*)s->x64, (const double*)s->w64, (const double*)b->d, s->r64, part1);
+    nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)b->d, s->r64, part1);
   }
-  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part1, gK);
+  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part1, nres);
   hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
                      (const double*)s->r64, dinv, r, y, p, d, part0, part1);
   hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
                      gE, s->max_iter, 1, guard, s->d_flags, 0);
   for (int it = 0; it < s->max_iter; ++it) {
     if (host_should_stop(s, it)) break;
-    if (m)
-      hipLaunchKernelGGL((k_rb_Av_rho<TV, V>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci, aval,
-                         (const V*)p, rho, s->rho_sc, w);
-    hipLaunchKernelGGL((k_rb_Kv<TV, V>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                       (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci, kval, n, s->sigma,
-                       (const V*)p, (const V*)w, (const double*)nullptr, Kp, part2);
-    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part2, gK, guard);
+    if (m) op_av_rho<V>(s, (const V*)p, w);
+    const int nk = op_kv<V>(s, (const V*)p, (const V*)w, (const double*)nullptr, Kp, part2);
+    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part2, nk, guard);
     hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, d, r, y,
                        (const V*)p, (const V*)Kp, dinv, part0, part1);
     hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0,
@@ -653,9 +886,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
                      (const V*)d, s->x64, b->d);
   if (m) {
-    hipLaunchKernelGGL(k_rb_finish_z<double>, dim3(gA), dim3(kBlock), 0, S(), (const int*)s->A->rb_rows,
-                       s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci, aval64, (const double*)s->x64, b->d,
-                       s->ax64, n, s->polishing, s->rho_sc);
+    op_finish_z(s, b->d);
     s->ax_valid = 1;
     s->ax_Aver = s->A->version;
   }
@@ -729,6 +960,8 @@ void pcg_free(LinSysSolver* self) {
   dev_free(s->w); dev_free(s->dinv); dev_free(s->dtmp);
   dev_free(s->st); dev_free(s->parts);
   dev_free(s->krp); dev_free(s->kci); dev_free(s->kval64); dev_free(s->kval32); dev_free(s->k_rows);
+  panel_free(&s->pA32); panel_free(&s->pK32); panel_free(&s->pA64); panel_free(&s->pK64);
+  dev_free(s->partA); dev_free(s->partK);
   if (s->h_st) (void)hipHostFree(s->h_st);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
   free(s);
@@ -832,7 +1065,7 @@ OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** sp, const OSQPMatrix* P, 
   s->gT = A->tlanes;
   // K rows hold a P row and an A' row: size lanes for their sum
   s->gK = lanes_for_rows((long long)P->nnz + (long long)A->nnz_in, s->n);
-  if (build_kcsr(s)) {
+  if (build_kcsr(s) || build_panels(s)) {
     pcg_free(&s->base);
     *sp = nullptr;
     return OSQP_LINSYS_SOLVER_INIT_ERROR;
@@ -871,33 +1104,31 @@ OSQPInt osqp_amd_linsys_time_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_
   }
   hipLaunchKernelGGL(k_set_active, dim3(1), dim3(64), 0, S(), s->st, 1);
   HIPCHK(hipStreamSynchronize(S()));
-  const int gK = s->k_nb < kMaxReduceBlocks ? s->k_nb : kMaxReduceBlocks;
-  const int gA = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
-  const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
-  double* part2 = s->parts + 2 * kMaxReduceBlocks;
+  // the SpMV kernels of one CG iteration in the CG precision: the panel
+  // kernel alone where A / K are panelled, else the row-block CSR kernel
+  const PanelMatrix* PA = s->bits == 32 ? panels_A<float>(s) : panels_A<double>(s);
+  const PanelMatrix* PK = s->bits == 32 ? panels_K<float>(s) : panels_K<double>(s);
   HIPCHK(hipEventRecord(e0, S()));
   for (int r = 0; r < reps && m; ++r) {
-    if (s->bits == 32)
-      hipLaunchKernelGGL((k_rb_Av_rho<float, float>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci,
-                         (const float*)s->A->val32, (const float*)s->p, rho, s->rho_sc, (float*)s->w);
+    if (PA && s->bits == 32)
+      panel_spmv(*PA, active_flag(s), (const float*)s->p, n, (const float*)nullptr, (float*)s->partA);
+    else if (PA)
+      panel_spmv(*PA, active_flag(s), (const double*)s->p, n, (const double*)nullptr, (double*)s->partA);
+    else if (s->bits == 32)
+      op_av_rho<float>(s, (const float*)s->p, (float*)s->w);
     else
-      hipLaunchKernelGGL((k_rb_Av_rho<double, double>), dim3(gA), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                         (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci,
-                         (const double*)s->A->val, (const double*)s->p, rho, s->rho_sc, (double*)s->w);
+      op_av_rho<double>(s, (const double*)s->p, (double*)s->w);
   }
   HIPCHK(hipEventRecord(e1, S()));
   for (int r = 0; r < reps; ++r) {
-    if (s->bits == 32)
-      hipLaunchKernelGGL((k_rb_Kv<float, float>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                         (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci,
-                         (const float*)s->kval32, n, s->sigma, (const float*)s->p, (const float*)s->w,
-                         (const double*)nullptr, (float*)s->Kp, part2);
+    if (PK && s->bits == 32)
+      panel_spmv(*PK, active_flag(s), (const float*)s->p, n, (const float*)s->w, (float*)s->partK);
+    else if (PK)
+      panel_spmv(*PK, active_flag(s), (const double*)s->p, n, (const double*)s->w, (double*)s->partK);
+    else if (s->bits == 32)
+      op_kv<float>(s, (const float*)s->p, (const float*)s->w, nullptr, (float*)s->Kp, s->parts);
     else
-      hipLaunchKernelGGL((k_rb_Kv<double, double>), dim3(gK), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                         (const int*)s->k_rows, s->k_nb, (const int*)s->krp, (const int*)s->kci,
-                         (const double*)s->kval64, n, s->sigma, (const double*)s->p, (const double*)s->w,
-                         (const double*)nullptr, (double*)s->Kp, part2);
+      op_kv<double>(s, (const double*)s->p, (const double*)s->w, nullptr, (double*)s->Kp, s->parts);
   }
   HIPCHK(hipEventRecord(e2, S()));
   HIPCHK(hipEventSynchronize(e2));
@@ -906,12 +1137,14 @@ OSQPInt osqp_amd_linsys_time_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_
   HIPCHK(hipEventElapsedTime(&t12, e1, e2));
   if (ms_av) *ms_av = (double)t01 / reps;
   if (ms_kv) *ms_kv = (double)t12 / reps;
-  const double nnzA = (double)s->A->nnz_in, nnzP = (double)s->P->nnz;
-  // per launch: values + indices once, row pointers once, each gathered
-  // vector element once, rho (FP64) once, the output once
-  if (bytes_av) *bytes_av = nnzA * (vb + 4.0) + (m + 1.0) * 4.0 + n * vb + m * 8.0 + m * vb;
+  // bytes per launch.  Panels: values + 16-bit indices, row pointers and one
+  // partial per (panel, row), the multiplied vector, task descriptors.  CSR:
+  // values + 32-bit indices, row pointers, the vector, the output (+ rho).
+  const double nnzA = (double)s->A->nnz_in, nnzK = (double)s->P->nnz + (m ? nnzA : 0.0);
+  if (bytes_av)
+    *bytes_av = PA ? panel_spmv_bytes(*PA, (int)vb) : nnzA * (vb + 4.0) + (m + 1.0) * 4.0 + n * vb + m * (vb + 8.0);
   if (bytes_kv)
-    *bytes_kv = (nnzP + nnzA) * (vb + 4.0) + (n + 1.0) * 4.0 + m * vb + 2.0 * n * vb;
+    *bytes_kv = PK ? panel_spmv_bytes(*PK, (int)vb) : nnzK * (vb + 4.0) + (n + 1.0) * 4.0 + (n + (double)m) * vb + 2.0 * n * vb;
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
   HIPCHK(hipEventDestroy(e2));

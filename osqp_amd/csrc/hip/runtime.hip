@@ -135,6 +135,7 @@ OSQPInt osqp_algebra_init_libs(OSQPInt device) {
   if (HIPCHK(hipMalloc((void**)&g_rt.d_scalars, 64 * sizeof(double)))) return 1;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    g_rt.num_cu = prop.multiProcessorCount;
     snprintf(g_rt.device_name, sizeof(g_rt.device_name), "%s (%s, %d CUs)", prop.name,
              prop.gcnArchName, prop.multiProcessorCount);
   }

@@ -32,12 +32,12 @@ constexpr int kPerThread = kChunk / kBlock;
 // most kChunk rows, or is a single row longer than kChunk.  Integer-exact.
 int build_row_blocks(const int* rp, int rows, int* out /* capacity rows + 1 */);
 
-// Block-level driver.  Gather(c) -> TX value of the multiplied vector at
-// column c; Epi(row, sum) consumes the row total (called by exactly one
-// thread per row).
-template <typename TV, typename TX, typename Gather, typename Epi>
+// Block-level driver.  TI: column index type (int, or uint16_t for panel-local
+// indices).  Gather(c) -> TX value of the multiplied vector at column c;
+// Epi(row, sum) consumes the row total (called by exactly one thread per row).
+template <typename TV, typename TX, typename TI, typename Gather, typename Epi>
 __device__ __forceinline__ void rb_spmv_block(int rb, int re, const int* __restrict__ rp,
-                                              const int* __restrict__ ci, const TV* __restrict__ val,
+                                              const TI* __restrict__ ci, const TV* __restrict__ val,
                                               TX* __restrict__ prod, Gather gather, Epi epi) {
   const int tid = threadIdx.x;
   const int k0 = rp[rb];
@@ -51,7 +51,7 @@ __device__ __forceinline__ void rb_spmv_block(int rb, int re, const int* __restr
 #pragma unroll
       for (int j = 0; j < kPerThread; ++j) {
         const int k = base + j * kBlock + tid;
-        if (k < k1) acc += (TX)ld_stream(val + k) * gather(ld_stream(ci + k));
+        if (k < k1) acc += (TX)ld_stream(val + k) * gather((int)ld_stream(ci + k));
       }
     }
     acc = wave_sum(acc);
@@ -72,7 +72,7 @@ __device__ __forceinline__ void rb_spmv_block(int rb, int re, const int* __restr
   for (int j = 0; j < kPerThread; ++j) {
     const int e = j * kBlock + tid;
     const bool ok = e < len;
-    c[j] = ok ? ld_stream(ci + k0 + e) : 0;
+    c[j] = ok ? (int)ld_stream(ci + k0 + e) : 0;
     v[j] = ok ? ld_stream(val + k0 + e) : (TV)0;
   }
 #pragma unroll

@@ -186,6 +186,28 @@ def test_random_qp_vs_oracle(seed, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+def test_multi_panel_qp(precision):
+    """A QP wide enough for the column-panelled SpMV to use several panels in
+    both precisions: A is 30000 x 9000 with 30 nonzeros per row, K = [P A']
+    spans 39000 columns (5 FP32 panels of 8192, 10 FP64 panels of 4096; A
+    takes 2 and 3).  Checked against the oracle's PCG path (same algorithm,
+    FP64) and by the FP64 KKT certificate of the returned point."""
+    p = pr.random_qp(9000, 30000, 2.0 / 9000, 30.0 / 9000, seed=7)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    o = OracleSolver()
+    fx = dict(FIXTURE)
+    fx.update(st)
+    assert o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=2, **fx) == 0
+    o.solve()
+    s, r = gpu_solve(p, precision, **st)
+    assert r.info.status_val == o.info.status_val == 1
+    assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * abs(o.info.obj_val)
+    c = pr.kkt_certificate(p, r.x, r.y)
+    assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+    assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("N", [10, 30])
 def test_mpc_vs_oracle(N, precision):
     p = pr.mpc(N, seed=N)
@@ -292,11 +314,24 @@ def test_update_matrices():
 
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_long_rows_qp(precision):
-    """A QP whose constraint matrix has rows longer than one SpMV row block
-    (a dense budget row over 6000 variables, as in the portfolio model)."""
+    """A QP whose constraint matrix has rows longer than one SpMV task (a
+    dense budget row over 6000 variables and 60 half-dense factor rows, as in
+    the portfolio model).
+
+    The PCG path's duality gap stalls just above its tolerance on this
+    problem: the oracle's restatement of the reference PCG (linsys_solver=2,
+    FP64) ends "solved inaccurate" at 4000 iterations with |gap| = 2.8e-4 >
+    2.3e-4, while the direct solver converges at 2100.  So the engine must
+    return "solved" or "solved inaccurate", the direct oracle's objective to
+    1e-3, and a point that meets the eps test on the primal/dual residuals."""
     p = pr.portfolio(6000, seed=4)
     st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
     o = oracle_solve(p, **st)
     s, r = gpu_solve(p, precision, **st)
-    assert r.info.status_val == o.info.status_val == 1
+    assert o.info.status_val == 1
+    assert r.info.status_val in (1, 2), r.info.status
     assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * max(1.0, abs(o.info.obj_val))
+    c = pr.kkt_certificate(p, r.x, r.y)
+    assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+    assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+
