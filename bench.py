@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="set up the QP, launch only the timed SpMV kernels (reps=20 each, A then K) and a "
+                         "256 MiB device copy (counter calibration); for rocprofv3 runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (one MI355X per job); "
@@ -68,6 +71,16 @@ def main():
     assert rc == 0, f"setup failed: {rc}"
     torch.cuda.synchronize()
     log(f"[bench] FP32-PCG setup {time.time() - t0:.2f}s on {device_name()}")
+    if args.profile_kernels:
+        kt = eng.time_kernels(reps=20)
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernels": kt, "copy_bytes_each_way": a.numel() * 4}), flush=True)
+        eng.close()
+        return
     for w in range(args.warmup):
         eng.cold_start()
         r = eng.solve()

@@ -1,0 +1,109 @@
+/*
+ * osqp_amd_serve.h — batched serving engine for many small QPs on one GPU.
+ *
+ * No counterpart in the reference: OSQP solves one problem per solver object
+ * (include/public/osqp_api_functions.h osqp_setup / osqp_solve).  This engine
+ * keeps many problems resident in HBM and advances all of them together:
+ *
+ *   - a problem is added with the same data osqp_setup takes (CSC P upper
+ *     triangle, q, CSC A, l, u) and is validated the same way; it is admitted
+ *     as soon as a slot and device memory are free, then scaled, set up and
+ *     solved on the GPU by one workgroup, with the host core's ADMM semantics
+ *     and the PCG linear solver (FP32 CG with FP64 defect correction, or FP64);
+ *   - osqp_batch_step runs one epoch: every active problem advances up to a
+ *     fixed number of ADMM iterations in one persistent kernel per size bucket
+ *     (one hipGraph each); a workgroup that finishes a problem takes the next
+ *     one in the same launch; problems added meanwhile join the next epoch;
+ *   - a resident problem keeps its iterates and rho after a solve; a new q
+ *     (osqp_update_data_vec semantics) and a warm-started re-solve are
+ *     requested with osqp_batch_resolve.
+ *
+ * A problem's results do not depend on the other problems in flight or on the
+ * engine's tuning (team sizes): its arithmetic is fixed by its own data.
+ *
+ * Settings: those of OSQPSettings, with rho_is_vec = 1, polishing = 0 and
+ * adaptive_rho in {disabled, iterations}; time_limit is not applied.
+ * Calls on one engine must not overlap (one host thread drives an engine).
+ */
+#ifndef OSQP_AMD_SERVE_H
+#define OSQP_AMD_SERVE_H
+
+#include "osqp_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OSQPBatch OSQPBatch;
+
+typedef struct {
+  OSQPInt   status_val;
+  OSQPInt   iter;
+  OSQPInt   rho_updates;
+  OSQPFloat obj_val;
+  OSQPFloat dual_obj_val;
+  OSQPFloat prim_res;
+  OSQPFloat dual_res;
+  OSQPFloat duality_gap;
+  OSQPFloat rho_estimate;
+  OSQPFloat primdual_int;
+  OSQPFloat rel_kkt_error;
+  OSQPFloat latency;      /* seconds from add / resolve to the result seen by the host */
+  long long cg_iters;     /* CG iterations of this problem since it was added */
+} OSQPBatchInfo;
+
+/* precision: 32 (FP32 CG, FP64 residuals and ADMM state) or 64.
+ * max_active: problems on the GPU at once (0: 4096).
+ * arena_bytes: device memory for resident problems (0: 4 GiB).
+ * Returns 0, OSQP_SETTINGS_VALIDATION_ERROR or OSQP_MEM_ALLOC_ERROR. */
+OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* settings, OSQPInt precision, OSQPInt max_active,
+                          long long arena_bytes);
+
+/* Queue a problem (setup + first solve).  x0 / y0: optional warm start.
+ * resident = 1 keeps it on the GPU after the solve for osqp_batch_resolve;
+ * resident = 0 frees it once its result is polled.
+ * Returns 0 with *id set, or OSQP_DATA_VALIDATION_ERROR. */
+OSQPInt osqp_batch_add(OSQPBatch* batch, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
+                       const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0,
+                       OSQPInt resident, OSQPInt* id);
+
+/* Queue a warm-started re-solve of a resident problem whose last solve has
+ * ended and been polled; q_new (may be null) replaces q first.  Returns 0, or
+ * -1 when the problem is unknown, not resident or busy. */
+OSQPInt osqp_batch_resolve(OSQPBatch* batch, OSQPInt id, const OSQPFloat* q_new);
+
+/* One epoch; *completed (may be null) = solves that ended in it.  Returns 0,
+ * or OSQP_MEM_ALLOC_ERROR when a HIP call failed. */
+OSQPInt osqp_batch_step(OSQPBatch* batch, OSQPInt* completed);
+
+/* 1 and the result (x: n, y: m, either may be null) when the problem's solve
+ * has ended; 0 while it runs; -1 for an unknown id.  A non-resident problem is
+ * released by a successful poll. */
+OSQPInt osqp_batch_poll(OSQPBatch* batch, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OSQPBatchInfo* info);
+
+/* Drop a problem that is not running.  Returns 0 or -1. */
+OSQPInt osqp_batch_release(OSQPBatch* batch, OSQPInt id);
+
+/* Solves queued or running. */
+OSQPInt osqp_batch_pending(const OSQPBatch* batch);
+
+/* Tuned configuration of a size bucket (team = threads per workgroup,
+ * teams_per_cu = resident workgroups per CU); 0, or -1 if not tuned yet. */
+OSQPInt osqp_batch_tuning(const OSQPBatch* batch, OSQPInt bucket, OSQPInt* team, OSQPInt* teams_per_cu);
+
+/* Pin a size bucket to one configuration (team in {64, 128, 256, 512},
+ * teams_per_cu >= 1), which also stops the tuner for that bucket.  Results do
+ * not change; this exists for tests and for callers that tune offline.
+ * Returns 0 or -1. */
+OSQPInt osqp_batch_set_team(OSQPBatch* batch, OSQPInt bucket, OSQPInt team, OSQPInt teams_per_cu);
+
+/* Engine counters: epochs run, ADMM iterations, GPU time of the epochs (s). */
+OSQPInt osqp_batch_counters(const OSQPBatch* batch, long long* epochs, long long* admm_iters, OSQPFloat* gpu_s);
+
+OSQPInt osqp_batch_destroy(OSQPBatch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OSQP_AMD_SERVE_H */

@@ -69,8 +69,9 @@ void panel_refresh_values(PanelMatrix* P, const double* csr_val);
 void panel_spmv(const PanelMatrix& P, const int* active, const float* x0, int n0, const float* x1, float* partial);
 void panel_spmv(const PanelMatrix& P, const int* active, const double* x0, int n0, const double* x1,
                 double* partial);
-// bytes one panel_spmv launch moves at minimum (values, indices, row
-// pointers, x once, partials once)
+// bytes one panel_spmv launch moves at minimum: values + 16-bit indices,
+// np (R + 1) row pointers, x once, one partial per (panel, row), 24-byte task
+// descriptors
 double panel_spmv_bytes(const PanelMatrix& P, int value_bytes);
 
 }  // namespace osqp_hip

@@ -287,7 +287,7 @@ int panel_worth(long long nnz, int R, int C, int bits) {
 
 double panel_spmv_bytes(const PanelMatrix& P, int value_bytes) {
   return (double)P.nnz * (value_bytes + 2.0) + (double)P.np * (P.R + 1.0) * 4.0 + (double)P.C * value_bytes +
-         (double)P.np * P.R * value_bytes + (double)P.ntask * 20.0;
+         (double)P.np * P.R * value_bytes + (double)P.ntask * 24.0;
 }
 
 }  // namespace osqp_hip

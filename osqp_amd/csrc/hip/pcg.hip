@@ -233,8 +233,12 @@ __device__ __forceinline__ TX panel_sum(const TX* __restrict__ partial, int np, 
   return s;
 }
 
-// lanes per row for np panels
-inline int comb_group(int np) { return np >= 32 ? 8 : (np >= 8 ? 4 : 1); }
+// lanes per row: enough for about 256K threads, at least 4 panels per lane
+inline int comb_group(int np, int rows) {
+  int G = 1;
+  while (G < 8 && (long long)rows * G < 262144 && np >= 4 * G) G *= 2;
+  return G;
+}
 
 #define COMB_ROWS(i, j, R, G)                                                               \
   for (long long _t = (long long)blockIdx.x * blockDim.x + threadIdx.x, _stride = (long long)gridDim.x * blockDim.x; \
@@ -755,13 +759,16 @@ void op_av_rho(HipPcg* s, const V* v, V* w) {
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
   if (const PanelMatrix* P = panels_A<V>(s)) {
     panel_spmv(*P, active_flag(s), v, s->n, (const V*)nullptr, (V*)s->partA);
-    const int G = comb_group(P->np);
+    const int G = comb_group(P->np, m);
     const dim3 g(grid_for((long long)m * G, kBlock, 8192));
     if (G == 8)
       hipLaunchKernelGGL((k_comb_av<V, 8>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
                          (const V*)s->partA, rho, s->rho_sc, w);
     else if (G == 4)
       hipLaunchKernelGGL((k_comb_av<V, 4>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                         (const V*)s->partA, rho, s->rho_sc, w);
+    else if (G == 2)
+      hipLaunchKernelGGL((k_comb_av<V, 2>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
                          (const V*)s->partA, rho, s->rho_sc, w);
     else
       hipLaunchKernelGGL((k_comb_av<V, 1>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
@@ -781,13 +788,16 @@ int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* p
   const int n = s->n;
   if (const PanelMatrix* P = panels_K<V>(s)) {
     panel_spmv(*P, active_flag(s), v, n, w, (V*)s->partK);
-    const int G = comb_group(P->np);
+    const int G = comb_group(P->np, n);
     const int g = grid_for((long long)n * G, kBlock, kMaxReduceBlocks);
     if (G == 8)
       hipLaunchKernelGGL((k_comb_kv<V, 8>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
                          (const V*)s->partK, s->sigma, v, sub, Kv, parts);
     else if (G == 4)
       hipLaunchKernelGGL((k_comb_kv<V, 4>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                         (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+    else if (G == 2)
+      hipLaunchKernelGGL((k_comb_kv<V, 2>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
                          (const V*)s->partK, s->sigma, v, sub, Kv, parts);
     else
       hipLaunchKernelGGL((k_comb_kv<V, 1>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
@@ -806,13 +816,16 @@ void op_finish_z(HipPcg* s, double* b) {
   const int n = s->n, m = s->m;
   if (const PanelMatrix* P = panels_A<double>(s)) {
     panel_spmv(*P, nullptr, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
-    const int G = comb_group(P->np);
+    const int G = comb_group(P->np, m);
     const dim3 g(grid_for((long long)m * G, kBlock, 8192));
     if (G == 8)
       hipLaunchKernelGGL(k_comb_z<8>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
                          s->polishing, s->rho_sc);
     else if (G == 4)
       hipLaunchKernelGGL(k_comb_z<4>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
+                         s->polishing, s->rho_sc);
+    else if (G == 2)
+      hipLaunchKernelGGL(k_comb_z<2>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
                          s->polishing, s->rho_sc);
     else
       hipLaunchKernelGGL(k_comb_z<1>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,

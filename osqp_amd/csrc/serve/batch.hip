@@ -1,0 +1,1003 @@
+// batch.hip — batched serving engine (include/osqp_amd_serve.h).
+//
+// Device side: one persistent kernel per size bucket and epoch.  Its
+// workgroups take problems from the epoch's table with an atomic counter; a
+// workgroup runs its problem's pending command (setup, or a new solve) and then
+// up to `iters` ADMM iterations (team.h), writes the problem's state back to
+// its HBM block and takes the next problem, until the table is exhausted.
+//
+// Host side: problems are copied at osqp_batch_add, converted to CSR (A, A',
+// P with both triangles), and admitted when a slot, an HBM block of the device
+// arena and pinned staging memory are free.  Each epoch the host writes every
+// bucket's table into pinned memory, replays the bucket's hipGraph (one kernel
+// node; grid and team size fixed per graph), waits, and collects the results
+// the kernel wrote to pinned memory.  An online tuner tries team sizes and
+// teams per CU on the first epochs of every bucket and keeps the fastest
+// (ADMM iterations per second); results do not depend on that choice.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <vector>
+
+#include "osqp_amd_serve.h"
+#include "team.h"
+
+extern "C" OSQPInt osqp_algebra_init_libs(OSQPInt device);
+extern "C" void osqp_algebra_free_libs(void);
+
+namespace osqp_serve {
+
+// ---- epoch kernel --------------------------------------------------------------
+struct BEntry {
+  BProb* dev;           // the problem's descriptor in HBM
+  const BProb* init;    // host-mapped descriptor of a new problem (cmd 1)
+  const double* upd_q;  // cmd 2: new q (raw) or null
+  int cmd;              // 0 continue, 1 setup + solve, 2 new solve
+  int iters_done;       // written by the kernel
+};
+
+struct BTable {
+  int count;  // entries
+  int epoch;  // selects the work counter
+  int iters;  // ADMM iterations per problem and epoch
+  int pad;
+  BEntry e[1];
+};
+
+constexpr int kCounterRing = 1024;
+
+template <typename V>
+__device__ void run_entry(const Team& t, BProb& P, const BEntry& E, int iters, int* done_iters) {
+  Solver<V> sv(t, P);
+  if (E.cmd == 1) sv.setup();
+  else if (E.cmd == 2) sv.new_solve(E.upd_q);
+  int k = 0;
+  if (sv.S.phase == kPhaseRun) {
+    for (; k < iters;) {
+      ++k;
+      if (sv.iterate()) {
+        sv.finish();
+        break;
+      }
+    }
+  }
+  t.sync();
+  if (t.tid == 0) {
+    P.st = sv.S;
+    *done_iters = k;
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(1024) void k_epoch(BTable* tab, int* counters) {
+  __shared__ double red[kMaxRedBlocks];
+  __shared__ BProb P;
+  __shared__ int slot, done_iters;
+  Team t;
+  t.tid = threadIdx.x;
+  t.T = blockDim.x;
+  t.lane = threadIdx.x & (kWave - 1);
+  t.wv = threadIdx.x / kWave;
+  t.W = blockDim.x / kWave;
+  t.red = red;
+  const int ep = tab->epoch;
+  const int count = tab->count;
+  const int iters = tab->iters;
+  int* ctr = counters + (ep % kCounterRing);
+  // the next epoch's counter starts at zero (the next launch follows this one
+  // on the same stream)
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[(ep + 1) % kCounterRing] = 0;
+  for (;;) {
+    if (threadIdx.x == 0) slot = atomicAdd(ctr, 1);
+    __syncthreads();
+    const int s = slot;
+    __syncthreads();
+    if (s >= count) break;
+    const BEntry E = tab->e[s];
+    const BProb* src = E.cmd == 1 ? E.init : E.dev;
+    const int* sw = (const int*)src;
+    int* dw = (int*)&P;
+    for (int i = threadIdx.x; i < (int)(sizeof(BProb) / 4); i += blockDim.x) dw[i] = sw[i];
+    __syncthreads();
+    run_entry<V>(t, P, E, iters, &done_iters);
+    __syncthreads();
+    int* gw = (int*)E.dev;
+    for (int i = threadIdx.x; i < (int)(sizeof(BProb) / 4); i += blockDim.x) gw[i] = dw[i];
+    if (threadIdx.x == 0) tab->e[s].iters_done = done_iters;
+    __syncthreads();
+  }
+}
+
+// ---- host side ------------------------------------------------------------------
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// first-fit allocator over [0, size) with coalescing
+class RangeAlloc {
+ public:
+  void reset(size_t size) {
+    free_.clear();
+    if (size) free_[0] = size;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = align_up(bytes ? bytes : 1, 256);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes, start = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[start] = rest;
+        used_[*off] = bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off) {
+    auto u = used_.find(off);
+    if (u == used_.end()) return;
+    size_t start = off, len = u->second;
+    used_.erase(u);
+    auto next = free_.lower_bound(start);
+    if (next != free_.end() && start + len == next->first) {
+      len += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == start) {
+        start = prev->first;
+        len += prev->second;
+        free_.erase(prev);
+      }
+    }
+    free_[start] = len;
+  }
+
+ private:
+  std::map<size_t, size_t> free_, used_;
+};
+
+// host copy of a problem in CSR form (values raw, unscaled)
+struct HostData {
+  int n = 0, m = 0;
+  std::vector<int> Prp, Pci, Pdiag, Arp, Aci, Trp, Tci, Plong, Along, Tlong;
+  std::vector<double> Pval, Aval, Tval, q, l, u, x0, y0;
+  int LP = 1, LA = 1, LT = 1;
+};
+
+int lane_group(const std::vector<int>& rp, int rows, std::vector<int>* longs) {
+  long long nnz_short = 0;
+  int rows_short = 0;
+  longs->clear();
+  for (int r = 0; r < rows; ++r) {
+    const int len = rp[r + 1] - rp[r];
+    if (len > kLongRow) {
+      longs->push_back(r);
+    } else {
+      nnz_short += len;
+      rows_short++;
+    }
+  }
+  const double avg = rows_short ? (double)nnz_short / rows_short : 1.0;
+  int L = 1;
+  while (L < 64 && L < avg) L <<= 1;
+  return L;
+}
+
+// A (CSC, m x n) -> A CSR and A' CSR (= the CSC arrays); P (CSC upper
+// triangle) -> full symmetric CSR with sorted columns and diagonal positions
+void build_host(HostData* h, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
+                const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0) {
+  const int n = P->n, m = A->m;
+  h->n = n;
+  h->m = m;
+  // A' = CSC of A
+  const int nnzA = A->p[n];
+  h->Trp.assign(A->p, A->p + n + 1);
+  h->Tci.assign(A->i, A->i + nnzA);
+  h->Tval.assign(A->x, A->x + nnzA);
+  // A CSR (counting sort by row; columns ascending)
+  h->Arp.assign((size_t)m + 1, 0);
+  for (int k = 0; k < nnzA; ++k) h->Arp[(size_t)A->i[k] + 1]++;
+  for (int r = 0; r < m; ++r) h->Arp[(size_t)r + 1] += h->Arp[r];
+  h->Aci.resize((size_t)nnzA);
+  h->Aval.resize((size_t)nnzA);
+  {
+    std::vector<int> next(h->Arp.begin(), h->Arp.end() - 1);
+    for (int j = 0; j < n; ++j)
+      for (int k = A->p[j]; k < A->p[j + 1]; ++k) {
+        const int pos = next[A->i[k]]++;
+        h->Aci[pos] = j;
+        h->Aval[pos] = A->x[k];
+      }
+  }
+  // P full: row r = column r of the triangle (rows i <= r) then the entries
+  // (r, c) of the triangle's columns c > r, ascending
+  const int nnzPt = P->p[n];
+  std::vector<int> cnt((size_t)n, 0);
+  for (int c = 0; c < n; ++c)
+    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
+      const int r = P->i[k];
+      cnt[c]++;
+      if (r != c) cnt[r]++;
+    }
+  h->Prp.assign((size_t)n + 1, 0);
+  for (int r = 0; r < n; ++r) h->Prp[(size_t)r + 1] = h->Prp[r] + cnt[r];
+  const int nnzP = h->Prp[n];
+  h->Pci.resize((size_t)nnzP);
+  h->Pval.resize((size_t)nnzP);
+  h->Pdiag.assign((size_t)n, -1);
+  std::vector<int> next(h->Prp.begin(), h->Prp.end() - 1);
+  for (int c = 0; c < n; ++c)  // lower part of row c: column c of the triangle (entries (i, c), i <= c)
+    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
+      const int pos = next[c]++;
+      h->Pci[pos] = P->i[k];
+      h->Pval[pos] = P->x[k];
+      if (P->i[k] == c) h->Pdiag[c] = pos;
+    }
+  for (int c = 0; c < n; ++c)  // upper part of row r: (r, c) for c > r
+    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
+      const int r = P->i[k];
+      if (r == c) continue;
+      const int pos = next[r]++;
+      h->Pci[pos] = c;
+      h->Pval[pos] = P->x[k];
+    }
+  // a triangle column's rows are not required to be sorted: sort each row
+  for (int r = 0; r < n; ++r) {
+    const int s = h->Prp[r], e = h->Prp[(size_t)r + 1];
+    bool sorted = true;
+    for (int k = s + 1; k < e && sorted; ++k) sorted = h->Pci[k - 1] < h->Pci[k];
+    if (!sorted) {
+      std::vector<std::pair<int, double>> row;
+      for (int k = s; k < e; ++k) row.emplace_back(h->Pci[k], h->Pval[k]);
+      std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+        return a.first < b.first;
+      });
+      for (int k = s; k < e; ++k) {
+        h->Pci[k] = row[k - s].first;
+        h->Pval[k] = row[k - s].second;
+        if (h->Pci[k] == r) h->Pdiag[r] = k;
+      }
+    }
+  }
+  (void)nnzPt;
+  h->q.assign(q, q + n);
+  h->l.assign(l, l + m);
+  h->u.assign(u, u + m);
+  if (x0) h->x0.assign(x0, x0 + n);
+  if (y0) h->y0.assign(y0, y0 + m);
+  h->LP = lane_group(h->Prp, n, &h->Plong);
+  h->LA = lane_group(h->Arp, m, &h->Along);
+  h->LT = lane_group(h->Trp, n, &h->Tlong);
+}
+
+// byte layout of a problem's HBM block and of its pinned images
+struct Layout {
+  size_t desc;  // BProb at offset 0
+  // static image (copied from the pinned stage at setup): offsets from block start
+  size_t s_begin, s_end;
+  size_t Prp, Pci, Pval, Pdiag, Plong, Arp, Aci, Aval, Along, Trp, Tci, Tval, Tlong, q, l, u;
+  // dynamic
+  size_t ctype, D, Dinv, E, Einv, rho, rhoinv, xa, xb, za, zb, y, xt, zt, dx, dy, Ax, Px, Aty, tn, tm, tn2;
+  size_t x64, r64, rhs64, w64, ax64, prec, cd, cp, cKp, cr, cy, cw, cdinv;
+  size_t total;
+  // pinned: init descriptor + static image + warm start; results (+ q buffer)
+  size_t st_total, st_static, st_x0, st_y0;
+  size_t rs_total, rs_x, rs_y, rs_res, rs_q;
+};
+
+Layout make_layout(const HostData& h, int vbytes) {
+  Layout L{};
+  const size_t n = h.n, m = h.m;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    o = align_up(o, 64);
+    const size_t at = o;
+    o += bytes ? bytes : 8;
+    return at;
+  };
+  L.desc = take(sizeof(BProb));
+  o = align_up(o, 64);
+  L.s_begin = o;
+  L.Prp = take(4 * (n + 1));
+  L.Pci = take(4 * h.Pci.size());
+  L.Pval = take(8 * h.Pval.size());
+  L.Pdiag = take(4 * n);
+  L.Plong = take(4 * h.Plong.size());
+  L.Arp = take(4 * (m + 1));
+  L.Aci = take(4 * h.Aci.size());
+  L.Aval = take(8 * h.Aval.size());
+  L.Along = take(4 * h.Along.size());
+  L.Trp = take(4 * (n + 1));
+  L.Tci = take(4 * h.Tci.size());
+  L.Tval = take(8 * h.Tval.size());
+  L.Tlong = take(4 * h.Tlong.size());
+  L.q = take(8 * n);
+  L.l = take(8 * m);
+  L.u = take(8 * m);
+  o = align_up(o, 64);
+  L.s_end = o;
+  L.ctype = take(4 * m);
+  L.D = take(8 * n);
+  L.Dinv = take(8 * n);
+  L.E = take(8 * m);
+  L.Einv = take(8 * m);
+  L.rho = take(8 * m);
+  L.rhoinv = take(8 * m);
+  L.xa = take(8 * n);
+  L.xb = take(8 * n);
+  L.za = take(8 * m);
+  L.zb = take(8 * m);
+  L.y = take(8 * m);
+  L.xt = take(8 * n);
+  L.zt = take(8 * m);
+  L.dx = take(8 * n);
+  L.dy = take(8 * m);
+  L.Ax = take(8 * m);
+  L.Px = take(8 * n);
+  L.Aty = take(8 * n);
+  L.tn = take(8 * n);
+  L.tm = take(8 * m);
+  L.tn2 = take(8 * n);
+  L.x64 = take(8 * n);
+  L.r64 = take(8 * n);
+  L.rhs64 = take(8 * n);
+  L.w64 = take(8 * m);
+  L.ax64 = take(8 * m);
+  L.prec = take(8 * n);
+  L.cd = take(vbytes * n);
+  L.cp = take(vbytes * n);
+  L.cKp = take(vbytes * n);
+  L.cr = take(vbytes * n);
+  L.cy = take(vbytes * n);
+  L.cw = take(vbytes * m);
+  L.cdinv = take(vbytes * n);
+  L.total = align_up(o, 256);
+  // pinned stage: [init descriptor][static image][x0][y0]
+  size_t s = align_up(sizeof(BProb), 64);
+  L.st_static = s;
+  s += L.s_end - L.s_begin;
+  s = align_up(s, 64);
+  L.st_x0 = s;
+  s += h.x0.empty() ? 0 : 8 * n;
+  s = align_up(s, 64);
+  L.st_y0 = s;
+  s += h.y0.empty() ? 0 : 8 * m;
+  L.st_total = align_up(s, 256);
+  // pinned results: [x][y][BResult][q buffer]
+  size_t r = 0;
+  L.rs_x = r;
+  r = align_up(r + 8 * (n ? n : 1), 64);
+  L.rs_y = r;
+  r = align_up(r + 8 * (m ? m : 1), 64);
+  L.rs_res = r;
+  r = align_up(r + sizeof(BResult), 64);
+  L.rs_q = r;
+  r = align_up(r + 8 * (n ? n : 1), 64);
+  L.rs_total = align_up(r, 256);
+  return L;
+}
+
+enum : int { kQueued = 0, kActive = 1, kEnded = 2, kIdle = 3 };
+
+struct HProb {
+  int id = 0, bucket = 0, state = kQueued, cmd = 1, resident = 0;
+  int qnew = 0;  // a resolve brought a new q
+  HostData h;
+  Layout L{};
+  bool has_dev = false, has_stage = false, has_res = false;
+  size_t dev_off = 0, stage_off = 0, res_off = 0;
+  double t_req = 0.0, latency = 0.0;
+  long long iters_since_add = 0;
+};
+
+constexpr int kBuckets = 6;
+constexpr int kEpochIters = 25;
+
+int bucket_of(const HostData& h) {
+  const long long w = (long long)h.n + h.m + (long long)h.Aci.size() + (long long)h.Pci.size();
+  if (w <= (1 << 12)) return 0;
+  if (w <= (1 << 14)) return 1;
+  if (w <= (1 << 16)) return 2;
+  if (w <= (1 << 18)) return 3;
+  if (w <= (1 << 20)) return 4;
+  return 5;
+}
+
+struct Cand {
+  int T, tpc;
+};
+const Cand kCands[] = {{64, 16}, {128, 8}, {256, 4}, {256, 8}, {512, 2}, {512, 4}};
+constexpr int kNumCands = sizeof(kCands) / sizeof(kCands[0]);
+
+struct Bucket {
+  std::vector<int> active;
+  BTable* tab = nullptr;  // pinned
+  size_t tab_cap = 0;
+  int* counters = nullptr;  // device
+  int epoch = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int g_T = 0, g_tpc = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // tuner
+  int trial = 0;  // next candidate to try; kNumCands when tuned
+  double rate[kNumCands] = {0};
+  int best = 2;   // 256 x 4 until tuned
+  int fixT = 0, fixTpc = 0;  // pinned configuration (osqp_batch_set_team)
+};
+
+}  // namespace
+
+struct Engine {
+  BSettings set{};
+  int bits = 32;
+  int max_active = 4096;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  char* arena = nullptr;
+  size_t arena_size = 0;
+  RangeAlloc arena_alloc;
+  char* pool = nullptr;  // pinned
+  size_t pool_size = 0;
+  RangeAlloc pool_alloc;
+  std::map<int, HProb*> probs;
+  std::vector<int> queue;  // waiting for admission, in arrival order
+  int next_id = 1;
+  int n_active = 0;
+  Bucket buckets[kBuckets];
+  long long epochs = 0, admm_iters = 0;
+  double gpu_s = 0.0;
+  bool failed = false;
+};
+
+namespace {
+
+bool ok(Engine* E, hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  fprintf(stderr, "osqp_batch: %s: %s\n", what, hipGetErrorString(e));
+  E->failed = true;
+  return false;
+}
+
+void free_problem_memory(Engine* E, HProb* p, bool dev, bool stage, bool res) {
+  if (dev && p->has_dev) {
+    E->arena_alloc.release(p->dev_off);
+    p->has_dev = false;
+  }
+  if (stage && p->has_stage) {
+    E->pool_alloc.release(p->stage_off);
+    p->has_stage = false;
+  }
+  if (res && p->has_res) {
+    E->pool_alloc.release(p->res_off);
+    p->has_res = false;
+  }
+}
+
+template <typename T>
+void put(char* base, size_t off, const std::vector<T>& v) {
+  if (!v.empty()) memcpy(base + off, v.data(), sizeof(T) * v.size());
+}
+
+// reserve memory and write the pinned images of a queued problem
+bool admit(Engine* E, HProb* p) {
+  const HostData& h = p->h;
+  const Layout& L = p->L;
+  size_t d, s, r;
+  if (!E->arena_alloc.alloc(L.total, &d)) return false;
+  if (!E->pool_alloc.alloc(L.st_total, &s)) {
+    E->arena_alloc.release(d);
+    return false;
+  }
+  if (!E->pool_alloc.alloc(L.rs_total, &r)) {
+    E->arena_alloc.release(d);
+    E->pool_alloc.release(s);
+    return false;
+  }
+  p->dev_off = d;
+  p->stage_off = s;
+  p->res_off = r;
+  p->has_dev = p->has_stage = p->has_res = true;
+  char* dev = E->arena + d;
+  char* st = E->pool + s;
+  char* rs = E->pool + r;
+  // static image, laid out as in the HBM block (offsets from s_begin)
+  char* img = st + L.st_static;
+  const size_t b = L.s_begin;
+  memset(img, 0, L.s_end - L.s_begin);
+  put(img, L.Prp - b, h.Prp);
+  put(img, L.Pci - b, h.Pci);
+  put(img, L.Pval - b, h.Pval);
+  put(img, L.Pdiag - b, h.Pdiag);
+  put(img, L.Plong - b, h.Plong);
+  put(img, L.Arp - b, h.Arp);
+  put(img, L.Aci - b, h.Aci);
+  put(img, L.Aval - b, h.Aval);
+  put(img, L.Along - b, h.Along);
+  put(img, L.Trp - b, h.Trp);
+  put(img, L.Tci - b, h.Tci);
+  put(img, L.Tval - b, h.Tval);
+  put(img, L.Tlong - b, h.Tlong);
+  put(img, L.q - b, h.q);
+  put(img, L.l - b, h.l);
+  put(img, L.u - b, h.u);
+  if (!h.x0.empty()) put(st, L.st_x0, h.x0);
+  if (!h.y0.empty()) put(st, L.st_y0, h.y0);
+  // descriptor
+  BProb* P = (BProb*)st;
+  memset(P, 0, sizeof(BProb));
+  P->n = h.n;
+  P->m = h.m;
+  P->set = E->set;
+  P->st.phase = kPhaseSetup;
+  P->st.rho = E->set.rho0;
+  auto csr = [&](BCsr* M, int rows, int Lg, const std::vector<int>& longs, size_t rp, size_t ci, size_t val,
+                 size_t lg) {
+    M->rows = rows;
+    M->L = Lg;
+    M->nlong = (int)longs.size();
+    M->rp = (int*)(dev + rp);
+    M->ci = (int*)(dev + ci);
+    M->val = (double*)(dev + val);
+    M->longs = (int*)(dev + lg);
+  };
+  csr(&P->P, h.n, h.LP, h.Plong, L.Prp, L.Pci, L.Pval, L.Plong);
+  csr(&P->A, h.m, h.LA, h.Along, L.Arp, L.Aci, L.Aval, L.Along);
+  csr(&P->At, h.n, h.LT, h.Tlong, L.Trp, L.Tci, L.Tval, L.Tlong);
+  P->Pdiag = (int*)(dev + L.Pdiag);
+  P->ctype = (int*)(dev + L.ctype);
+#define DPTR(f) P->f = (double*)(dev + L.f)
+  DPTR(q);
+  DPTR(l);
+  DPTR(u);
+  DPTR(D);
+  DPTR(Dinv);
+  DPTR(E);
+  DPTR(Einv);
+  DPTR(rho);
+  DPTR(rhoinv);
+  DPTR(xa);
+  DPTR(xb);
+  DPTR(za);
+  DPTR(zb);
+  DPTR(y);
+  DPTR(xt);
+  DPTR(zt);
+  DPTR(dx);
+  DPTR(dy);
+  DPTR(Ax);
+  DPTR(Px);
+  DPTR(Aty);
+  DPTR(tn);
+  DPTR(tm);
+  DPTR(tn2);
+  DPTR(x64);
+  DPTR(r64);
+  DPTR(rhs64);
+  DPTR(w64);
+  DPTR(ax64);
+  DPTR(prec);
+#undef DPTR
+  P->cd = dev + L.cd;
+  P->cp = dev + L.cp;
+  P->cKp = dev + L.cKp;
+  P->cr = dev + L.cr;
+  P->cy = dev + L.cy;
+  P->cw = dev + L.cw;
+  P->cdinv = dev + L.cdinv;
+  P->stage = img;
+  P->stage_bytes = (long long)(L.s_end - L.s_begin);
+  P->arena_static = dev + L.s_begin;
+  P->upd_q = nullptr;
+  P->warm_x = h.x0.empty() ? nullptr : (const double*)(st + L.st_x0);
+  P->warm_y = h.y0.empty() ? nullptr : (const double*)(st + L.st_y0);
+  P->out_x = (double*)(rs + L.rs_x);
+  P->out_y = (double*)(rs + L.rs_y);
+  P->out = (BResult*)(rs + L.rs_res);
+  memset(P->out, 0, sizeof(BResult));
+  p->cmd = 1;
+  return true;
+}
+
+bool ensure_table(Engine* E, Bucket& B, size_t count) {
+  if (B.tab && B.tab_cap >= count) return true;
+  if (B.tab) {
+    if (!ok(E, hipHostFree(B.tab), "hipHostFree")) return false;
+    B.tab = nullptr;
+  }
+  size_t cap = std::max<size_t>(count, 256);
+  cap = std::max<size_t>(cap, (size_t)E->max_active);
+  if (!ok(E, hipHostMalloc((void**)&B.tab, sizeof(BTable) + sizeof(BEntry) * cap, hipHostMallocDefault),
+          "hipHostMalloc table"))
+    return false;
+  B.tab_cap = cap;
+  // the graph holds the table address
+  if (B.exec) {
+    hipGraphExecDestroy(B.exec);
+    B.exec = nullptr;
+  }
+  if (B.graph) {
+    hipGraphDestroy(B.graph);
+    B.graph = nullptr;
+  }
+  return true;
+}
+
+bool ensure_graph(Engine* E, Bucket& B, int T, int tpc) {
+  if (B.exec && B.g_T == T && B.g_tpc == tpc) return true;
+  if (B.exec) hipGraphExecDestroy(B.exec);
+  if (B.graph) hipGraphDestroy(B.graph);
+  B.exec = nullptr;
+  B.graph = nullptr;
+  if (!ok(E, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
+  const dim3 grid((unsigned)(tpc * E->cus)), block((unsigned)T);
+  if (E->bits == 32)
+    hipLaunchKernelGGL(k_epoch<float>, grid, block, 0, E->stream, B.tab, B.counters);
+  else
+    hipLaunchKernelGGL(k_epoch<double>, grid, block, 0, E->stream, B.tab, B.counters);
+  if (!ok(E, hipStreamEndCapture(E->stream, &B.graph), "end capture")) return false;
+  if (!ok(E, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "graph instantiate")) return false;
+  B.g_T = T;
+  B.g_tpc = tpc;
+  return true;
+}
+
+}  // namespace
+}  // namespace osqp_serve
+
+using namespace osqp_serve;
+
+struct OSQPBatch {
+  Engine e;
+};
+
+extern "C" {
+
+OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt precision, OSQPInt max_active,
+                          long long arena_bytes) {
+  if (!batch || !s) return OSQP_SETTINGS_VALIDATION_ERROR;
+  *batch = nullptr;
+  if (precision != 32 && precision != 64) return OSQP_SETTINGS_VALIDATION_ERROR;
+  if (s->rho_is_vec != 1 || s->polishing != 0) {
+    fprintf(stderr, "osqp_batch: rho_is_vec = 1 and polishing = 0 are required\n");
+    return OSQP_SETTINGS_VALIDATION_ERROR;
+  }
+  if (s->adaptive_rho != OSQP_ADAPTIVE_RHO_UPDATE_DISABLED && s->adaptive_rho != OSQP_ADAPTIVE_RHO_UPDATE_ITERATIONS) {
+    fprintf(stderr, "osqp_batch: adaptive_rho must be disabled or iteration-based\n");
+    return OSQP_SETTINGS_VALIDATION_ERROR;
+  }
+  if (s->rho <= 0 || s->sigma <= 0 || s->alpha <= 0 || s->alpha >= 2 || s->max_iter <= 0 || s->eps_abs < 0 ||
+      s->eps_rel < 0 || (s->eps_abs == 0 && s->eps_rel == 0) || s->eps_prim_inf <= 0 || s->eps_dual_inf <= 0 ||
+      s->scaling < 0 || s->check_termination < 0 || s->cg_max_iter <= 0 || s->cg_tol_reduction <= 0 ||
+      s->cg_tol_fraction <= 0 || s->cg_tol_fraction >= 1 || s->adaptive_rho_tolerance < 1.0 ||
+      s->adaptive_rho_interval < 0) {
+    fprintf(stderr, "osqp_batch: invalid settings\n");
+    return OSQP_SETTINGS_VALIDATION_ERROR;
+  }
+  if (osqp_algebra_init_libs(s->device)) return OSQP_MEM_ALLOC_ERROR;
+  OSQPBatch* b = new OSQPBatch();
+  Engine* E = &b->e;
+  BSettings& B = E->set;
+  B.rho0 = s->rho;
+  B.sigma = s->sigma;
+  B.alpha = s->alpha;
+  B.eps_abs = s->eps_abs;
+  B.eps_rel = s->eps_rel;
+  B.eps_pinf = s->eps_prim_inf;
+  B.eps_dinf = s->eps_dual_inf;
+  B.ad_tol = s->adaptive_rho_tolerance;
+  B.cg_tol_fraction = s->cg_tol_fraction;
+  B.max_iter = s->max_iter;
+  B.scaling = s->scaling;
+  B.check_term = s->check_termination;
+  B.adaptive = s->adaptive_rho == OSQP_ADAPTIVE_RHO_UPDATE_ITERATIONS;
+  B.ad_interval = s->adaptive_rho_interval;
+  if (B.adaptive && B.ad_interval == 0)  // osqp_setup's default (osqp_api.c:615-620)
+    B.ad_interval = s->check_termination ? OSQP_ADAPTIVE_RHO_MULTIPLE_TERMINATION * s->check_termination
+                                         : OSQP_ADAPTIVE_RHO_FIXED;
+  B.scaled_term = s->scaled_termination;
+  B.check_gap = s->check_dualgap;
+  B.warm = s->warm_starting;
+  B.cg_max_iter = s->cg_max_iter;
+  B.cg_red_thresh = s->cg_tol_reduction;
+  E->bits = (int)precision;
+  E->max_active = max_active > 0 ? (int)max_active : 4096;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, s->device) == hipSuccess && prop.multiProcessorCount > 0)
+    E->cus = prop.multiProcessorCount;
+  bool good = ok(E, hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking), "stream");
+  E->arena_size = arena_bytes > 0 ? (size_t)arena_bytes : ((size_t)4 << 30);
+  good = good && ok(E, hipMalloc((void**)&E->arena, E->arena_size), "hipMalloc arena");
+  E->pool_size = std::max<size_t>(E->arena_size / 4, (size_t)64 << 20);
+  E->pool_size = std::min<size_t>(E->pool_size, (size_t)1 << 30);
+  good = good && ok(E, hipHostMalloc((void**)&E->pool, E->pool_size, hipHostMallocDefault), "hipHostMalloc pool");
+  for (int k = 0; k < kBuckets && good; ++k) {
+    Bucket& Bk = E->buckets[k];
+    good = ok(E, hipMalloc((void**)&Bk.counters, sizeof(int) * kCounterRing), "hipMalloc counters") &&
+           ok(E, hipMemset(Bk.counters, 0, sizeof(int) * kCounterRing), "hipMemset counters") &&
+           ok(E, hipEventCreate(&Bk.e0), "event") && ok(E, hipEventCreate(&Bk.e1), "event");
+  }
+  if (!good) {
+    osqp_batch_destroy(b);
+    return OSQP_MEM_ALLOC_ERROR;
+  }
+  E->arena_alloc.reset(E->arena_size);
+  E->pool_alloc.reset(E->pool_size);
+  *batch = b;
+  return 0;
+}
+
+OSQPInt osqp_batch_add(OSQPBatch* b, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
+                       const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0,
+                       OSQPInt resident, OSQPInt* id) {
+  if (!b || !P || !A || !q || !id) return OSQP_DATA_VALIDATION_ERROR;
+  const int n = P->n, m = A->m;
+  // src/auxil.c validate_data
+  if (n <= 0 || m < 0 || P->m != n || A->n != n) return OSQP_DATA_VALIDATION_ERROR;
+  if (m > 0 && (!l || !u)) return OSQP_DATA_VALIDATION_ERROR;
+  if (n > kMaxRedBlocks * kRedBlock || m > kMaxRedBlocks * kRedBlock) {
+    fprintf(stderr, "osqp_batch: problem too large for the batched engine (n, m <= %d)\n", kMaxRedBlocks * kRedBlock);
+    return OSQP_DATA_VALIDATION_ERROR;
+  }
+  for (int j = 0; j < n; ++j)
+    for (int k = P->p[j]; k < P->p[j + 1]; ++k)
+      if (P->i[k] > j || P->i[k] < 0) return OSQP_DATA_VALIDATION_ERROR;
+  for (int j = 0; j < n; ++j)
+    for (int k = A->p[j]; k < A->p[j + 1]; ++k)
+      if (A->i[k] < 0 || A->i[k] >= m) return OSQP_DATA_VALIDATION_ERROR;
+  for (int i = 0; i < m; ++i)
+    if (l[i] > u[i]) return OSQP_DATA_VALIDATION_ERROR;
+  Engine* E = &b->e;
+  HProb* p = new HProb();
+  p->id = E->next_id++;
+  p->resident = resident ? 1 : 0;
+  build_host(&p->h, P, q, A, l, u, x0, y0);
+  p->L = make_layout(p->h, E->bits / 8);
+  p->bucket = bucket_of(p->h);
+  p->state = kQueued;
+  p->t_req = now_s();
+  E->probs[p->id] = p;
+  E->queue.push_back(p->id);
+  *id = p->id;
+  return 0;
+}
+
+OSQPInt osqp_batch_resolve(OSQPBatch* b, OSQPInt id, const OSQPFloat* q_new) {
+  if (!b) return -1;
+  Engine* E = &b->e;
+  auto it = E->probs.find(id);
+  if (it == E->probs.end()) return -1;
+  HProb* p = it->second;
+  if (!p->resident || p->state != kIdle || !p->has_dev || !p->has_res) return -1;
+  char* rs = E->pool + p->res_off;
+  BResult* out = (BResult*)(rs + p->L.rs_res);
+  memset(out, 0, sizeof(BResult));
+  double* qb = (double*)(rs + p->L.rs_q);
+  if (q_new) memcpy(qb, q_new, sizeof(double) * (size_t)p->h.n);
+  p->cmd = 2;
+  p->qnew = q_new ? 1 : 0;
+  p->state = kActive;
+  p->t_req = now_s();
+  E->buckets[p->bucket].active.push_back(p->id);
+  E->n_active++;
+  return 0;
+}
+
+OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
+  if (completed) *completed = 0;
+  if (!b) return OSQP_MEM_ALLOC_ERROR;
+  Engine* E = &b->e;
+  if (E->failed) return OSQP_MEM_ALLOC_ERROR;
+  // admission in arrival order while slots and memory allow
+  size_t kept = 0;
+  for (size_t k = 0; k < E->queue.size(); ++k) {
+    HProb* p = E->probs[E->queue[k]];
+    if (E->n_active < E->max_active && admit(E, p)) {
+      p->state = kActive;
+      E->buckets[p->bucket].active.push_back(p->id);
+      E->n_active++;
+    } else {
+      E->queue[kept++] = E->queue[k];
+    }
+  }
+  E->queue.resize(kept);
+  // launch every bucket with work
+  bool launched[kBuckets] = {false};
+  for (int k = 0; k < kBuckets; ++k) {
+    Bucket& B = E->buckets[k];
+    if (B.active.empty()) continue;
+    if (!ensure_table(E, B, B.active.size())) return OSQP_MEM_ALLOC_ERROR;
+    B.tab->count = (int)B.active.size();
+    B.tab->epoch = ++B.epoch;
+    B.tab->iters = kEpochIters;
+    for (size_t i = 0; i < B.active.size(); ++i) {
+      HProb* p = E->probs[B.active[i]];
+      BEntry& en = B.tab->e[i];
+      en.dev = (BProb*)(E->arena + p->dev_off);
+      en.init = p->cmd == 1 ? (const BProb*)(E->pool + p->stage_off) : nullptr;
+      en.cmd = p->cmd;
+      en.iters_done = 0;
+      en.upd_q = nullptr;
+      if (p->cmd == 2 && p->qnew) en.upd_q = (const double*)(E->pool + p->res_off + p->L.rs_q);
+    }
+    const int c = B.trial < kNumCands ? B.trial : B.best;
+    const int T = B.fixT ? B.fixT : kCands[c].T, tpc = B.fixT ? B.fixTpc : kCands[c].tpc;
+    if (!ensure_graph(E, B, T, tpc)) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipEventRecord(B.e0, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipGraphLaunch(B.exec, E->stream), "graph launch")) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipEventRecord(B.e1, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
+    launched[k] = true;
+  }
+  if (!ok(E, hipStreamSynchronize(E->stream), "epoch")) return OSQP_MEM_ALLOC_ERROR;
+  const double t_now = now_s();
+  int done = 0;
+  for (int k = 0; k < kBuckets; ++k) {
+    if (!launched[k]) continue;
+    Bucket& B = E->buckets[k];
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, B.e0, B.e1);
+    long long its = 0;
+    std::vector<int> still;
+    still.reserve(B.active.size());
+    for (size_t i = 0; i < B.active.size(); ++i) {
+      HProb* p = E->probs[B.active[i]];
+      its += B.tab->e[i].iters_done;
+      p->iters_since_add += B.tab->e[i].iters_done;
+      if (p->cmd == 1) {
+        // setup done: the stage and the host copy are no longer read
+        free_problem_memory(E, p, false, true, false);
+        HostData keep;
+        keep.n = p->h.n;
+        keep.m = p->h.m;
+        p->h = std::move(keep);
+      }
+      p->cmd = 0;
+      const BResult* out = (const BResult*)(E->pool + p->res_off + p->L.rs_res);
+      if (out->done) {
+        p->state = kEnded;
+        p->latency = t_now - p->t_req;
+        E->n_active--;
+        done++;
+      } else {
+        still.push_back(p->id);
+      }
+    }
+    B.active.swap(still);
+    E->admm_iters += its;
+    E->gpu_s += ms * 1e-3;
+    // online tuner: iterations per second of each candidate on a full epoch
+    if (B.trial < kNumCands) {
+      B.rate[B.trial] = ms > 0 ? its / (double)ms : 0.0;
+      B.trial++;
+      if (B.trial == kNumCands) {
+        int best = 0;
+        for (int c = 1; c < kNumCands; ++c)
+          if (B.rate[c] > B.rate[best]) best = c;
+        B.best = best;
+      }
+    }
+  }
+  E->epochs++;
+  if (completed) *completed = done;
+  return 0;
+}
+
+OSQPInt osqp_batch_poll(OSQPBatch* b, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OSQPBatchInfo* info) {
+  if (!b) return -1;
+  Engine* E = &b->e;
+  auto it = E->probs.find(id);
+  if (it == E->probs.end()) return -1;
+  HProb* p = it->second;
+  if (p->state != kEnded) return p->state == kIdle ? 1 : 0;
+  const char* rs = E->pool + p->res_off;
+  const BResult* out = (const BResult*)(rs + p->L.rs_res);
+  if (x) memcpy(x, rs + p->L.rs_x, sizeof(double) * (size_t)p->h.n);
+  if (y && p->h.m) memcpy(y, rs + p->L.rs_y, sizeof(double) * (size_t)p->h.m);
+  if (info) {
+    info->status_val = out->status;
+    info->iter = out->iter;
+    info->rho_updates = out->rho_updates;
+    info->obj_val = out->obj_val;
+    info->dual_obj_val = out->dual_obj_val;
+    info->prim_res = out->prim_res;
+    info->dual_res = out->dual_res;
+    info->duality_gap = out->gap;
+    info->rho_estimate = out->rho_estimate;
+    info->primdual_int = out->primdual_int;
+    info->rel_kkt_error = out->rel_kkt;
+    info->latency = p->latency;
+    info->cg_iters = out->cg_iters;
+  }
+  if (p->resident) {
+    p->state = kIdle;
+  } else {
+    free_problem_memory(E, p, true, true, true);
+    E->probs.erase(it);
+    delete p;
+  }
+  return 1;
+}
+
+OSQPInt osqp_batch_release(OSQPBatch* b, OSQPInt id) {
+  if (!b) return -1;
+  Engine* E = &b->e;
+  auto it = E->probs.find(id);
+  if (it == E->probs.end()) return -1;
+  HProb* p = it->second;
+  if (p->state == kActive) return -1;
+  if (p->state == kQueued) E->queue.erase(std::remove(E->queue.begin(), E->queue.end(), p->id), E->queue.end());
+  free_problem_memory(E, p, true, true, true);
+  E->probs.erase(it);
+  delete p;
+  return 0;
+}
+
+OSQPInt osqp_batch_pending(const OSQPBatch* b) {
+  if (!b) return 0;
+  return (OSQPInt)(b->e.queue.size() + (size_t)b->e.n_active);
+}
+
+OSQPInt osqp_batch_tuning(const OSQPBatch* b, OSQPInt bucket, OSQPInt* team, OSQPInt* tpc) {
+  if (!b || bucket < 0 || bucket >= kBuckets) return -1;
+  const Bucket& B = b->e.buckets[bucket];
+  if (B.trial < kNumCands) return -1;
+  if (team) *team = B.fixT ? B.fixT : kCands[B.best].T;
+  if (tpc) *tpc = B.fixT ? B.fixTpc : kCands[B.best].tpc;
+  return 0;
+}
+
+OSQPInt osqp_batch_set_team(OSQPBatch* b, OSQPInt bucket, OSQPInt team, OSQPInt tpc) {
+  if (!b || bucket < 0 || bucket >= kBuckets || tpc < 1 || tpc > 64) return -1;
+  if (team != 64 && team != 128 && team != 256 && team != 512) return -1;
+  Bucket& B = b->e.buckets[bucket];
+  B.fixT = (int)team;
+  B.fixTpc = (int)tpc;
+  B.trial = kNumCands;
+  return 0;
+}
+
+OSQPInt osqp_batch_counters(const OSQPBatch* b, long long* epochs, long long* iters, OSQPFloat* gpu_s) {
+  if (!b) return -1;
+  if (epochs) *epochs = b->e.epochs;
+  if (iters) *iters = b->e.admm_iters;
+  if (gpu_s) *gpu_s = b->e.gpu_s;
+  return 0;
+}
+
+OSQPInt osqp_batch_destroy(OSQPBatch* b) {
+  if (!b) return 0;
+  Engine* E = &b->e;
+  if (E->stream) hipStreamSynchronize(E->stream);
+  for (auto& kv : E->probs) delete kv.second;
+  E->probs.clear();
+  for (int k = 0; k < kBuckets; ++k) {
+    Bucket& B = E->buckets[k];
+    if (B.exec) hipGraphExecDestroy(B.exec);
+    if (B.graph) hipGraphDestroy(B.graph);
+    if (B.tab) hipHostFree(B.tab);
+    if (B.counters) hipFree(B.counters);
+    if (B.e0) hipEventDestroy(B.e0);
+    if (B.e1) hipEventDestroy(B.e1);
+  }
+  if (E->arena) hipFree(E->arena);
+  if (E->pool) hipHostFree(E->pool);
+  if (E->stream) hipStreamDestroy(E->stream);
+  delete b;
+  osqp_algebra_free_libs();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1020 @@
+// team.h — one OSQP problem solved by one workgroup (a "team").
+//
+// The batched engine keeps many small problems resident in HBM and runs one
+// team per problem.  Everything here is device code that restates, for one
+// problem held by one workgroup, the host core's ADMM step (src/osqp_api.c
+// osqp_solve loop :698-903, src/auxil.c, src/scaling.c) and the PCG linear
+// solver of the single-problem engine (tolerance schedule
+// algebra/cuda/lin_sys/indirect/cuda_pcg_interface.cu:32-64, CG
+// cuda_pcg.cu:113-208, with FP64 defect correction around an FP32 or FP64 CG).
+//
+// Results do not depend on the team size (threads per workgroup) or on the
+// other problems in flight:
+//   * a CSR row is reduced by one group of L lanes (L fixed per matrix from its
+//     row lengths) or, for rows longer than 64, by one wavefront, in a fixed
+//     order, whatever group or wavefront takes the row;
+//   * a sum / max over a vector is formed over fixed 256-element blocks (each
+//     block by one wavefront: 4 elements per lane in index order, then a fixed
+//     butterfly) and the block results are added in block order.
+// So the engine's tuner can change team sizes freely, and a problem's bits are
+// the same alone and in any mix.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#include "../hip/hip_internal.h"
+
+namespace osqp_serve {
+
+using osqp_hip::kWave;
+
+constexpr int kRedBlock = 256;     // elements per reduction block
+constexpr int kMaxRedBlocks = 512; // vectors up to 131072 entries
+constexpr int kLongRow = 64;       // rows longer than this go to whole wavefronts
+constexpr int kMaxTeamWaves = 16;
+
+// phases of a resident problem
+enum : int { kPhaseIdle = 0, kPhaseSetup = 1, kPhaseUpdate = 2, kPhaseRun = 3, kPhaseDone = 4 };
+
+// status values (include/public/osqp_api_constants.h)
+enum : int {
+  kSolved = 1,
+  kSolvedInaccurate = 2,
+  kPrimalInfeasible = 3,
+  kPrimalInfeasibleInaccurate = 4,
+  kDualInfeasible = 5,
+  kDualInfeasibleInaccurate = 6,
+  kMaxIterReached = 7,
+  kNonCvx = 9,
+  kUnsolved = 11
+};
+
+constexpr double kInfty = 1e30;
+constexpr double kMinScaling = 1e-4;
+constexpr double kMaxScaling = 1e4;
+constexpr double kDivisionTol = 1.0 / kInfty;
+constexpr double kRhoMin = 1e-6;
+constexpr double kRhoMax = 1e6;
+constexpr double kRhoTol = 1e-4;
+constexpr double kRhoEqOverIneq = 1e3;
+constexpr double kZeroDeadzone = 1e-15;
+constexpr double kCgTolMin = 1e-7;
+
+// CSR matrix with its lane-group size and long-row list
+struct BCsr {
+  int rows;
+  int L;        // lanes per short row (power of two, 1..64)
+  int nlong;    // rows longer than kLongRow
+  int* rp;
+  int* ci;
+  double* val;
+  int* longs;
+};
+
+// scalar state that survives between epochs (stored with the problem)
+struct BState {
+  int phase;
+  int iter;          // ADMM iterations of the current solve
+  int swapped;       // parity of the x/x_prev and z/z_prev buffer swaps
+  int status;
+  int rho_updates;
+  int can_check;     // the last iteration ran a termination check
+  int ax_valid;      // ax64 == A x64
+  int cg_iter;       // CG iterations of the last linear solve
+  int cg_zero_iters; // consecutive linear solves with no CG iteration
+  int cg_solves;
+  long long cg_total;
+  double cg_eps_prev, cg_red_factor;
+  double c, cinv;    // cost scaling
+  double rho;        // current scalar rho
+  double obj_val, dual_obj_val, prim_res, dual_res, gap, primdual_int, rel_kkt, rho_estimate;
+  double scaled_prim_res, scaled_dual_res, scaled_gap, xtPx, qtx, SC, last_rel_kkt;
+};
+
+// settings copied into every problem
+struct BSettings {
+  double rho0, sigma, alpha, eps_abs, eps_rel, eps_pinf, eps_dinf;
+  double ad_tol, cg_tol_fraction;
+  int max_iter, scaling, check_term, adaptive, ad_interval, scaled_term, check_gap, warm;
+  int cg_max_iter, cg_red_thresh;
+};
+
+// result record written to pinned host memory when a solve ends
+struct BResult {
+  int status, iter, rho_updates, done;
+  double obj_val, dual_obj_val, prim_res, dual_res, gap, rho_estimate, primdual_int, rel_kkt;
+  long long cg_iters;
+};
+
+// one resident problem: device pointers into its arena block
+struct BProb {
+  int n, m;
+  BSettings set;
+  BState st;
+  BCsr P, A, At;   // P full (both triangles), A, A' (= A's CSC)
+  int* Pdiag;      // position of P's diagonal in each row, -1 if none
+  int* ctype;      // constraint type: -1 loose, 0 inequality, 1 equality
+  double *q, *l, *u, *D, *Dinv, *E, *Einv, *rho, *rhoinv;
+  double *xa, *xb, *za, *zb, *y, *xt, *zt, *dx, *dy, *Ax, *Px, *Aty, *tn, *tm, *tn2;
+  double *x64, *r64, *rhs64, *w64, *ax64, *prec;
+  void *cd, *cp, *cKp, *cr, *cy, *cw, *cdinv;  // CG vectors (FP32 or FP64)
+  // host-mapped staging (read in the setup / update phases) and results
+  const char* stage;     // static data image, same layout as the arena prefix
+  long long stage_bytes;
+  char* arena_static;    // destination of the static image
+  const double* upd_q;   // new q for an update request (raw, unscaled) or null
+  const double* warm_x;  // warm start (raw) or null
+  const double* warm_y;
+  double* out_x;         // results (pinned host)
+  double* out_y;
+  BResult* out;
+};
+
+// ---- team primitives -------------------------------------------------------
+struct Team {
+  int tid, T, lane, wv, W;
+  double* red;  // LDS, kMaxRedBlocks doubles
+  __device__ void sync() const { __syncthreads(); }
+};
+
+__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
+__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+
+template <typename V>
+struct KahanAcc {
+  V s = 0, c = 0;
+  __device__ void add(V v) {
+    const V y = v - c;
+    const V t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ double value() const { return (double)s - (double)c; }
+};
+template <>
+struct KahanAcc<double> {
+  double s = 0;
+  __device__ void add(double v) { s += v; }
+  __device__ double value() const { return s; }
+};
+
+// sum (MAX = false) or max (MAX = true, of non-negative terms) of f(i), i < n;
+// block-structured, independent of the team size.  Ends with every thread
+// holding the result; contains two barriers.
+template <bool MAX, typename F>
+__device__ double team_reduce(const Team& t, int n, F f) {
+  const int nb = (n + kRedBlock - 1) / kRedBlock;
+  for (int b = t.wv; b < nb; b += t.W) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kRedBlock / kWave; ++j) {
+      const int i = b * kRedBlock + j * kWave + t.lane;
+      if (i < n) {
+        const double v = f(i);
+        acc = MAX ? dmax(acc, v) : acc + v;
+      }
+    }
+    acc = MAX ? osqp_hip::wave_max(acc) : osqp_hip::wave_sum(acc);
+    if (t.lane == 0) t.red[b] = acc;
+  }
+  t.sync();
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s = MAX ? dmax(s, t.red[b]) : s + t.red[b];
+  t.sync();
+  return s;
+}
+
+// dot product in the CG precision V (Kahan-compensated in FP32), same blocking
+template <typename V, typename F>
+__device__ double team_dot(const Team& t, int n, F f) {
+  const int nb = (n + kRedBlock - 1) / kRedBlock;
+  for (int b = t.wv; b < nb; b += t.W) {
+    KahanAcc<V> acc;
+#pragma unroll
+    for (int j = 0; j < kRedBlock / kWave; ++j) {
+      const int i = b * kRedBlock + j * kWave + t.lane;
+      if (i < n) acc.add(f(i));
+    }
+    const double v = osqp_hip::wave_sum(acc.value());
+    if (t.lane == 0) t.red[b] = v;
+  }
+  t.sync();
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += t.red[b];
+  t.sync();
+  return s;
+}
+
+// y-rows of a CSR product: for each row r, epi(r, sum_k elem(k)) with the row
+// reduced by L lanes (short rows) or a wavefront (long rows).  Caller syncs.
+template <typename TX, typename Elem, typename Epi>
+__device__ void team_rows(const Team& t, const BCsr& M, Elem elem, Epi epi) {
+  const int L = M.L;
+  const int G = t.T / L;
+  const int g = t.tid / L, gl = t.tid % L;
+  for (int r0 = 0; r0 < M.rows; r0 += G) {
+    const int r = r0 + g;
+    const bool own = r < M.rows;
+    int s = 0, e = 0;
+    if (own) {
+      s = M.rp[r];
+      e = M.rp[r + 1];
+    }
+    const bool lng = (e - s) > kLongRow;
+    TX acc = 0;
+    if (own && !lng)
+      for (int k = s + gl; k < e; k += L) acc += elem(k);
+    for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+    if (own && !lng && gl == 0) epi(r, acc);
+  }
+  for (int j = t.wv; j < M.nlong; j += t.W) {
+    const int r = M.longs[j];
+    const int s = M.rp[r], e = M.rp[r + 1];
+    TX acc = 0;
+    for (int k = s + t.lane; k < e; k += kWave) acc += elem(k);
+    acc = osqp_hip::wave_sum(acc);
+    if (t.lane == 0) epi(r, acc);
+  }
+}
+
+// y = M x (FP64); caller syncs before reading y
+__device__ inline void team_spmv(const Team& t, const BCsr& M, const double* x, double* y) {
+  team_rows<double>(
+      t, M, [&](int k) { return M.val[k] * x[M.ci[k]]; }, [&](int r, double s) { y[r] = s; });
+}
+
+// ---- the ADMM step of one problem -------------------------------------------
+// The problem descriptor is shared by the team (LDS, read-only here); the
+// scalar state S is a private copy in every thread, updated identically by all
+// of them from team-wide reductions (no shared read-modify-write).
+template <typename V>
+struct Solver {
+  const Team& t;
+  const BProb& p;
+  BState S;
+  __device__ Solver(const Team& tt, const BProb& pp) : t(tt), p(pp), S(pp.st) {}
+
+  __device__ double* x() { return S.swapped ? p.xb : p.xa; }
+  __device__ double* xprev() { return S.swapped ? p.xa : p.xb; }
+  __device__ double* z() { return S.swapped ? p.zb : p.za; }
+  __device__ double* zprev() { return S.swapped ? p.za : p.zb; }
+  __device__ bool unscaled_term() const { return p.set.scaling && !p.set.scaled_term; }
+
+  __device__ double norm_inf(const double* v, int n) {
+    return team_reduce<true>(t, n, [&](int i) { return fabs(v[i]); });
+  }
+  __device__ double scaled_norm_inf(const double* s, const double* v, int n) {
+    return team_reduce<true>(t, n, [&](int i) { return fabs(s[i] * v[i]); });
+  }
+  __device__ double dot(const double* a, const double* b, int n) {
+    return team_reduce<false>(t, n, [&](int i) { return a[i] * b[i]; });
+  }
+
+  // ---- scaling (src/scaling.c:48-164) ----------------------------------------
+  __device__ static double limit_scalar(double v) {
+    v = v < kMinScaling ? 1.0 : v;
+    return v > kMaxScaling ? kMaxScaling : v;
+  }
+  __device__ static double limit_elem(double v) {
+    v = v < kMinScaling ? 1.0 : v;
+    return v > kMaxScaling ? kMaxScaling : v;
+  }
+  // P entry (row r, col c) scaled as the triangular storage does:
+  // (v * s[min]) * s[max]
+  __device__ void scale_P(const double* s) {
+    for (int r = t.tid; r < p.n; r += t.T)
+      for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) {
+        const int c = p.P.ci[k];
+        const int lo = c < r ? c : r, hi = c < r ? r : c;
+        p.P.val[k] = (p.P.val[k] * s[lo]) * s[hi];
+      }
+  }
+  // A (i, j) and A' (j, i): (v * E_i) * D_j in both copies
+  __device__ void scale_A(const double* e, const double* d) {
+    for (int r = t.tid; r < p.m; r += t.T)
+      for (int k = p.A.rp[r]; k < p.A.rp[r + 1]; ++k) p.A.val[k] = (p.A.val[k] * e[r]) * d[p.A.ci[k]];
+    for (int r = t.tid; r < p.n; r += t.T)
+      for (int k = p.At.rp[r]; k < p.At.rp[r + 1]; ++k) p.At.val[k] = (p.At.val[k] * e[p.At.ci[k]]) * d[r];
+  }
+  __device__ void scale_data() {
+    const int n = p.n, m = p.m;
+    double* Dt = p.tn;    // D_temp
+    double* DtA = p.tn2;  // D_temp_A
+    double* Et = p.tm;    // E_temp
+    S.c = 1.0;
+    for (int i = t.tid; i < n; i += t.T) p.D[i] = 1.0;
+    for (int i = t.tid; i < m; i += t.T) p.E[i] = 1.0;
+    t.sync();
+    for (int it = 0; it < p.set.scaling; ++it) {
+      // inf-norms of the KKT columns: P columns (= full rows), A columns (= A' rows), A rows
+      for (int r = t.tid; r < n; r += t.T) {
+        double a = 0.0, b = 0.0;
+        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) a = dmax(a, fabs(p.P.val[k]));
+        for (int k = p.At.rp[r]; k < p.At.rp[r + 1]; ++k) b = dmax(b, fabs(p.At.val[k]));
+        Dt[r] = a;
+        DtA[r] = b;
+      }
+      for (int r = t.tid; r < m; r += t.T) {
+        double a = 0.0;
+        for (int k = p.A.rp[r]; k < p.A.rp[r + 1]; ++k) a = dmax(a, fabs(p.A.val[k]));
+        Et[r] = a;
+      }
+      for (int i = t.tid; i < n; i += t.T) {
+        double v = Dt[i] > DtA[i] ? Dt[i] : DtA[i];
+        Dt[i] = 1.0 / sqrt(limit_elem(v));
+      }
+      for (int i = t.tid; i < m; i += t.T) Et[i] = 1.0 / sqrt(limit_elem(Et[i]));
+      t.sync();
+      scale_P(Dt);
+      scale_A(Et, Dt);
+      for (int i = t.tid; i < n; i += t.T) {
+        p.q[i] = p.q[i] * Dt[i];
+        p.D[i] = p.D[i] * Dt[i];
+      }
+      for (int i = t.tid; i < m; i += t.T) p.E[i] = p.E[i] * Et[i];
+      t.sync();
+      // cost scaling: mean of P's column norms vs ||q||
+      for (int r = t.tid; r < n; r += t.T) {
+        double a = 0.0;
+        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) a = dmax(a, fabs(p.P.val[k]));
+        Dt[r] = a;
+      }
+      t.sync();
+      double c_temp = team_reduce<false>(t, n, [&](int i) { return fabs(Dt[i]); });
+      c_temp = c_temp / n;
+      double inf_q = norm_inf(p.q, n);
+      inf_q = limit_scalar(inf_q);
+      c_temp = c_temp > inf_q ? c_temp : inf_q;
+      c_temp = limit_scalar(c_temp);
+      c_temp = 1.0 / c_temp;
+      for (int r = t.tid; r < n; r += t.T)
+        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) p.P.val[k] *= c_temp;
+      for (int i = t.tid; i < n; i += t.T) p.q[i] *= c_temp;
+      S.c *= c_temp;  // every thread holds the same value
+      t.sync();
+    }
+    S.cinv = 1.0 / S.c;
+    for (int i = t.tid; i < n; i += t.T) p.Dinv[i] = 1.0 / p.D[i];
+    for (int i = t.tid; i < m; i += t.T) {
+      p.Einv[i] = 1.0 / p.E[i];
+      p.l[i] = p.l[i] * p.E[i];
+      p.u[i] = p.u[i] * p.E[i];
+    }
+    t.sync();
+  }
+  __device__ void no_scaling() {
+    S.c = 1.0;
+    S.cinv = 1.0;
+    for (int i = t.tid; i < p.n; i += t.T) p.D[i] = p.Dinv[i] = 1.0;
+    for (int i = t.tid; i < p.m; i += t.T) p.E[i] = p.Einv[i] = 1.0;
+    t.sync();
+  }
+
+  // ---- rho (src/auxil.c:76-122, osqp_api.c osqp_update_rho) -------------------
+  // classify constraints and set rho / 1/rho; returns whether a type changed
+  __device__ int set_rho_vec() {
+    S.rho = dmin(dmax(S.rho, kRhoMin), kRhoMax);
+    const double infval = kInfty * kMinScaling;
+    const double changed = team_reduce<true>(t, p.m, [&](int i) {
+      const int old = p.ctype[i];
+      int ty;
+      if ((p.l[i] < -infval) && (p.u[i] > infval)) ty = -1;
+      else if (p.u[i] - p.l[i] < kRhoTol) ty = 1;
+      else ty = 0;
+      return ty != old ? 1.0 : 0.0;
+    });
+    for (int i = t.tid; i < p.m; i += t.T) {
+      int ty;
+      if ((p.l[i] < -infval) && (p.u[i] > infval)) ty = -1;
+      else if (p.u[i] - p.l[i] < kRhoTol) ty = 1;
+      else ty = 0;
+      p.ctype[i] = ty;
+      const double r = ty < 0 ? kRhoMin : (ty == 0 ? S.rho : kRhoEqOverIneq * S.rho);
+      p.rho[i] = r;
+      p.rhoinv[i] = 1.0 / r;
+    }
+    t.sync();
+    return changed > 0.0;
+  }
+  __device__ void update_rho(double rho_new) {
+    S.rho = dmin(dmax(rho_new, kRhoMin), kRhoMax);
+    for (int i = t.tid; i < p.m; i += t.T) {
+      const int ty = p.ctype[i];
+      const double r = ty < 0 ? kRhoMin : (ty == 0 ? S.rho : kRhoEqOverIneq * S.rho);
+      p.rho[i] = r;
+      p.rhoinv[i] = 1.0 / r;
+    }
+    t.sync();
+    update_precond();
+  }
+
+  // Jacobi preconditioner: 1 / (sigma + diag(P) + sum_j rho_j A_ji^2)
+  __device__ void update_precond() {
+    V* dinv = (V*)p.cdinv;
+    team_rows<double>(
+        t, p.At,
+        [&](int k) {
+          const double a = p.At.val[k];
+          return a * a * p.rho[p.At.ci[k]];
+        },
+        [&](int r, double s) { p.prec[r] = s; });
+    t.sync();
+    for (int i = t.tid; i < p.n; i += t.T) {
+      const double pd = p.Pdiag[i] >= 0 ? p.P.val[p.Pdiag[i]] : 0.0;
+      const double d = p.set.sigma + pd + p.prec[i];
+      dinv[i] = (V)(1.0 / d);
+    }
+    t.sync();
+  }
+
+  // ---- PCG with FP64 defect correction (one linear solve) ----------------------
+  // b = [b1 (n); b2 (m)] in/out: out = [x~ ; A x~]
+  __device__ void linsys_solve(double* b1, double* b2, int admm_iter) {
+    const int n = p.n, m = p.m;
+    const double sg = p.set.sigma;
+    V* d = (V*)p.cd;
+    V* pp = (V*)p.cp;
+    V* Kp = (V*)p.cKp;
+    V* r = (V*)p.cr;
+    V* y = (V*)p.cy;
+    V* w = (V*)p.cw;
+    const V* dinv = (const V*)p.cdinv;
+    const bool guard = sizeof(V) == 4;
+    // zero-iteration bookkeeping of the previous solve (cuda_pcg_interface.cu:266-268)
+    if (S.cg_solves > 0) {
+      if (S.cg_iter == 0) S.cg_zero_iters++;
+      else S.cg_zero_iters = 0;
+    }
+    const bool cached = m > 0 && S.ax_valid && admm_iter > 1;
+    double eps;
+    double rn;
+    if (!cached) {
+      // rhs64 = b1 + A'(rho o b2); ||rhs64||_inf
+      team_rows<double>(
+          t, p.At, [&](int k) { return p.At.val[k] * (p.rho[p.At.ci[k]] * b2[p.At.ci[k]]); },
+          [&](int row, double s) { p.rhs64[row] = b1[row] + s; });
+      t.sync();
+      const double rhs_norm = norm_inf(p.rhs64, n);
+      if (admm_iter == 1) {
+        S.cg_red_factor = p.set.cg_tol_fraction;
+        S.cg_eps_prev = (rhs_norm < kCgTolMin) ? 1.0 : rhs_norm * S.cg_red_factor;
+        eps = S.cg_eps_prev;
+      } else {
+        eps = next_tolerance();
+      }
+      // w64 = rho o (A x64); r64 = sigma x64 + P x64 + A' w64 - rhs64
+      if (m) {
+        team_rows<double>(
+            t, p.A, [&](int k) { return p.A.val[k] * p.x64[p.A.ci[k]]; },
+            [&](int row, double s) { p.w64[row] = p.rho[row] * s; });
+        t.sync();
+      }
+      residual64(p.rhs64);
+    } else {
+      eps = next_tolerance();
+      for (int i = t.tid; i < m; i += t.T) p.w64[i] = p.rho[i] * (p.ax64[i] - b2[i]);
+      t.sync();
+      residual64(b1);
+    }
+    rn = norm_inf(p.r64, n);
+    S.cg_solves++;
+    const double scale = (rn > 0.0 && isfinite(rn)) ? rn : 1.0;
+    const double eps_cmp = eps / scale;
+    const double inv = 1.0 / scale;
+    // CG on K d = -r64 / scale
+    for (int i = t.tid; i < n; i += t.T) {
+      const V ri = (V)(p.r64[i] * inv);
+      const V yi = dinv[i] * ri;
+      r[i] = ri;
+      y[i] = yi;
+      pp[i] = -yi;
+      d[i] = (V)0;
+    }
+    t.sync();
+    double rty = team_dot<V>(t, n, [&](int i) { return y[i] * r[i]; });
+    double rnorm = team_reduce<true>(t, n, [&](int i) { return fabs((double)r[i]); });
+    int it = 0;
+    bool go = rnorm > eps_cmp && it < p.set.cg_max_iter;
+    if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
+    while (go) {
+      if (m) {
+        team_rows<V>(
+            t, p.A, [&](int k) { return (V)p.A.val[k] * pp[p.A.ci[k]]; },
+            [&](int row, V s) { w[row] = (V)p.rho[row] * s; });
+        t.sync();
+      }
+      // Kp = sigma p + P p + A' w
+      team_rows<V>(
+          t, p.P, [&](int k) { return (V)p.P.val[k] * pp[p.P.ci[k]]; }, [&](int row, V s) { Kp[row] = s; });
+      t.sync();
+      if (m)
+        team_rows<V>(
+            t, p.At, [&](int k) { return (V)p.At.val[k] * w[p.At.ci[k]]; },
+            [&](int row, V s) { Kp[row] = (V)sg * pp[row] + Kp[row] + s; });
+      else
+        for (int i = t.tid; i < n; i += t.T) Kp[i] = (V)sg * pp[i] + Kp[i];
+      t.sync();
+      const double pkp = team_dot<V>(t, n, [&](int i) { return pp[i] * Kp[i]; });
+      if (guard && !(pkp > 0.0 && isfinite(pkp))) break;
+      const V alpha = (V)(rty / pkp);
+      for (int i = t.tid; i < n; i += t.T) {
+        d[i] += alpha * pp[i];
+        const V ri = r[i] + alpha * Kp[i];
+        r[i] = ri;
+        y[i] = dinv[i] * ri;
+      }
+      t.sync();
+      const double rty_new = team_dot<V>(t, n, [&](int i) { return y[i] * r[i]; });
+      rnorm = team_reduce<true>(t, n, [&](int i) { return fabs((double)r[i]); });
+      const double beta = rty_new / rty;
+      it++;
+      rty = rty_new;
+      go = rnorm > eps_cmp && it < p.set.cg_max_iter;
+      if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
+      if (go) {
+        const V vb = (V)beta;
+        for (int i = t.tid; i < n; i += t.T) pp[i] = vb * pp[i] - y[i];
+        t.sync();
+      }
+    }
+    S.cg_iter = it;
+    S.cg_total += it;
+    // x64 += scale d; b1 = x64; b2 = A x64 (the A x cache)
+    for (int i = t.tid; i < n; i += t.T) {
+      double xi = p.x64[i];
+      if (it > 0) xi += scale * (double)d[i];
+      p.x64[i] = xi;
+      b1[i] = xi;
+    }
+    t.sync();
+    if (m) {
+      team_rows<double>(
+          t, p.A, [&](int k) { return p.A.val[k] * p.x64[p.A.ci[k]]; },
+          [&](int row, double s) {
+            p.ax64[row] = s;
+            b2[row] = s;
+          });
+      t.sync();
+      S.ax_valid = 1;
+    }
+  }
+  // tolerance of a non-first ADMM iteration (cuda_pcg_interface.cu:46-60)
+  __device__ double next_tolerance() {
+    if (S.cg_zero_iters >= p.set.cg_red_thresh) {
+      S.cg_red_factor /= 2;
+      S.cg_zero_iters = 0;
+    }
+    double eps = S.cg_red_factor * sqrt(S.scaled_prim_res * S.scaled_dual_res);
+    eps = eps < S.cg_eps_prev ? eps : S.cg_eps_prev;
+    eps = eps > kCgTolMin ? eps : kCgTolMin;
+    S.cg_eps_prev = eps;
+    return eps;
+  }
+  // r64 = sigma x64 + P x64 + A' w64 - rhs
+  __device__ void residual64(const double* rhs) {
+    const int n = p.n;
+    team_rows<double>(
+        t, p.P, [&](int k) { return p.P.val[k] * p.x64[p.P.ci[k]]; }, [&](int row, double s) { p.tn[row] = s; });
+    t.sync();
+    if (p.m)
+      team_rows<double>(
+          t, p.At, [&](int k) { return p.At.val[k] * p.w64[p.At.ci[k]]; },
+          [&](int row, double s) { p.r64[row] = p.set.sigma * p.x64[row] + p.tn[row] + s - rhs[row]; });
+    else
+      for (int i = t.tid; i < n; i += t.T) p.r64[i] = p.set.sigma * p.x64[i] + p.tn[i] - rhs[i];
+    t.sync();
+  }
+
+  // ---- ADMM pieces (src/auxil.c) ----------------------------------------------
+  __device__ void cold_start() {
+    for (int i = t.tid; i < p.n; i += t.T) {
+      x()[i] = 0.0;
+      p.x64[i] = 0.0;
+    }
+    for (int i = t.tid; i < p.m; i += t.T) {
+      z()[i] = 0.0;
+      p.y[i] = 0.0;
+    }
+    S.ax_valid = 0;
+    t.sync();
+  }
+  // osqp_warm_start (osqp_api.c:1260-1293) from raw x0 / y0
+  __device__ void warm_start(const double* x0, const double* y0) {
+    if (x0) {
+      for (int i = t.tid; i < p.n; i += t.T) {
+        double v = x0[i];
+        if (p.set.scaling) v = v * p.Dinv[i];
+        x()[i] = v;
+      }
+    }
+    if (y0) {
+      for (int i = t.tid; i < p.m; i += t.T) {
+        double v = y0[i];
+        if (p.set.scaling) v = (v * p.Einv[i]) * S.c;
+        p.y[i] = v;
+      }
+    }
+    t.sync();
+    if (x0) {
+      double* zz = z();
+      double* xx = x();
+      team_rows<double>(
+          t, p.A, [&](int k) { return p.A.val[k] * xx[p.A.ci[k]]; }, [&](int row, double s) { zz[row] = s; });
+      t.sync();
+    }
+    for (int i = t.tid; i < p.n; i += t.T) p.x64[i] = x()[i];
+    S.ax_valid = 0;
+    t.sync();
+  }
+
+  __device__ void compute_obj_gap(const double* xx, const double* yy) {
+    const double quad = dot(p.Px, xx, p.n);
+    const double lin = dot(p.q, xx, p.n);
+    const double infval = kInfty * kMinScaling;
+    // y projected on the polar recession cone, deadzone rounded
+    for (int i = t.tid; i < p.m; i += t.T) {
+      double v = yy[i];
+      if (p.u[i] > +infval) {
+        if (p.l[i] < -infval) v = 0.0;
+        else v = dmin(v, 0.0);
+      } else if (p.l[i] < -infval) {
+        v = dmax(v, 0.0);
+      }
+      if (fabs(v) < kZeroDeadzone) v = 0.0;
+      p.tm[i] = v;
+    }
+    t.sync();
+    double sup = team_reduce<false>(t, p.m, [&](int i) { return p.u[i] * dmax(p.tm[i], 0.0); });
+    sup += team_reduce<false>(t, p.m, [&](int i) { return p.l[i] * dmin(p.tm[i], 0.0); });
+    double prim = 0.5 * quad + lin;
+    double dual = -0.5 * quad - sup;
+    S.scaled_gap = quad + lin + sup;
+    if (p.set.scaling) {
+      prim *= S.cinv;
+      dual *= S.cinv;
+      S.gap = S.cinv * S.scaled_gap;
+    } else {
+      S.gap = S.scaled_gap;
+    }
+    S.obj_val = prim;
+    S.dual_obj_val = dual;
+    S.xtPx = quad;
+    S.qtx = lin;
+    S.SC = sup;
+  }
+
+  __device__ double compute_prim_res() {
+    double* xx = x();
+    const double* zz = z();
+    team_spmv(t, p.A, xx, p.Ax);
+    t.sync();
+    for (int i = t.tid; i < p.m; i += t.T) p.tm[i] = p.Ax[i] - zz[i];
+    t.sync();
+    S.scaled_prim_res = norm_inf(p.tm, p.m);
+    if (unscaled_term()) return scaled_norm_inf(p.Einv, p.tm, p.m);
+    return S.scaled_prim_res;
+  }
+  __device__ double compute_dual_res() {
+    double* xx = x();
+    team_spmv(t, p.P, xx, p.Px);
+    if (p.m) team_spmv(t, p.At, p.y, p.Aty);
+    t.sync();
+    for (int i = t.tid; i < p.n; i += t.T) {
+      double v = p.q[i] + p.Px[i];
+      if (p.m) v = v + p.Aty[i];
+      p.tn[i] = v;
+    }
+    t.sync();
+    S.scaled_dual_res = norm_inf(p.tn, p.n);
+    if (unscaled_term()) return S.cinv * scaled_norm_inf(p.Dinv, p.tn, p.n);
+    return S.scaled_dual_res;
+  }
+  __device__ void update_info(int iter) {
+    (void)iter;
+    S.prim_res = p.m == 0 ? 0.0 : compute_prim_res();
+    S.dual_res = compute_dual_res();
+    compute_obj_gap(x(), p.y);
+    S.primdual_int += fabs(S.gap);
+    S.rel_kkt = dmax(dmax(S.dual_res, S.prim_res), S.gap);
+  }
+
+  __device__ double prim_tol(double ea, double er) {
+    double a, b;
+    if (unscaled_term()) {
+      a = scaled_norm_inf(p.Einv, z(), p.m);
+      b = scaled_norm_inf(p.Einv, p.Ax, p.m);
+    } else {
+      a = norm_inf(z(), p.m);
+      b = norm_inf(p.Ax, p.m);
+    }
+    return ea + er * dmax(a, b);
+  }
+  __device__ double dual_tol(double ea, double er) {
+    double r, s;
+    if (unscaled_term()) {
+      r = scaled_norm_inf(p.Dinv, p.q, p.n);
+      s = scaled_norm_inf(p.Dinv, p.Aty, p.n);
+      r = dmax(r, s);
+      s = scaled_norm_inf(p.Dinv, p.Px, p.n);
+      r = dmax(r, s);
+      r *= S.cinv;
+    } else {
+      r = norm_inf(p.q, p.n);
+      s = norm_inf(p.Aty, p.n);
+      r = dmax(r, s);
+      s = norm_inf(p.Px, p.n);
+      r = dmax(r, s);
+    }
+    return ea + er * r;
+  }
+  __device__ double gap_tol(double ea, double er) {
+    double r = fabs(S.xtPx);
+    r = dmax(r, fabs(S.qtx));
+    r = dmax(r, fabs(S.SC));
+    if (unscaled_term()) r = S.cinv * r;
+    return ea + er * r;
+  }
+  __device__ bool is_primal_infeasible(double eps_pinf) {
+    const double infval = kInfty * kMinScaling;
+    for (int i = t.tid; i < p.m; i += t.T) {
+      double v = p.dy[i];
+      if (p.u[i] > +infval) {
+        if (p.l[i] < -infval) v = 0.0;
+        else v = dmin(v, 0.0);
+      } else if (p.l[i] < -infval) {
+        v = dmax(v, 0.0);
+      }
+      p.dy[i] = v;
+    }
+    t.sync();
+    double ndy;
+    if (unscaled_term()) ndy = team_reduce<true>(t, p.m, [&](int i) { return fabs(p.E[i] * p.dy[i]); });
+    else ndy = norm_inf(p.dy, p.m);
+    if (ndy > kDivisionTol) {
+      double lhs = team_reduce<false>(t, p.m, [&](int i) { return p.u[i] * dmax(p.dy[i], 0.0); });
+      lhs += team_reduce<false>(t, p.m, [&](int i) { return p.l[i] * dmin(p.dy[i], 0.0); });
+      if (lhs < 0.0) {
+        team_spmv(t, p.At, p.dy, p.tn);
+        t.sync();
+        double nat;
+        if (unscaled_term()) nat = team_reduce<true>(t, p.n, [&](int i) { return fabs(p.tn[i] * p.Dinv[i]); });
+        else nat = norm_inf(p.tn, p.n);
+        return nat < eps_pinf * ndy;
+      }
+    }
+    return false;
+  }
+  __device__ bool is_dual_infeasible(double eps_dinf) {
+    double ndx, cs;
+    if (unscaled_term()) {
+      ndx = scaled_norm_inf(p.D, p.dx, p.n);
+      cs = S.c;
+    } else {
+      ndx = norm_inf(p.dx, p.n);
+      cs = 1.0;
+    }
+    if (ndx > kDivisionTol) {
+      if (dot(p.q, p.dx, p.n) < 0.0) {
+        team_spmv(t, p.P, p.dx, p.tn);
+        t.sync();
+        double npd;
+        if (unscaled_term()) npd = team_reduce<true>(t, p.n, [&](int i) { return fabs(p.tn[i] * p.Dinv[i]); });
+        else npd = norm_inf(p.tn, p.n);
+        if (npd < cs * eps_dinf * ndx) {
+          team_spmv(t, p.A, p.dx, p.tm);
+          t.sync();
+          if (unscaled_term())
+            for (int i = t.tid; i < p.m; i += t.T) p.tm[i] = p.tm[i] * p.Einv[i];
+          t.sync();
+          const double infval = kInfty * kMinScaling, tol = eps_dinf * ndx;
+          const double bad = team_reduce<true>(t, p.m, [&](int i) {
+            const double yv = p.tm[i], l = p.l[i], u = p.u[i];
+            return (((u < +infval) && (yv > +tol)) || ((l > -infval) && (yv < -tol))) ? 1.0 : 0.0;
+          });
+          return bad == 0.0;
+        }
+      }
+    }
+    return false;
+  }
+  // returns 1 when the solve ends (src/auxil.c:808-945)
+  __device__ int check_termination(int approximate) {
+    double ea = p.set.eps_abs, er = p.set.eps_rel, epi = p.set.eps_pinf, edi = p.set.eps_dinf;
+    if ((S.prim_res > kInfty) || (S.dual_res > kInfty)) {
+      S.status = kNonCvx;
+      S.obj_val = __longlong_as_double(0x7ff8000000000000LL);
+      return 1;
+    }
+    if (approximate) {
+      ea *= 10;
+      er *= 10;
+      epi *= 10;
+      edi *= 10;
+    }
+    bool prim_ok = false, dual_ok = false, gap_ok = false, prim_inf = false, dual_inf = false;
+    if (p.m == 0) {
+      prim_ok = true;
+    } else {
+      if (S.prim_res < prim_tol(ea, er)) prim_ok = true;
+      else prim_inf = is_primal_infeasible(epi);
+    }
+    if (S.dual_res < dual_tol(ea, er)) dual_ok = true;
+    else dual_inf = is_dual_infeasible(edi);
+    if (p.set.check_gap) {
+      const double eg = gap_tol(ea, er);
+      if (unscaled_term()) gap_ok = fabs(S.gap) < eg;
+      else gap_ok = fabs(S.scaled_gap) < eg;
+    } else {
+      gap_ok = true;
+    }
+    if (prim_ok && dual_ok && gap_ok) {
+      S.status = approximate ? kSolvedInaccurate : kSolved;
+      return 1;
+    } else if (prim_inf) {
+      S.status = approximate ? kPrimalInfeasibleInaccurate : kPrimalInfeasible;
+      if (unscaled_term()) {
+        for (int i = t.tid; i < p.m; i += t.T) p.dy[i] = p.dy[i] * p.E[i];
+        t.sync();
+      }
+      S.obj_val = kInfty;
+      return 1;
+    } else if (dual_inf) {
+      S.status = approximate ? kDualInfeasibleInaccurate : kDualInfeasible;
+      if (unscaled_term()) {
+        for (int i = t.tid; i < p.n; i += t.T) p.dx[i] = p.dx[i] * p.D[i];
+        t.sync();
+      }
+      S.obj_val = -kInfty;
+      return 1;
+    }
+    return 0;
+  }
+
+  __device__ double rho_estimate() {
+    const double pr = S.scaled_prim_res / (dmax(norm_inf(z(), p.m), norm_inf(p.Ax, p.m)) + kDivisionTol);
+    double dn = dmax(norm_inf(p.q, p.n), norm_inf(p.Aty, p.n));
+    dn = dmax(dn, norm_inf(p.Px, p.n));
+    const double du = S.scaled_dual_res / (dn + kDivisionTol);
+    const double est = S.rho * sqrt(pr / du);
+    return dmin(dmax(est, kRhoMin), kRhoMax);
+  }
+
+  __device__ bool has_solution() const {
+    const int s = S.status;
+    return s != kPrimalInfeasible && s != kPrimalInfeasibleInaccurate && s != kDualInfeasible &&
+           s != kDualInfeasibleInaccurate && s != kNonCvx;
+  }
+
+  // setup of a new problem (osqp_setup: scale_data, set_rho_vec, linsys
+  // init, cold start) and an optional osqp_warm_start
+  __device__ void setup() {
+    const int4* s4 = (const int4*)p.stage;
+    int4* d4 = (int4*)p.arena_static;
+    const long long n4 = p.stage_bytes / 16;
+    for (long long i = t.tid; i < n4; i += t.T) d4[i] = s4[i];
+    for (int i = t.tid; i < p.m; i += t.T) p.ctype[i] = 0;
+    t.sync();
+    S.rho = p.set.rho0;
+    if (p.set.scaling) scale_data();
+    else no_scaling();
+    set_rho_vec();
+    update_precond();
+    S.swapped = 0;
+    cold_start();
+    if (p.warm_x || p.warm_y) warm_start(p.warm_x, p.warm_y);
+    S.status = kUnsolved;
+    S.iter = 0;
+    S.rho_updates = 0;
+    S.can_check = 0;
+    S.obj_val = kInfty;
+    S.prim_res = kInfty;
+    S.dual_res = kInfty;
+    S.rel_kkt = kInfty;
+    S.last_rel_kkt = kInfty;
+    S.rho_estimate = S.rho;
+    S.primdual_int = 0.0;
+    S.cg_iter = 0;
+    S.cg_zero_iters = 0;
+    S.cg_solves = 0;
+    S.cg_total = 0;
+    S.cg_eps_prev = 0.0;
+    S.cg_red_factor = p.set.cg_tol_fraction;
+    S.phase = kPhaseRun;
+  }
+
+  // a new solve of a resident problem: osqp_update_data_vec(q) (if given),
+  // reset_info, and osqp_solve's cold start when warm starting is off
+  __device__ void new_solve(const double* q_new) {
+    if (q_new) {
+      for (int i = t.tid; i < p.n; i += t.T) {
+        double v = q_new[i];
+        if (p.set.scaling) v = (v * p.D[i]) * S.c;
+        p.q[i] = v;
+      }
+      t.sync();
+    }
+    S.status = kUnsolved;
+    S.rho_updates = 0;
+    S.iter = 0;
+    S.can_check = 0;
+    if (!p.set.warm) cold_start();
+    S.phase = kPhaseRun;
+  }
+
+  // end of a solve: objective, final status, unscaled solution to the host
+  __device__ void finish() {
+    if (!S.can_check) {
+      update_info(S.iter);
+      check_termination(0);
+    }
+    if (has_solution()) compute_obj_gap(x(), p.y);
+    if (S.status == kUnsolved) {
+      if (!check_termination(1)) S.status = kMaxIterReached;
+    }
+    S.rho_estimate = rho_estimate();
+    const double nanv = __longlong_as_double(0x7ff8000000000000LL);
+    if (has_solution()) {
+      for (int i = t.tid; i < p.n; i += t.T) p.out_x[i] = p.set.scaling ? x()[i] * p.D[i] : x()[i];
+      for (int i = t.tid; i < p.m; i += t.T)
+        p.out_y[i] = p.set.scaling ? (p.y[i] * p.E[i]) * S.cinv : p.y[i];
+    } else {
+      for (int i = t.tid; i < p.n; i += t.T) p.out_x[i] = nanv;
+      for (int i = t.tid; i < p.m; i += t.T) p.out_y[i] = nanv;
+      cold_start();
+    }
+    t.sync();
+    if (t.tid == 0) {
+      BResult* o = p.out;
+      o->status = S.status;
+      o->iter = S.iter;
+      o->rho_updates = S.rho_updates;
+      o->obj_val = S.obj_val;
+      o->dual_obj_val = S.dual_obj_val;
+      o->prim_res = S.prim_res;
+      o->dual_res = S.dual_res;
+      o->gap = S.gap;
+      o->rho_estimate = S.rho_estimate;
+      o->primdual_int = S.primdual_int;
+      o->rel_kkt = S.rel_kkt;
+      o->cg_iters = S.cg_total;
+      __threadfence_system();
+      o->done = 1;
+    }
+    S.phase = kPhaseDone;
+  }
+
+  // one ADMM iteration; returns 1 when the solve has ended
+  __device__ int iterate() {
+    const int iter = ++S.iter;
+    const double a = p.set.alpha;
+    S.swapped ^= 1;  // swap(x, x_prev), swap(z, z_prev)
+    double* xx = x();
+    double* xp = xprev();
+    double* zz = z();
+    double* zp = zprev();
+    // compute_rhs
+    for (int i = t.tid; i < p.n; i += t.T) p.xt[i] = p.set.sigma * xp[i] + (-1.0) * p.q[i];
+    for (int i = t.tid; i < p.m; i += t.T) {
+      const double v = p.rhoinv[i] * p.y[i];
+      p.zt[i] = (-1.0) * v + 1.0 * zp[i];
+    }
+    t.sync();
+    linsys_solve(p.xt, p.zt, iter);
+    // update_x, update_z, update_y
+    for (int i = t.tid; i < p.n; i += t.T) {
+      const double v = a * p.xt[i] + (1.0 - a) * xp[i];
+      xx[i] = v;
+      p.dx[i] = v - xp[i];
+    }
+    for (int i = t.tid; i < p.m; i += t.T) {
+      double v = p.rhoinv[i] * p.y[i];
+      v = 1.0 * v + a * p.zt[i] + (1.0 - a) * zp[i];
+      v = dmin(dmax(v, p.l[i]), p.u[i]);
+      zz[i] = v;
+      double dyv = a * p.zt[i] + (1.0 - a) * zp[i] + (-1.0) * v;
+      dyv = dyv * p.rho[i];
+      p.dy[i] = dyv;
+      p.y[i] = p.y[i] + dyv;
+    }
+    t.sync();
+    const int ct = p.set.check_term;
+    const bool can_check = ct && (iter % ct == 0);
+    const bool can_adapt = p.set.adaptive && p.set.ad_interval && (iter % p.set.ad_interval == 0);
+    S.can_check = can_check;
+    if (can_check || can_adapt || iter == 1) update_info(iter);
+    if (can_check && check_termination(0)) return 1;
+    if (can_adapt) {
+      const double rho_new = rho_estimate();
+      S.rho_estimate = rho_new;
+      if ((rho_new > S.rho * p.set.ad_tol) || (rho_new < S.rho / p.set.ad_tol)) {
+        update_rho(rho_new);
+        S.rho_updates += 1;
+        S.last_rel_kkt = S.rel_kkt;
+      }
+    }
+    return iter >= p.set.max_iter ? 1 : 0;
+  }
+};
+
+}  // namespace osqp_serve

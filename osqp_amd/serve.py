@@ -1,0 +1,170 @@
+"""Python front end of the batched serving engine (include/osqp_amd_serve.h).
+
+    eng = BatchEngine(precision=32, eps_abs=1e-4, eps_rel=1e-4)
+    ids = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"]) for p in problems]
+    while eng.pending():
+        eng.step()                     # one epoch on the GPU
+    results = [eng.poll(i) for i in ids]
+
+Settings are OSQP's (keyword arguments as in OSQP.setup).  All work happens in
+libosqp_amd.so; this module marshals arrays only.
+"""
+import ctypes as C
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _types as T
+from .solver import load_library
+
+_bound = False
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("status_val", T.OSQPInt), ("iter", T.OSQPInt), ("rho_updates", T.OSQPInt),
+        ("obj_val", T.OSQPFloat), ("dual_obj_val", T.OSQPFloat), ("prim_res", T.OSQPFloat),
+        ("dual_res", T.OSQPFloat), ("duality_gap", T.OSQPFloat), ("rho_estimate", T.OSQPFloat),
+        ("primdual_int", T.OSQPFloat), ("rel_kkt_error", T.OSQPFloat), ("latency", T.OSQPFloat),
+        ("cg_iters", C.c_longlong),
+    ]
+
+
+def _bind(lib):
+    global _bound
+    if _bound:
+        return lib
+    vp = C.c_void_p
+    lib.osqp_batch_create.argtypes = [C.POINTER(vp), vp, C.c_int, C.c_int, C.c_longlong]
+    lib.osqp_batch_create.restype = C.c_int
+    lib.osqp_batch_add.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_int)]
+    lib.osqp_batch_add.restype = C.c_int
+    lib.osqp_batch_resolve.argtypes = [vp, C.c_int, vp]
+    lib.osqp_batch_resolve.restype = C.c_int
+    lib.osqp_batch_step.argtypes = [vp, C.POINTER(C.c_int)]
+    lib.osqp_batch_step.restype = C.c_int
+    lib.osqp_batch_poll.argtypes = [vp, C.c_int, vp, vp, C.POINTER(BatchInfo)]
+    lib.osqp_batch_poll.restype = C.c_int
+    lib.osqp_batch_release.argtypes = [vp, C.c_int]
+    lib.osqp_batch_release.restype = C.c_int
+    lib.osqp_batch_pending.argtypes = [vp]
+    lib.osqp_batch_pending.restype = C.c_int
+    lib.osqp_batch_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.osqp_batch_tuning.restype = C.c_int
+    lib.osqp_batch_set_team.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    lib.osqp_batch_set_team.restype = C.c_int
+    lib.osqp_batch_counters.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                        C.POINTER(C.c_double)]
+    lib.osqp_batch_counters.restype = C.c_int
+    lib.osqp_batch_destroy.argtypes = [vp]
+    lib.osqp_batch_destroy.restype = C.c_int
+    _bound = True
+    return lib
+
+
+class BatchEngine:
+    """Many QPs resident on one GPU, advanced together epoch by epoch."""
+
+    def __init__(self, precision=32, max_active=4096, arena_bytes=0, **settings):
+        self.lib = _bind(load_library())
+        s = T.default_settings()
+        T.apply_settings(s, **settings)
+        self._settings = s
+        self._b = C.c_void_p()
+        rc = self.lib.osqp_batch_create(C.byref(self._b), C.addressof(s), int(precision), int(max_active),
+                                        int(arena_bytes))
+        if rc:
+            raise RuntimeError(f"osqp_batch_create failed ({rc})")
+        self.precision = precision
+        self._dims = {}
+
+    # ---- requests -------------------------------------------------------------
+    def add(self, P, q, A, l, u, x0=None, y0=None, resident=False):
+        Ph, Ah = T.CscHolder(P), T.CscHolder(A)
+        q = T.as_f64(q)
+        l = np.clip(T.as_f64(l), -T.OSQP_INFTY, T.OSQP_INFTY)
+        u = np.clip(T.as_f64(u), -T.OSQP_INFTY, T.OSQP_INFTY)
+        x0, y0 = T.as_f64(x0), T.as_f64(y0)
+        pid = C.c_int(0)
+        rc = self.lib.osqp_batch_add(self._b, Ph.ptr, T.dptr(q), Ah.ptr, T.dptr(l), T.dptr(u), T.dptr(x0),
+                                     T.dptr(y0), 1 if resident else 0, C.byref(pid))
+        if rc:
+            raise ValueError(f"osqp_batch_add rejected the problem ({rc})")
+        self._dims[pid.value] = (Ph.n, Ah.m)
+        return pid.value
+
+    def resolve(self, pid, q=None):
+        q = T.as_f64(q)
+        rc = self.lib.osqp_batch_resolve(self._b, int(pid), T.dptr(q))
+        if rc:
+            raise RuntimeError(f"osqp_batch_resolve({pid}) failed: not resident, unknown or busy")
+
+    def step(self):
+        done = C.c_int(0)
+        rc = self.lib.osqp_batch_step(self._b, C.byref(done))
+        if rc:
+            raise RuntimeError(f"osqp_batch_step failed ({rc})")
+        return done.value
+
+    def poll(self, pid):
+        """Result namespace (x, y, info) once the solve has ended, else None."""
+        n, m = self._dims[pid]
+        x = np.empty(n)
+        y = np.empty(max(m, 1))
+        info = BatchInfo()
+        rc = self.lib.osqp_batch_poll(self._b, int(pid), T.dptr(x), T.dptr(y), C.byref(info))
+        if rc < 0:
+            raise KeyError(pid)
+        if rc == 0:
+            return None
+        inf = SimpleNamespace(**{f: getattr(info, f) for f, _ in BatchInfo._fields_})
+        inf.status = T.STATUS_NAMES.get(inf.status_val, "unknown")
+        return SimpleNamespace(x=x, y=y[:m], info=inf)
+
+    def release(self, pid):
+        rc = self.lib.osqp_batch_release(self._b, int(pid))
+        self._dims.pop(pid, None)
+        return rc
+
+    def pending(self):
+        return self.lib.osqp_batch_pending(self._b)
+
+    def tuning(self, bucket):
+        t, k = C.c_int(0), C.c_int(0)
+        if self.lib.osqp_batch_tuning(self._b, int(bucket), C.byref(t), C.byref(k)):
+            return None
+        return t.value, k.value
+
+    def set_team(self, bucket, team, teams_per_cu):
+        if self.lib.osqp_batch_set_team(self._b, int(bucket), int(team), int(teams_per_cu)):
+            raise ValueError("unsupported team configuration")
+
+    def counters(self):
+        e, i, g = C.c_longlong(0), C.c_longlong(0), C.c_double(0)
+        self.lib.osqp_batch_counters(self._b, C.byref(e), C.byref(i), C.byref(g))
+        return dict(epochs=e.value, admm_iters=i.value, gpu_s=g.value)
+
+    # ---- convenience ----------------------------------------------------------
+    def solve_batch(self, problems, max_epochs=100000):
+        """Adds every problem (dicts with P, q, A, l, u), runs epochs until all
+        have ended and returns their results in order."""
+        ids = [self.add(p["P"], p["q"], p["A"], p["l"], p["u"], p.get("x0"), p.get("y0")) for p in problems]
+        out = {}
+        for _ in range(max_epochs):
+            if not self.pending():
+                break
+            self.step()
+        for pid in ids:
+            out[pid] = self.poll(pid)
+        return [out[i] for i in ids]
+
+    def close(self):
+        if self._b:
+            self.lib.osqp_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,200 @@
+"""GPU tests of the batched serving engine (include/osqp_amd_serve.h).
+
+The engine has no counterpart in the reference; its per-problem arithmetic
+restates the host core's ADMM step and the PCG linear solver, so it is checked
+against the single-problem engine (osqp_setup / osqp_solve through the same
+library) on the same data:
+  * FP64 engine vs FP64 single-problem engine: same status, objective to 1e-6
+    relative (only summation orders differ), iterations within one
+    termination-check interval;
+  * FP32 engine (FP32 CG, FP64 residuals) vs FP64 single-problem engine on the
+    MPC stream: identical statuses, objective to 1e-3 relative;
+  * each result is bit-identical alone and in a mix of other problems, and for
+    any team size (the engine's arithmetic is fixed by the problem's data);
+  * resident problems with q updates and warm-started re-solves follow
+    osqp_update_data_vec + osqp_solve;
+  * admission is limited by slots and arena memory, and problems added while
+    others run are admitted as capacity frees up.
+"""
+import numpy as np
+import pytest
+
+from osqp_amd import OSQP
+from osqp_amd import problems as pr
+from osqp_amd.serve import BatchEngine
+
+pytestmark = pytest.mark.gpu
+
+ST = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000)
+
+
+def single(p, precision, **kw):
+    st = dict(ST)
+    st.update(kw)
+    s = OSQP(precision=precision)
+    assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
+    r = s.solve()
+    s.close()
+    return r
+
+
+def mixed_problems():
+    probs = [pr.mpc(N, seed=100 + N) for N in (10, 17, 25, 40)]
+    probs += [pr.random_qp(60, 300, 0.1, 0.1, seed=s) for s in range(3)]
+    port = pr.portfolio(400, seed=3)
+    probs.append(port)
+    return probs
+
+
+def run_all(eng, probs, resident=False):
+    ids = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=resident) for p in probs]
+    for _ in range(100000):
+        if not eng.pending():
+            break
+        eng.step()
+    return [eng.poll(i) for i in ids]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_batch_matches_single_problem_engine(precision):
+    probs = mixed_problems()
+    eng = BatchEngine(precision=precision, **ST)
+    res = run_all(eng, probs)
+    eng.close()
+    for k, (p, rb) in enumerate(zip(probs, res)):
+        rs = single(p, 64)
+        print(f"[{k}] batch{precision}: {rb.info.status} it={rb.info.iter} obj={rb.info.obj_val:.10e} | "
+              f"single64: {rs.info.status} it={rs.info.iter} obj={rs.info.obj_val:.10e}", flush=True)
+        assert rb.info.status_val == rs.info.status_val
+        tol = 1e-6 if precision == 64 else 1e-3
+        assert abs(rb.info.obj_val - rs.info.obj_val) <= tol * max(1.0, abs(rs.info.obj_val))
+        if precision == 64:
+            assert abs(rb.info.iter - rs.info.iter) <= 25
+        c = pr.kkt_certificate(p, rb.x, rb.y)
+        assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+        assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+
+
+def test_mpc_stream_statuses_vs_fp64_plain():
+    """FP32 batched engine on an MPC stream (N ~ U{10..50}) against the plain
+    FP64 single-problem engine: identical statuses, objectives to 1e-3."""
+    rng = np.random.default_rng(7)
+    probs = [pr.mpc(int(N), seed=int(s)) for N, s in zip(rng.integers(10, 51, 24), range(24))]
+    eng = BatchEngine(precision=32, **ST)
+    res = run_all(eng, probs)
+    eng.close()
+    for p, rb in zip(probs, res):
+        rs = single(p, 64)
+        assert rb.info.status_val == rs.info.status_val
+        if rs.info.status_val == 1:
+            assert abs(rb.info.obj_val - rs.info.obj_val) <= 1e-3 * max(1.0, abs(rs.info.obj_val))
+
+
+def same(a, b):
+    assert a.info.status_val == b.info.status_val
+    assert a.info.iter == b.info.iter
+    assert a.info.obj_val == b.info.obj_val or (np.isnan(a.info.obj_val) and np.isnan(b.info.obj_val))
+    np.testing.assert_array_equal(a.x, b.x)
+    np.testing.assert_array_equal(a.y, b.y)
+
+
+def test_alone_vs_mix_bit_identical():
+    targets = [pr.mpc(20, seed=5), pr.random_qp(50, 200, 0.1, 0.1, seed=9), pr.portfolio(300, seed=1)]
+    alone = []
+    for p in targets:
+        eng = BatchEngine(precision=32, **ST)
+        alone.append(run_all(eng, [p])[0])
+        eng.close()
+    rng = np.random.default_rng(0)
+    others = [pr.mpc(int(N), seed=1000 + k) for k, N in enumerate(rng.integers(10, 51, 60))]
+    others += [pr.random_qp(40, 150, 0.15, 0.15, seed=50 + k) for k in range(10)]
+    mix = others[:35] + targets[:1] + others[35:50] + targets[1:] + others[50:]
+    eng = BatchEngine(precision=32, max_active=32, **ST)
+    res = run_all(eng, mix)
+    eng.close()
+    got = [res[next(i for i, q in enumerate(mix) if q is t)] for t in targets]
+    for a, b in zip(alone, got):
+        same(a, b)
+
+
+def test_team_size_does_not_change_results():
+    probs = [pr.mpc(30, seed=2), pr.random_qp(80, 400, 0.08, 0.08, seed=4), pr.portfolio(500, seed=8)]
+    outs = []
+    for team, tpc in ((64, 8), (512, 2), (256, 4)):
+        eng = BatchEngine(precision=32, **ST)
+        for b in range(6):
+            eng.set_team(b, team, tpc)
+        outs.append(run_all(eng, probs))
+        eng.close()
+    for r in outs[1:]:
+        for a, b in zip(outs[0], r):
+            same(a, b)
+
+
+def test_resident_resolve_follows_update_data_vec():
+    """Portfolio with new mu per request: resident problem + resolve(q) vs
+    osqp_update_data_vec(q) + warm-started osqp_solve (FP64 both)."""
+    n = 300
+    p = pr.portfolio(n, seed=11)
+    k = p["P"].shape[0] - n
+    rng = np.random.default_rng(3)
+    qs = [pr.portfolio_q(rng.standard_normal(n), k) for _ in range(3)]
+    s = OSQP(precision=64)
+    assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **ST) == 0
+    ref = [s.solve()]
+    for q in qs:
+        s.update(q=q)
+        ref.append(s.solve())
+    s.close()
+    eng = BatchEngine(precision=64, **ST)
+    pid = eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=True)
+    got = []
+    for j in range(len(qs) + 1):
+        if j:
+            eng.resolve(pid, qs[j - 1])
+        while eng.pending():
+            eng.step()
+        got.append(eng.poll(pid))
+    eng.close()
+    for a, b in zip(got, ref):
+        assert a.info.status_val == b.info.status_val
+        assert abs(a.info.obj_val - b.info.obj_val) <= 1e-6 * max(1.0, abs(b.info.obj_val))
+        assert abs(a.info.iter - b.info.iter) <= 25
+
+
+def test_admission_by_slots_and_memory():
+    probs = [pr.mpc(N, seed=N) for N in range(10, 30)]
+    eng = BatchEngine(precision=32, max_active=3, arena_bytes=8 << 20, **ST)
+    ids = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"]) for p in probs[:10]]
+    eng.step()
+    assert 0 < eng.pending() <= 10
+    # requests arriving while others run are admitted as capacity frees up
+    ids += [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"]) for p in probs[10:]]
+    for _ in range(100000):
+        if not eng.pending():
+            break
+        eng.step()
+    res = [eng.poll(i) for i in ids]
+    eng.close()
+    for p, r in zip(probs, res):
+        assert r is not None
+        assert r.info.status_val == single(p, 64).info.status_val
+
+
+@pytest.mark.parametrize("case", [1, 2, 3, 4])
+def test_infeasibility_in_batch(case):
+    p = pr.primal_dual_infeasibility(case)
+    eng = BatchEngine(precision=64, **ST)
+    rb = run_all(eng, [p])[0]
+    eng.close()
+    rs = single(p, 64)
+    assert rb.info.status_val == rs.info.status_val
+
+
+def test_settings_rejected():
+    with pytest.raises(RuntimeError):
+        BatchEngine(precision=32, polishing=1)
+    with pytest.raises(RuntimeError):
+        BatchEngine(precision=32, adaptive_rho=2)
+    with pytest.raises(RuntimeError):
+        BatchEngine(precision=16)
