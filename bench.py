@@ -35,6 +35,92 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "panel_spmv_large_qp_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    return None
+
+
+def serve_mpc(n_req, seed):
+    """MPC stream (nx=12, nu=4, N ~ U{10..50}) through the batched FP32-PCG
+    engine with Poisson arrivals at a rate above saturation (up to 4096 in
+    flight); latency = result seen by the host - arrival.  The plain FP64
+    engine (osqp_setup + osqp_solve per request, one at a time) on a sample of
+    the same requests is the GPU comparison; statuses are compared on it."""
+    import torch
+    from osqp_amd import OSQP
+    from osqp_amd import problems as pr
+    from osqp_amd.serve import BatchEngine
+    st = dict(eps_abs=1e-4, eps_rel=1e-4)
+    rng = np.random.default_rng(seed + 17)
+    Ns = rng.integers(10, 51, n_req)
+    t0 = time.time()
+    reqs = [pr.mpc(int(N), seed=seed * 100000 + k) for k, N in enumerate(Ns)]
+    log(f"[serve] generated {n_req} MPC requests in {time.time() - t0:.1f}s")
+    eng = BatchEngine(precision=32, max_active=4096, **st)
+    eng.solve_batch(reqs[:512])  # warm-up: code paths and the team-size tuner
+    rate = 1e6  # requests/s: far above saturation, so the engine is the bottleneck
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    ids, arr_of, lat, status, iters = [], {}, [], {}, []
+    nxt = 0
+    pending = set()
+    while nxt < n_req or pending:
+        now = time.perf_counter() - t_start
+        while nxt < n_req and arrivals[nxt] <= now and len(pending) < 4096:
+            p = reqs[nxt]
+            pid = eng.add(p["P"], p["q"], p["A"], p["l"], p["u"])
+            arr_of[pid] = (nxt, arrivals[nxt])
+            pending.add(pid)
+            nxt += 1
+        done = eng.step()
+        t_seen = time.perf_counter() - t_start
+        for pid in done:
+            r = eng.poll(pid)
+            k, a = arr_of[pid]
+            lat.append(t_seen - a)
+            status[k] = r.info.status_val
+            iters.append(r.info.iter)
+            pending.discard(pid)
+    elapsed = time.perf_counter() - t_start
+    cnt = eng.counters()
+    tuning = [eng.tuning(b) for b in range(6)]
+    eng.close()
+    qps = n_req / elapsed
+    # plain FP64 engine, one request at a time (setup + solve), on a sample
+    k_plain = 24
+    same = 0
+    ts = time.perf_counter()
+    for k in range(k_plain):
+        p = reqs[k]
+        s = OSQP(precision=64)
+        assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
+        r = s.solve()
+        same += int(r.info.status_val == status[k])
+        s.close()
+    plain_qps = k_plain / (time.perf_counter() - ts)
+    log(f"[serve] batched FP32: {qps:.0f} QPs/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
+        f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms; plain FP64: {plain_qps:.1f} QPs/s; "
+        f"statuses identical on {same}/{k_plain}")
+    return {"workload": f"MPC stream nx=12 nu=4 N~U{{10..50}}, {n_req} requests, Poisson arrivals at "
+                        f"{rate:.0e}/s, <=4096 in flight, eps 1e-4",
+            "qps": qps, "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                       "p99": float(np.percentile(lat, 99) * 1e3)},
+            "mean_admm_iters": float(np.mean(iters)),
+            "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
+            "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
+            "plain_sample": k_plain, "statuses_identical_on_sample": same,
+            "engine": {"epochs": cnt["epochs"], "gpu_s": cnt["gpu_s"], "tuned_team_per_bucket": tuning}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,6 +130,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--mpc-requests", type=int, default=4096)
     ap.add_argument("--profile-kernels", action="store_true",
                     help="set up the QP, launch only the timed SpMV kernels (reps=20 each, A then K) and a "
                          "256 MiB device copy (counter calibration); for rocprofv3 runs")
@@ -108,6 +196,15 @@ def main():
     cg_iters = st1.pcg_iters - st0.pcg_iters
     ms_per_admm = elapsed * 1e3 / max(admm_iters, 1)
 
+    # ---- whole ADMM iteration: algorithmic bytes / measured time ----------------
+    # PCG bytes from the library (same formulas as the kernels' accounting);
+    # host-core vector passes per ADMM iteration: compute_rhs 3n + 5m,
+    # update_x 6n, update_z 11m, update_y 10m element touches of 8 bytes
+    pb = eng.pcg_bytes()
+    cg_per_admm = cg_iters / max(admm_iters, 1)
+    admm_bytes = pb["per_solve"] + cg_per_admm * pb["per_cg_iter"] + 8.0 * (9 * n + 26 * m)
+    admm_gbs = admm_bytes / (ms_per_admm * 1e-3) / 1e9
+
     # ---- dominant kernels: CG operator, timed with HIP events ------------------
     kt = eng.time_kernels(reps=20)
     dom = "k_Kv" if kt["ms_kv"] >= kt["ms_av"] else "k_Av_rho"
@@ -155,6 +252,9 @@ def main():
                           f"{mean_iters:.0f} iterations per GPU solve")
         log(f"[bench] cpu oracle: {k_cpu} ADMM iterations in {tc:.1f}s")
 
+    # ---- served MPC stream: batched FP32-PCG engine vs plain FP64 engine -------
+    serving = None if args.no_serving else serve_mpc(args.mpc_requests, args.seed)
+
     out = {
         "metric": METRIC,
         "value": qps,
@@ -177,14 +277,18 @@ def main():
         "cg_iters_per_admm_iter": cg_iters / max(admm_iters, 1),
         "ms_per_admm_iter": ms_per_admm,
         "kkt_certificate": {"prim": cert32["prim"], "dual": cert32["dual"], "obj": cert32["obj"]},
+        "admm_iter": {"bytes": admm_bytes, "ms": ms_per_admm, "achieved_GBs": admm_gbs,
+                      "frac_of_peak": admm_gbs / HBM_PEAK_GBS, "bytes_per_cg_iter": pb["per_cg_iter"],
+                      "bytes_per_linsys_outer": pb["per_solve"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
                      "ms_per_launch": dom_ms, "bytes_per_launch": dom_bytes,
                      "other_kernel": {"name": "k_Av_rho" if dom == "k_Kv" else "k_Kv",
                                       "ms_per_launch": min(kt["ms_kv"], kt["ms_av"]),
                                       "bytes_per_launch": kt["bytes_av"] if dom == "k_Kv" else kt["bytes_kv"]}},
         "gpu_fp64_plain": fp64,
         "cpu_baseline": cpu,
+        "serving_mpc": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -156,6 +156,10 @@ void       osqp_amd_linsys_counters(LinSysSolver* s, OSQPInt* cg_iters, OSQPInt*
  * launches timed with HIP events, and the algorithmic bytes of one launch. */
 OSQPInt    osqp_amd_linsys_time_kernels(LinSysSolver* s, OSQPInt reps, double* ms_av, double* ms_kv,
                                         double* bytes_av, double* bytes_kv);
+/* Algorithmic bytes (each array element touched once per pass) of one CG
+ * iteration and of the per-solve work outside the CG loop (cached A x path:
+ * w = rho o (A x - b2), FP64 residual, correction, z = A x) of this linsys. */
+OSQPInt    osqp_amd_linsys_bytes(LinSysSolver* s, double* per_cg_iter, double* per_solve);
 /* Last HIP error seen by the backend (0 if none); cleared on read. */
 OSQPInt    osqp_amd_last_error(void);
 

@@ -76,6 +76,10 @@ OSQPInt osqp_batch_resolve(OSQPBatch* batch, OSQPInt id, const OSQPFloat* q_new)
  * or OSQP_MEM_ALLOC_ERROR when a HIP call failed. */
 OSQPInt osqp_batch_step(OSQPBatch* batch, OSQPInt* completed);
 
+/* Ids of the solves that ended in the last osqp_batch_step (at most cap of
+ * them written to ids); returns their number. */
+OSQPInt osqp_batch_last_completed(const OSQPBatch* batch, OSQPInt* ids, OSQPInt cap);
+
 /* 1 and the result (x: n, y: m, either may be null) when the problem's solve
  * has ended; 0 while it runs; -1 for an unknown id.  A non-resident problem is
  * released by a successful poll. */

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libosqp_amd.so")
 
 HIP_SOURCES = ["hip/runtime.hip", "hip/vector.hip", "hip/matrix.hip", "hip/panel.hip", "hip/pcg.hip", "serve/batch.hip"]
-C_SOURCES = ["host/auxil.c", "host/osqp_api.c"]
+C_SOURCES = ["host/auxil.c", "host/osqp_api.c", "host/polish.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

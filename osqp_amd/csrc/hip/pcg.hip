@@ -858,6 +858,10 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   const V* dinv = (const V*)s->dinv;
 
   s->host_solves++;
+  // the residual norms steer the tolerance of ADMM solves; a polishing solver
+  // has none (its tolerance follows ||rhs||)
+  const double spr = s->scaled_prim_res ? *s->scaled_prim_res : 0.0;
+  const double sdr = s->scaled_dual_res ? *s->scaled_dual_res : 0.0;
   const bool cached = m > 0 && s->ax_valid && s->ax_Aver == s->A->version && admm_iter > 1 && !s->polishing;
   int nres;
   if (!cached) {
@@ -866,14 +870,14 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
                        (const int*)s->krp, (const int*)s->kci, (const double*)s->kval64, n, (const double*)b->d, rho,
                        s->rho_sc, s->rhs64, part0);
     hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, gK, admm_iter,
-                       *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
+                       spr, sdr, s->polishing, s->tol_fraction,
                        s->reduction_threshold);
     // r64 = K x64 - rhs64 with the FP64 values (the exact K)
     if (m) op_av_rho<double>(s, (const double*)s->x64, s->w64);
     nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)s->rhs64, s->r64, part1);
   } else {
     hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)nullptr, 0, admm_iter,
-                       *s->scaled_prim_res, *s->scaled_dual_res, s->polishing, s->tol_fraction,
+                       spr, sdr, s->polishing, s->tol_fraction,
                        s->reduction_threshold);
     // w64 = rho o (A x64 - b2) from the cached A x64;  r64 = sigma x64 + P x64 + A' w64 - b1
     hipLaunchKernelGGL(k_w_from_ax, dim3(gM), dim3(kBlock), 0, S(), (const PcgState*)s->st, m,
@@ -1167,6 +1171,41 @@ OSQPInt osqp_amd_linsys_time_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_
   s->host_solves = 0;
   HIPCHK(hipStreamSynchronize(S()));
   if (s->h_flags) memset(s->h_flags, 0, sizeof(int) * (size_t)s->flags_len);
+  return 0;
+}
+
+OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per_solve) {
+  HipPcg* s = (HipPcg*)ls;
+  const double n = s->n, m = s->m, vb = s->bits == 32 ? 4.0 : 8.0;
+  const double nnzA = m ? (double)s->A->nnz_in : 0.0, nnzK = (double)s->P->nnz + nnzA;
+  // one SpMV (+ its combine): panels (panel_spmv_bytes + partials read once,
+  // epilogue vectors) or row-block CSR (values, 32-bit indices, row pointers,
+  // x, y)
+  auto av = [&](const PanelMatrix* P, double b) {
+    const double epi = m * (8.0 + b);  // rho read, w written
+    if (P) return panel_spmv_bytes(*P, (int)b) + (double)P->np * m * b + epi;
+    return nnzA * (b + 4.0) + (m + 1.0) * 4.0 + n * b + epi;
+  };
+  auto kv = [&](const PanelMatrix* P, double b) {
+    const double epi = 3.0 * n * b;  // p (sigma p), sub or nothing, Kp written
+    if (P) return panel_spmv_bytes(*P, (int)b) + (double)P->np * n * b + epi;
+    return nnzK * (b + 4.0) + (n + 1.0) * 4.0 + (n + m) * b + epi;
+  };
+  const PanelMatrix* A32 = s->bits == 32 ? panels_A<float>(s) : panels_A<double>(s);
+  const PanelMatrix* K32 = s->bits == 32 ? panels_K<float>(s) : panels_K<double>(s);
+  const PanelMatrix* A64 = panels_A<double>(s);
+  const PanelMatrix* K64 = panels_K<double>(s);
+  // CG iteration: A p, K p, update (d, p, r, Kp, dinv read; d, r, y written),
+  // p update (p, y read; p written)
+  const double cg = (m ? av(A32, vb) : 0.0) + kv(K32, vb) + 8.0 * n * vb + 3.0 * n * vb;
+  // per solve: w64 from the cached A x (ax, b2, rho read; w written), FP64
+  // residual, CG init (r64, dinv read; r, y, p, d written), correction (d,
+  // x64 read; x64, b1 written), z = A x64 (A panels; b2 and ax written: the
+  // same 16 bytes per row as av's rho read and w write)
+  const double outer = 4.0 * m * 8.0 + kv(K64, 8.0) + n * (8.0 + vb) + 4.0 * n * vb + n * (vb + 8.0) +
+                       2.0 * n * 8.0 + (m ? av(A64, 8.0) : 0.0);
+  if (per_cg_iter) *per_cg_iter = cg;
+  if (per_solve) *per_solve = outer;
   return 0;
 }
 

@@ -381,6 +381,21 @@ void update_info(OSQPSolver* solver, OSQPInt iter) {
   work->summary_printed = 0;
 }
 
+/* auxil.c:676-762 (polishing branch): residuals and objective of the
+ * polished point; the duality-gap integral is not accumulated */
+void update_info_polish(OSQPSolver* solver) {
+  OSQPInfo* info = solver->info;
+  OSQPWorkspace* work = solver->work;
+  if (work->data->m == 0) work->pol_prim_res = 0.;
+  else work->pol_prim_res = compute_prim_res(solver, work->pol_x, work->pol_z);
+  work->pol_dual_res = compute_dual_res(solver, work->pol_x, work->pol_y);
+  compute_obj_val_dual_gap(solver, work->pol_x, work->pol_y, &work->pol_obj_val, &work->pol_dual_obj_val,
+                           &work->pol_duality_gap);
+  info->polish_time = osqp_amd_toc(work);
+  info->rel_kkt_error = c_max(c_max(work->pol_dual_res, work->pol_prim_res), work->pol_duality_gap);
+  work->summary_printed = 0;
+}
+
 /* auxil.c:765-783 */
 void reset_info(OSQPInfo* info) {
   info->solve_time = 0.0;

@@ -57,6 +57,13 @@ struct OSQPWorkspace_ {
   OSQPInt rho_updated;
   OSQPFloat last_rel_kkt;
   OSQPInt summary_printed;
+  /* polishing (OSQPPolish, include/private/types.h:47-62) */
+  OSQPMatrix*  pol_Ared;
+  OSQPVectori* pol_active;
+  OSQPVectorf *pol_x, *pol_z, *pol_y;
+  OSQPInt      pol_n_active;
+  OSQPFloat    pol_obj_val, pol_dual_obj_val, pol_duality_gap, pol_prim_res, pol_dual_res;
+  OSQPFloat    polish_t;  /* run time of the last polish */
 };
 
 /* timer (src/timing_linux.c:29-47) */
@@ -78,6 +85,9 @@ void      compute_obj_val_dual_gap(const OSQPSolver* solver, const OSQPVectorf* 
 OSQPInt   has_solution(const OSQPInfo* info);
 void      store_solution(OSQPSolver* solver, OSQPSolution* solution);
 void      update_info(OSQPSolver* solver, OSQPInt iter);
+void      update_info_polish(OSQPSolver* solver);
+/* polish (src/polish.c) */
+OSQPInt   polish(OSQPSolver* solver);
 void      reset_info(OSQPInfo* info);
 void      update_status(OSQPInfo* info, OSQPInt status_val);
 OSQPInt   check_termination(OSQPSolver* solver, OSQPInt approximate);

@@ -185,6 +185,11 @@ OSQPInt osqp_setup(OSQPSolver** solverp, const OSQPCscMatrix* P, const OSQPFloat
   work->delta_y = OSQPVectorf_calloc(m);
   work->Atdelta_y = OSQPVectorf_calloc(n);
   work->delta_x = OSQPVectorf_calloc(n);
+  work->pol_active = OSQPVectori_calloc(m);
+  work->pol_x = OSQPVectorf_calloc(n);
+  work->pol_z = OSQPVectorf_calloc(m);
+  work->pol_y = OSQPVectorf_calloc(m);
+  ALLOC_OR_FAIL(work->pol_active && work->pol_x && work->pol_z && work->pol_y);
   work->Pdelta_x = OSQPVectorf_calloc(n);
   work->Adelta_x = OSQPVectorf_calloc(m);
   ALLOC_OR_FAIL(work->Ax && work->Px && work->Aty && work->delta_y && work->Atdelta_y);
@@ -394,6 +399,16 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
   osqp_amd_algebra_sync();
   solver->info->solve_time = osqp_amd_toc(work);
 
+  /* src/osqp_api.c:981-990 */
+  solver->info->polish_time = 0.0;
+  if (settings->polishing && (solver->info->status_val == OSQP_SOLVED)) {
+    exitflag = polish(solver);
+    if (exitflag > 0) {
+      fprintf(stderr, "ERROR in osqp_solve: Failed polishing\n");
+      goto exit;
+    }
+  }
+
   if (work->first_run)
     solver->info->run_time = solver->info->setup_time + solver->info->solve_time + solver->info->polish_time;
   else
@@ -459,6 +474,10 @@ OSQPInt osqp_cleanup(OSQPSolver* solver) {
     OSQPVectorf_free(work->delta_y);
     OSQPVectorf_free(work->Atdelta_y);
     OSQPVectorf_free(work->delta_x);
+    OSQPVectori_free(work->pol_active);
+    OSQPVectorf_free(work->pol_x);
+    OSQPVectorf_free(work->pol_z);
+    OSQPVectorf_free(work->pol_y);
     OSQPVectorf_free(work->Pdelta_x);
     OSQPVectorf_free(work->Adelta_x);
     if (solver->settings) free(solver->settings);
@@ -671,4 +690,9 @@ OSQPInt osqp_amd_time_pcg_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_a
                                   double* bytes_kv) {
   if (!solver || !solver->work || !solver->work->linsys_solver) return 1;
   return osqp_amd_linsys_time_kernels(solver->work->linsys_solver, reps, ms_av, ms_kv, bytes_av, bytes_kv);
+}
+
+OSQPInt osqp_amd_pcg_bytes(OSQPSolver* solver, double* per_cg_iter, double* per_solve) {
+  if (!solver || !solver->work || !solver->work->linsys_solver) return 1;
+  return osqp_amd_linsys_bytes(solver->work->linsys_solver, per_cg_iter, per_solve);
 }

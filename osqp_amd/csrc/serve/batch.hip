@@ -455,6 +455,7 @@ struct Engine {
   RangeAlloc pool_alloc;
   std::map<int, HProb*> probs;
   std::vector<int> queue;  // waiting for admission, in arrival order
+  std::vector<int> last_done;  // solves that ended in the last epoch
   int next_id = 1;
   int n_active = 0;
   Bucket buckets[kBuckets];
@@ -801,6 +802,7 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
   if (!b) return OSQP_MEM_ALLOC_ERROR;
   Engine* E = &b->e;
   if (E->failed) return OSQP_MEM_ALLOC_ERROR;
+  E->last_done.clear();
   // admission in arrival order while slots and memory allow
   size_t kept = 0;
   for (size_t k = 0; k < E->queue.size(); ++k) {
@@ -870,6 +872,7 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
         p->state = kEnded;
         p->latency = t_now - p->t_req;
         E->n_active--;
+        E->last_done.push_back(p->id);
         done++;
       } else {
         still.push_back(p->id);
@@ -893,6 +896,14 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
   E->epochs++;
   if (completed) *completed = done;
   return 0;
+}
+
+OSQPInt osqp_batch_last_completed(const OSQPBatch* b, OSQPInt* ids, OSQPInt cap) {
+  if (!b) return 0;
+  const std::vector<int>& d = b->e.last_done;
+  if (ids)
+    for (size_t k = 0; k < d.size() && (OSQPInt)k < cap; ++k) ids[k] = d[k];
+  return (OSQPInt)d.size();
 }
 
 OSQPInt osqp_batch_poll(OSQPBatch* b, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OSQPBatchInfo* info) {

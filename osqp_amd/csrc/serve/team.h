@@ -135,35 +135,35 @@ struct BProb {
 struct Team {
   int tid, T, lane, wv, W;
   double* red;  // LDS, kMaxRedBlocks doubles
-  __device__ void sync() const { __syncthreads(); }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
-__device__ inline double dmax(double a, double b) { return a > b ? a : b; }
-__device__ inline double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 
 template <typename V>
 struct KahanAcc {
   V s = 0, c = 0;
-  __device__ void add(V v) {
+  __device__ __forceinline__ void add(V v) {
     const V y = v - c;
     const V t = s + y;
     c = (t - s) - y;
     s = t;
   }
-  __device__ double value() const { return (double)s - (double)c; }
+  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 template <>
 struct KahanAcc<double> {
   double s = 0;
-  __device__ void add(double v) { s += v; }
-  __device__ double value() const { return s; }
+  __device__ __forceinline__ void add(double v) { s += v; }
+  __device__ __forceinline__ double value() const { return s; }
 };
 
 // sum (MAX = false) or max (MAX = true, of non-negative terms) of f(i), i < n;
 // block-structured, independent of the team size.  Ends with every thread
 // holding the result; contains two barriers.
 template <bool MAX, typename F>
-__device__ double team_reduce(const Team& t, int n, F f) {
+__device__ __forceinline__ double team_reduce(const Team& t, int n, F f) {
   const int nb = (n + kRedBlock - 1) / kRedBlock;
   for (int b = t.wv; b < nb; b += t.W) {
     double acc = 0.0;
@@ -187,7 +187,7 @@ __device__ double team_reduce(const Team& t, int n, F f) {
 
 // dot product in the CG precision V (Kahan-compensated in FP32), same blocking
 template <typename V, typename F>
-__device__ double team_dot(const Team& t, int n, F f) {
+__device__ __forceinline__ double team_dot(const Team& t, int n, F f) {
   const int nb = (n + kRedBlock - 1) / kRedBlock;
   for (int b = t.wv; b < nb; b += t.W) {
     KahanAcc<V> acc;
@@ -206,42 +206,47 @@ __device__ double team_dot(const Team& t, int n, F f) {
   return s;
 }
 
-// y-rows of a CSR product: for each row r, epi(r, sum_k elem(k)) with the row
-// reduced by L lanes (short rows) or a wavefront (long rows).  Caller syncs.
+// y-rows of a CSR product: for each row r, epi(r, sum_k elem(val_k, col_k))
+// with the row reduced by L lanes (short rows) or a wavefront (long rows).
+// The matrix is taken by value so its pointers live in registers.  Caller syncs.
 template <typename TX, typename Elem, typename Epi>
-__device__ void team_rows(const Team& t, const BCsr& M, Elem elem, Epi epi) {
-  const int L = M.L;
+__device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem, Epi epi) {
+  const int* __restrict__ rp = M.rp;
+  const int* __restrict__ ci = M.ci;
+  const double* __restrict__ val = M.val;
+  const int L = M.L, rows = M.rows;
   const int G = t.T / L;
   const int g = t.tid / L, gl = t.tid % L;
-  for (int r0 = 0; r0 < M.rows; r0 += G) {
+  for (int r0 = 0; r0 < rows; r0 += G) {
     const int r = r0 + g;
-    const bool own = r < M.rows;
+    const bool own = r < rows;
     int s = 0, e = 0;
     if (own) {
-      s = M.rp[r];
-      e = M.rp[r + 1];
+      s = rp[r];
+      e = rp[r + 1];
     }
     const bool lng = (e - s) > kLongRow;
     TX acc = 0;
     if (own && !lng)
-      for (int k = s + gl; k < e; k += L) acc += elem(k);
+      for (int k = s + gl; k < e; k += L) acc += elem(val[k], ci[k]);
     for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
     if (own && !lng && gl == 0) epi(r, acc);
   }
   for (int j = t.wv; j < M.nlong; j += t.W) {
     const int r = M.longs[j];
-    const int s = M.rp[r], e = M.rp[r + 1];
+    const int s = rp[r], e = rp[r + 1];
     TX acc = 0;
-    for (int k = s + t.lane; k < e; k += kWave) acc += elem(k);
+    for (int k = s + t.lane; k < e; k += kWave) acc += elem(val[k], ci[k]);
     acc = osqp_hip::wave_sum(acc);
     if (t.lane == 0) epi(r, acc);
   }
 }
 
 // y = M x (FP64); caller syncs before reading y
-__device__ inline void team_spmv(const Team& t, const BCsr& M, const double* x, double* y) {
+__device__ __forceinline__ void team_spmv(const Team& t, const BCsr M, const double* __restrict__ x,
+                                          double* __restrict__ y) {
   team_rows<double>(
-      t, M, [&](int k) { return M.val[k] * x[M.ci[k]]; }, [&](int r, double s) { y[r] = s; });
+      t, M, [=](double v, int c) { return v * x[c]; }, [=](int r, double s) { y[r] = s; });
 }
 
 // ---- the ADMM step of one problem -------------------------------------------
@@ -253,36 +258,36 @@ struct Solver {
   const Team& t;
   const BProb& p;
   BState S;
-  __device__ Solver(const Team& tt, const BProb& pp) : t(tt), p(pp), S(pp.st) {}
+  __device__ __forceinline__ Solver(const Team& tt, const BProb& pp) : t(tt), p(pp), S(pp.st) {}
 
-  __device__ double* x() { return S.swapped ? p.xb : p.xa; }
-  __device__ double* xprev() { return S.swapped ? p.xa : p.xb; }
-  __device__ double* z() { return S.swapped ? p.zb : p.za; }
-  __device__ double* zprev() { return S.swapped ? p.za : p.zb; }
-  __device__ bool unscaled_term() const { return p.set.scaling && !p.set.scaled_term; }
+  __device__ __forceinline__ double* x() { return S.swapped ? p.xb : p.xa; }
+  __device__ __forceinline__ double* xprev() { return S.swapped ? p.xa : p.xb; }
+  __device__ __forceinline__ double* z() { return S.swapped ? p.zb : p.za; }
+  __device__ __forceinline__ double* zprev() { return S.swapped ? p.za : p.zb; }
+  __device__ __forceinline__ bool unscaled_term() const { return p.set.scaling && !p.set.scaled_term; }
 
-  __device__ double norm_inf(const double* v, int n) {
+  __device__ __forceinline__ double norm_inf(const double* v, int n) {
     return team_reduce<true>(t, n, [&](int i) { return fabs(v[i]); });
   }
-  __device__ double scaled_norm_inf(const double* s, const double* v, int n) {
+  __device__ __forceinline__ double scaled_norm_inf(const double* s, const double* v, int n) {
     return team_reduce<true>(t, n, [&](int i) { return fabs(s[i] * v[i]); });
   }
-  __device__ double dot(const double* a, const double* b, int n) {
+  __device__ __forceinline__ double dot(const double* a, const double* b, int n) {
     return team_reduce<false>(t, n, [&](int i) { return a[i] * b[i]; });
   }
 
   // ---- scaling (src/scaling.c:48-164) ----------------------------------------
-  __device__ static double limit_scalar(double v) {
+  __device__ __forceinline__ static double limit_scalar(double v) {
     v = v < kMinScaling ? 1.0 : v;
     return v > kMaxScaling ? kMaxScaling : v;
   }
-  __device__ static double limit_elem(double v) {
+  __device__ __forceinline__ static double limit_elem(double v) {
     v = v < kMinScaling ? 1.0 : v;
     return v > kMaxScaling ? kMaxScaling : v;
   }
   // P entry (row r, col c) scaled as the triangular storage does:
   // (v * s[min]) * s[max]
-  __device__ void scale_P(const double* s) {
+  __device__ __forceinline__ void scale_P(const double* s) {
     for (int r = t.tid; r < p.n; r += t.T)
       for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) {
         const int c = p.P.ci[k];
@@ -291,13 +296,13 @@ struct Solver {
       }
   }
   // A (i, j) and A' (j, i): (v * E_i) * D_j in both copies
-  __device__ void scale_A(const double* e, const double* d) {
+  __device__ __forceinline__ void scale_A(const double* e, const double* d) {
     for (int r = t.tid; r < p.m; r += t.T)
       for (int k = p.A.rp[r]; k < p.A.rp[r + 1]; ++k) p.A.val[k] = (p.A.val[k] * e[r]) * d[p.A.ci[k]];
     for (int r = t.tid; r < p.n; r += t.T)
       for (int k = p.At.rp[r]; k < p.At.rp[r + 1]; ++k) p.At.val[k] = (p.At.val[k] * e[p.At.ci[k]]) * d[r];
   }
-  __device__ void scale_data() {
+  __device__ __forceinline__ void scale_data() {
     const int n = p.n, m = p.m;
     double* Dt = p.tn;    // D_temp
     double* DtA = p.tn2;  // D_temp_A
@@ -363,7 +368,7 @@ struct Solver {
     }
     t.sync();
   }
-  __device__ void no_scaling() {
+  __device__ __forceinline__ void no_scaling() {
     S.c = 1.0;
     S.cinv = 1.0;
     for (int i = t.tid; i < p.n; i += t.T) p.D[i] = p.Dinv[i] = 1.0;
@@ -373,7 +378,7 @@ struct Solver {
 
   // ---- rho (src/auxil.c:76-122, osqp_api.c osqp_update_rho) -------------------
   // classify constraints and set rho / 1/rho; returns whether a type changed
-  __device__ int set_rho_vec() {
+  __device__ __forceinline__ int set_rho_vec() {
     S.rho = dmin(dmax(S.rho, kRhoMin), kRhoMax);
     const double infval = kInfty * kMinScaling;
     const double changed = team_reduce<true>(t, p.m, [&](int i) {
@@ -397,7 +402,7 @@ struct Solver {
     t.sync();
     return changed > 0.0;
   }
-  __device__ void update_rho(double rho_new) {
+  __device__ __forceinline__ void update_rho(double rho_new) {
     S.rho = dmin(dmax(rho_new, kRhoMin), kRhoMax);
     for (int i = t.tid; i < p.m; i += t.T) {
       const int ty = p.ctype[i];
@@ -410,19 +415,20 @@ struct Solver {
   }
 
   // Jacobi preconditioner: 1 / (sigma + diag(P) + sum_j rho_j A_ji^2)
-  __device__ void update_precond() {
-    V* dinv = (V*)p.cdinv;
+  __device__ __forceinline__ void update_precond() {
+    V* __restrict__ dinv = (V*)p.cdinv;
+    const double* __restrict__ rho = p.rho;
+    double* __restrict__ prec = p.prec;
+    const int* __restrict__ pdiag = p.Pdiag;
+    const double* __restrict__ pval = p.P.val;
+    const double sigma = p.set.sigma;
+    const int n = p.n;
     team_rows<double>(
-        t, p.At,
-        [&](int k) {
-          const double a = p.At.val[k];
-          return a * a * p.rho[p.At.ci[k]];
-        },
-        [&](int r, double s) { p.prec[r] = s; });
+        t, p.At, [=](double a, int c) { return a * a * rho[c]; }, [=](int r, double s) { prec[r] = s; });
     t.sync();
-    for (int i = t.tid; i < p.n; i += t.T) {
-      const double pd = p.Pdiag[i] >= 0 ? p.P.val[p.Pdiag[i]] : 0.0;
-      const double d = p.set.sigma + pd + p.prec[i];
+    for (int i = t.tid; i < n; i += t.T) {
+      const double pd = pdiag[i] >= 0 ? pval[pdiag[i]] : 0.0;
+      const double d = sigma + pd + prec[i];
       dinv[i] = (V)(1.0 / d);
     }
     t.sync();
@@ -430,16 +436,24 @@ struct Solver {
 
   // ---- PCG with FP64 defect correction (one linear solve) ----------------------
   // b = [b1 (n); b2 (m)] in/out: out = [x~ ; A x~]
-  __device__ void linsys_solve(double* b1, double* b2, int admm_iter) {
+  __device__ __forceinline__ void linsys_solve(double* __restrict__ b1, double* __restrict__ b2, int admm_iter) {
     const int n = p.n, m = p.m;
     const double sg = p.set.sigma;
-    V* d = (V*)p.cd;
-    V* pp = (V*)p.cp;
-    V* Kp = (V*)p.cKp;
-    V* r = (V*)p.cr;
-    V* y = (V*)p.cy;
-    V* w = (V*)p.cw;
-    const V* dinv = (const V*)p.cdinv;
+    const int cg_max = p.set.cg_max_iter;
+    V* __restrict__ d = (V*)p.cd;
+    V* __restrict__ pp = (V*)p.cp;
+    V* __restrict__ Kp = (V*)p.cKp;
+    V* __restrict__ r = (V*)p.cr;
+    V* __restrict__ y = (V*)p.cy;
+    V* __restrict__ w = (V*)p.cw;
+    const V* __restrict__ dinv = (const V*)p.cdinv;
+    const double* __restrict__ rho = p.rho;
+    double* __restrict__ x64 = p.x64;
+    double* __restrict__ r64 = p.r64;
+    double* __restrict__ rhs64 = p.rhs64;
+    double* __restrict__ w64 = p.w64;
+    double* __restrict__ ax64 = p.ax64;
+    const BCsr A = p.A, P = p.P, At = p.At;
     const bool guard = sizeof(V) == 4;
     // zero-iteration bookkeeping of the previous solve (cuda_pcg_interface.cu:266-268)
     if (S.cg_solves > 0) {
@@ -448,14 +462,13 @@ struct Solver {
     }
     const bool cached = m > 0 && S.ax_valid && admm_iter > 1;
     double eps;
-    double rn;
     if (!cached) {
       // rhs64 = b1 + A'(rho o b2); ||rhs64||_inf
       team_rows<double>(
-          t, p.At, [&](int k) { return p.At.val[k] * (p.rho[p.At.ci[k]] * b2[p.At.ci[k]]); },
-          [&](int row, double s) { p.rhs64[row] = b1[row] + s; });
+          t, At, [=](double v, int c) { return v * (rho[c] * b2[c]); },
+          [=](int row, double s) { rhs64[row] = b1[row] + s; });
       t.sync();
-      const double rhs_norm = norm_inf(p.rhs64, n);
+      const double rhs_norm = norm_inf(rhs64, n);
       if (admm_iter == 1) {
         S.cg_red_factor = p.set.cg_tol_fraction;
         S.cg_eps_prev = (rhs_norm < kCgTolMin) ? 1.0 : rhs_norm * S.cg_red_factor;
@@ -466,25 +479,24 @@ struct Solver {
       // w64 = rho o (A x64); r64 = sigma x64 + P x64 + A' w64 - rhs64
       if (m) {
         team_rows<double>(
-            t, p.A, [&](int k) { return p.A.val[k] * p.x64[p.A.ci[k]]; },
-            [&](int row, double s) { p.w64[row] = p.rho[row] * s; });
+            t, A, [=](double v, int c) { return v * x64[c]; }, [=](int row, double s) { w64[row] = rho[row] * s; });
         t.sync();
       }
-      residual64(p.rhs64);
+      residual64(rhs64);
     } else {
       eps = next_tolerance();
-      for (int i = t.tid; i < m; i += t.T) p.w64[i] = p.rho[i] * (p.ax64[i] - b2[i]);
+      for (int i = t.tid; i < m; i += t.T) w64[i] = rho[i] * (ax64[i] - b2[i]);
       t.sync();
       residual64(b1);
     }
-    rn = norm_inf(p.r64, n);
+    const double rn = norm_inf(r64, n);
     S.cg_solves++;
     const double scale = (rn > 0.0 && isfinite(rn)) ? rn : 1.0;
     const double eps_cmp = eps / scale;
     const double inv = 1.0 / scale;
     // CG on K d = -r64 / scale
     for (int i = t.tid; i < n; i += t.T) {
-      const V ri = (V)(p.r64[i] * inv);
+      const V ri = (V)(r64[i] * inv);
       const V yi = dinv[i] * ri;
       r[i] = ri;
       y[i] = yi;
@@ -492,30 +504,29 @@ struct Solver {
       d[i] = (V)0;
     }
     t.sync();
-    double rty = team_dot<V>(t, n, [&](int i) { return y[i] * r[i]; });
-    double rnorm = team_reduce<true>(t, n, [&](int i) { return fabs((double)r[i]); });
+    double rty = team_dot<V>(t, n, [=](int i) { return y[i] * r[i]; });
+    double rnorm = team_reduce<true>(t, n, [=](int i) { return fabs((double)r[i]); });
     int it = 0;
-    bool go = rnorm > eps_cmp && it < p.set.cg_max_iter;
+    bool go = rnorm > eps_cmp && it < cg_max;
     if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
     while (go) {
       if (m) {
         team_rows<V>(
-            t, p.A, [&](int k) { return (V)p.A.val[k] * pp[p.A.ci[k]]; },
-            [&](int row, V s) { w[row] = (V)p.rho[row] * s; });
+            t, A, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { w[row] = (V)rho[row] * s; });
         t.sync();
       }
       // Kp = sigma p + P p + A' w
       team_rows<V>(
-          t, p.P, [&](int k) { return (V)p.P.val[k] * pp[p.P.ci[k]]; }, [&](int row, V s) { Kp[row] = s; });
+          t, P, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { Kp[row] = s; });
       t.sync();
       if (m)
         team_rows<V>(
-            t, p.At, [&](int k) { return (V)p.At.val[k] * w[p.At.ci[k]]; },
-            [&](int row, V s) { Kp[row] = (V)sg * pp[row] + Kp[row] + s; });
+            t, At, [=](double v, int c) { return (V)v * w[c]; },
+            [=](int row, V s) { Kp[row] = (V)sg * pp[row] + Kp[row] + s; });
       else
         for (int i = t.tid; i < n; i += t.T) Kp[i] = (V)sg * pp[i] + Kp[i];
       t.sync();
-      const double pkp = team_dot<V>(t, n, [&](int i) { return pp[i] * Kp[i]; });
+      const double pkp = team_dot<V>(t, n, [=](int i) { return pp[i] * Kp[i]; });
       if (guard && !(pkp > 0.0 && isfinite(pkp))) break;
       const V alpha = (V)(rty / pkp);
       for (int i = t.tid; i < n; i += t.T) {
@@ -525,12 +536,12 @@ struct Solver {
         y[i] = dinv[i] * ri;
       }
       t.sync();
-      const double rty_new = team_dot<V>(t, n, [&](int i) { return y[i] * r[i]; });
-      rnorm = team_reduce<true>(t, n, [&](int i) { return fabs((double)r[i]); });
+      const double rty_new = team_dot<V>(t, n, [=](int i) { return y[i] * r[i]; });
+      rnorm = team_reduce<true>(t, n, [=](int i) { return fabs((double)r[i]); });
       const double beta = rty_new / rty;
       it++;
       rty = rty_new;
-      go = rnorm > eps_cmp && it < p.set.cg_max_iter;
+      go = rnorm > eps_cmp && it < cg_max;
       if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
       if (go) {
         const V vb = (V)beta;
@@ -542,17 +553,17 @@ struct Solver {
     S.cg_total += it;
     // x64 += scale d; b1 = x64; b2 = A x64 (the A x cache)
     for (int i = t.tid; i < n; i += t.T) {
-      double xi = p.x64[i];
+      double xi = x64[i];
       if (it > 0) xi += scale * (double)d[i];
-      p.x64[i] = xi;
+      x64[i] = xi;
       b1[i] = xi;
     }
     t.sync();
     if (m) {
       team_rows<double>(
-          t, p.A, [&](int k) { return p.A.val[k] * p.x64[p.A.ci[k]]; },
-          [&](int row, double s) {
-            p.ax64[row] = s;
+          t, A, [=](double v, int c) { return v * x64[c]; },
+          [=](int row, double s) {
+            ax64[row] = s;
             b2[row] = s;
           });
       t.sync();
@@ -560,7 +571,7 @@ struct Solver {
     }
   }
   // tolerance of a non-first ADMM iteration (cuda_pcg_interface.cu:46-60)
-  __device__ double next_tolerance() {
+  __device__ __forceinline__ double next_tolerance() {
     if (S.cg_zero_iters >= p.set.cg_red_thresh) {
       S.cg_red_factor /= 2;
       S.cg_zero_iters = 0;
@@ -572,22 +583,27 @@ struct Solver {
     return eps;
   }
   // r64 = sigma x64 + P x64 + A' w64 - rhs
-  __device__ void residual64(const double* rhs) {
+  __device__ __forceinline__ void residual64(const double* __restrict__ rhs) {
     const int n = p.n;
+    const double sigma = p.set.sigma;
+    const double* __restrict__ x64 = p.x64;
+    const double* __restrict__ w64 = p.w64;
+    double* __restrict__ tn = p.tn;
+    double* __restrict__ r64 = p.r64;
     team_rows<double>(
-        t, p.P, [&](int k) { return p.P.val[k] * p.x64[p.P.ci[k]]; }, [&](int row, double s) { p.tn[row] = s; });
+        t, p.P, [=](double v, int c) { return v * x64[c]; }, [=](int row, double s) { tn[row] = s; });
     t.sync();
     if (p.m)
       team_rows<double>(
-          t, p.At, [&](int k) { return p.At.val[k] * p.w64[p.At.ci[k]]; },
-          [&](int row, double s) { p.r64[row] = p.set.sigma * p.x64[row] + p.tn[row] + s - rhs[row]; });
+          t, p.At, [=](double v, int c) { return v * w64[c]; },
+          [=](int row, double s) { r64[row] = sigma * x64[row] + tn[row] + s - rhs[row]; });
     else
-      for (int i = t.tid; i < n; i += t.T) p.r64[i] = p.set.sigma * p.x64[i] + p.tn[i] - rhs[i];
+      for (int i = t.tid; i < n; i += t.T) r64[i] = sigma * x64[i] + tn[i] - rhs[i];
     t.sync();
   }
 
   // ---- ADMM pieces (src/auxil.c) ----------------------------------------------
-  __device__ void cold_start() {
+  __device__ __forceinline__ void cold_start() {
     for (int i = t.tid; i < p.n; i += t.T) {
       x()[i] = 0.0;
       p.x64[i] = 0.0;
@@ -600,7 +616,7 @@ struct Solver {
     t.sync();
   }
   // osqp_warm_start (osqp_api.c:1260-1293) from raw x0 / y0
-  __device__ void warm_start(const double* x0, const double* y0) {
+  __device__ __forceinline__ void warm_start(const double* x0, const double* y0) {
     if (x0) {
       for (int i = t.tid; i < p.n; i += t.T) {
         double v = x0[i];
@@ -620,7 +636,7 @@ struct Solver {
       double* zz = z();
       double* xx = x();
       team_rows<double>(
-          t, p.A, [&](int k) { return p.A.val[k] * xx[p.A.ci[k]]; }, [&](int row, double s) { zz[row] = s; });
+          t, p.A, [=](double v, int c) { return v * xx[c]; }, [=](int row, double s) { zz[row] = s; });
       t.sync();
     }
     for (int i = t.tid; i < p.n; i += t.T) p.x64[i] = x()[i];
@@ -628,7 +644,7 @@ struct Solver {
     t.sync();
   }
 
-  __device__ void compute_obj_gap(const double* xx, const double* yy) {
+  __device__ __forceinline__ void compute_obj_gap(const double* xx, const double* yy) {
     const double quad = dot(p.Px, xx, p.n);
     const double lin = dot(p.q, xx, p.n);
     const double infval = kInfty * kMinScaling;
@@ -664,7 +680,7 @@ struct Solver {
     S.SC = sup;
   }
 
-  __device__ double compute_prim_res() {
+  __device__ __forceinline__ double compute_prim_res() {
     double* xx = x();
     const double* zz = z();
     team_spmv(t, p.A, xx, p.Ax);
@@ -675,7 +691,7 @@ struct Solver {
     if (unscaled_term()) return scaled_norm_inf(p.Einv, p.tm, p.m);
     return S.scaled_prim_res;
   }
-  __device__ double compute_dual_res() {
+  __device__ __forceinline__ double compute_dual_res() {
     double* xx = x();
     team_spmv(t, p.P, xx, p.Px);
     if (p.m) team_spmv(t, p.At, p.y, p.Aty);
@@ -690,7 +706,7 @@ struct Solver {
     if (unscaled_term()) return S.cinv * scaled_norm_inf(p.Dinv, p.tn, p.n);
     return S.scaled_dual_res;
   }
-  __device__ void update_info(int iter) {
+  __device__ __forceinline__ void update_info(int iter) {
     (void)iter;
     S.prim_res = p.m == 0 ? 0.0 : compute_prim_res();
     S.dual_res = compute_dual_res();
@@ -699,7 +715,7 @@ struct Solver {
     S.rel_kkt = dmax(dmax(S.dual_res, S.prim_res), S.gap);
   }
 
-  __device__ double prim_tol(double ea, double er) {
+  __device__ __forceinline__ double prim_tol(double ea, double er) {
     double a, b;
     if (unscaled_term()) {
       a = scaled_norm_inf(p.Einv, z(), p.m);
@@ -710,7 +726,7 @@ struct Solver {
     }
     return ea + er * dmax(a, b);
   }
-  __device__ double dual_tol(double ea, double er) {
+  __device__ __forceinline__ double dual_tol(double ea, double er) {
     double r, s;
     if (unscaled_term()) {
       r = scaled_norm_inf(p.Dinv, p.q, p.n);
@@ -728,14 +744,14 @@ struct Solver {
     }
     return ea + er * r;
   }
-  __device__ double gap_tol(double ea, double er) {
+  __device__ __forceinline__ double gap_tol(double ea, double er) {
     double r = fabs(S.xtPx);
     r = dmax(r, fabs(S.qtx));
     r = dmax(r, fabs(S.SC));
     if (unscaled_term()) r = S.cinv * r;
     return ea + er * r;
   }
-  __device__ bool is_primal_infeasible(double eps_pinf) {
+  __device__ __forceinline__ bool is_primal_infeasible(double eps_pinf) {
     const double infval = kInfty * kMinScaling;
     for (int i = t.tid; i < p.m; i += t.T) {
       double v = p.dy[i];
@@ -765,7 +781,7 @@ struct Solver {
     }
     return false;
   }
-  __device__ bool is_dual_infeasible(double eps_dinf) {
+  __device__ __forceinline__ bool is_dual_infeasible(double eps_dinf) {
     double ndx, cs;
     if (unscaled_term()) {
       ndx = scaled_norm_inf(p.D, p.dx, p.n);
@@ -799,7 +815,7 @@ struct Solver {
     return false;
   }
   // returns 1 when the solve ends (src/auxil.c:808-945)
-  __device__ int check_termination(int approximate) {
+  __device__ __forceinline__ int check_termination(int approximate) {
     double ea = p.set.eps_abs, er = p.set.eps_rel, epi = p.set.eps_pinf, edi = p.set.eps_dinf;
     if ((S.prim_res > kInfty) || (S.dual_res > kInfty)) {
       S.status = kNonCvx;
@@ -851,7 +867,7 @@ struct Solver {
     return 0;
   }
 
-  __device__ double rho_estimate() {
+  __device__ __forceinline__ double rho_estimate() {
     const double pr = S.scaled_prim_res / (dmax(norm_inf(z(), p.m), norm_inf(p.Ax, p.m)) + kDivisionTol);
     double dn = dmax(norm_inf(p.q, p.n), norm_inf(p.Aty, p.n));
     dn = dmax(dn, norm_inf(p.Px, p.n));
@@ -860,7 +876,7 @@ struct Solver {
     return dmin(dmax(est, kRhoMin), kRhoMax);
   }
 
-  __device__ bool has_solution() const {
+  __device__ __forceinline__ bool has_solution() const {
     const int s = S.status;
     return s != kPrimalInfeasible && s != kPrimalInfeasibleInaccurate && s != kDualInfeasible &&
            s != kDualInfeasibleInaccurate && s != kNonCvx;
@@ -868,7 +884,7 @@ struct Solver {
 
   // setup of a new problem (osqp_setup: scale_data, set_rho_vec, linsys
   // init, cold start) and an optional osqp_warm_start
-  __device__ void setup() {
+  __device__ __forceinline__ void setup() {
     const int4* s4 = (const int4*)p.stage;
     int4* d4 = (int4*)p.arena_static;
     const long long n4 = p.stage_bytes / 16;
@@ -905,7 +921,7 @@ struct Solver {
 
   // a new solve of a resident problem: osqp_update_data_vec(q) (if given),
   // reset_info, and osqp_solve's cold start when warm starting is off
-  __device__ void new_solve(const double* q_new) {
+  __device__ __forceinline__ void new_solve(const double* q_new) {
     if (q_new) {
       for (int i = t.tid; i < p.n; i += t.T) {
         double v = q_new[i];
@@ -923,7 +939,7 @@ struct Solver {
   }
 
   // end of a solve: objective, final status, unscaled solution to the host
-  __device__ void finish() {
+  __device__ __forceinline__ void finish() {
     if (!S.can_check) {
       update_info(S.iter);
       check_termination(0);
@@ -965,37 +981,48 @@ struct Solver {
   }
 
   // one ADMM iteration; returns 1 when the solve has ended
-  __device__ int iterate() {
+  __device__ __forceinline__ int iterate() {
     const int iter = ++S.iter;
-    const double a = p.set.alpha;
+    const double a = p.set.alpha, sigma = p.set.sigma;
+    const int n = p.n, m = p.m;
     S.swapped ^= 1;  // swap(x, x_prev), swap(z, z_prev)
-    double* xx = x();
-    double* xp = xprev();
-    double* zz = z();
-    double* zp = zprev();
+    double* __restrict__ xx = x();
+    const double* __restrict__ xp = xprev();
+    double* __restrict__ zz = z();
+    const double* __restrict__ zp = zprev();
+    double* __restrict__ xt = p.xt;
+    double* __restrict__ zt = p.zt;
+    double* __restrict__ y = p.y;
+    double* __restrict__ dx = p.dx;
+    double* __restrict__ dy = p.dy;
+    const double* __restrict__ q = p.q;
+    const double* __restrict__ rho = p.rho;
+    const double* __restrict__ rhoinv = p.rhoinv;
+    const double* __restrict__ l = p.l;
+    const double* __restrict__ u = p.u;
     // compute_rhs
-    for (int i = t.tid; i < p.n; i += t.T) p.xt[i] = p.set.sigma * xp[i] + (-1.0) * p.q[i];
-    for (int i = t.tid; i < p.m; i += t.T) {
-      const double v = p.rhoinv[i] * p.y[i];
-      p.zt[i] = (-1.0) * v + 1.0 * zp[i];
+    for (int i = t.tid; i < n; i += t.T) xt[i] = sigma * xp[i] + (-1.0) * q[i];
+    for (int i = t.tid; i < m; i += t.T) {
+      const double v = rhoinv[i] * y[i];
+      zt[i] = (-1.0) * v + 1.0 * zp[i];
     }
     t.sync();
-    linsys_solve(p.xt, p.zt, iter);
+    linsys_solve(xt, zt, iter);
     // update_x, update_z, update_y
-    for (int i = t.tid; i < p.n; i += t.T) {
-      const double v = a * p.xt[i] + (1.0 - a) * xp[i];
+    for (int i = t.tid; i < n; i += t.T) {
+      const double v = a * xt[i] + (1.0 - a) * xp[i];
       xx[i] = v;
-      p.dx[i] = v - xp[i];
+      dx[i] = v - xp[i];
     }
-    for (int i = t.tid; i < p.m; i += t.T) {
-      double v = p.rhoinv[i] * p.y[i];
-      v = 1.0 * v + a * p.zt[i] + (1.0 - a) * zp[i];
-      v = dmin(dmax(v, p.l[i]), p.u[i]);
+    for (int i = t.tid; i < m; i += t.T) {
+      double v = rhoinv[i] * y[i];
+      v = 1.0 * v + a * zt[i] + (1.0 - a) * zp[i];
+      v = dmin(dmax(v, l[i]), u[i]);
       zz[i] = v;
-      double dyv = a * p.zt[i] + (1.0 - a) * zp[i] + (-1.0) * v;
-      dyv = dyv * p.rho[i];
-      p.dy[i] = dyv;
-      p.y[i] = p.y[i] + dyv;
+      double dyv = a * zt[i] + (1.0 - a) * zp[i] + (-1.0) * v;
+      dyv = dyv * rho[i];
+      dy[i] = dyv;
+      y[i] = y[i] + dyv;
     }
     t.sync();
     const int ct = p.set.check_term;

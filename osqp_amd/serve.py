@@ -43,6 +43,8 @@ def _bind(lib):
     lib.osqp_batch_resolve.restype = C.c_int
     lib.osqp_batch_step.argtypes = [vp, C.POINTER(C.c_int)]
     lib.osqp_batch_step.restype = C.c_int
+    lib.osqp_batch_last_completed.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
+    lib.osqp_batch_last_completed.restype = C.c_int
     lib.osqp_batch_poll.argtypes = [vp, C.c_int, vp, vp, C.POINTER(BatchInfo)]
     lib.osqp_batch_poll.restype = C.c_int
     lib.osqp_batch_release.argtypes = [vp, C.c_int]
@@ -100,11 +102,16 @@ class BatchEngine:
             raise RuntimeError(f"osqp_batch_resolve({pid}) failed: not resident, unknown or busy")
 
     def step(self):
+        """Runs one epoch; returns the ids whose solves ended in it."""
         done = C.c_int(0)
         rc = self.lib.osqp_batch_step(self._b, C.byref(done))
         if rc:
             raise RuntimeError(f"osqp_batch_step failed ({rc})")
-        return done.value
+        if done.value == 0:
+            return []
+        ids = (C.c_int * done.value)()
+        k = self.lib.osqp_batch_last_completed(self._b, ids, done.value)
+        return list(ids[:k])
 
     def poll(self, pid):
         """Result namespace (x, y, info) once the solve has ended, else None."""

@@ -64,6 +64,8 @@ def load_library():
         lib.osqp_amd_solver_stats.restype = C.c_int
         lib.osqp_amd_time_pcg_kernels.argtypes = [C.POINTER(T.OSQPSolver), C.c_int] + [C.POINTER(C.c_double)] * 4
         lib.osqp_amd_time_pcg_kernels.restype = C.c_int
+        lib.osqp_amd_pcg_bytes.argtypes = [C.POINTER(T.OSQPSolver), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.osqp_amd_pcg_bytes.restype = C.c_int
         lib.osqp_algebra_device_name.argtypes = [C.c_char_p, C.c_int]
         lib.osqp_amd_last_error.restype = C.c_int
         _lib = lib
@@ -195,6 +197,14 @@ class OSQP:
         if rc:
             raise RuntimeError("osqp_amd_time_pcg_kernels failed")
         return dict(ms_av=a.value, ms_kv=k.value, bytes_av=ba.value, bytes_kv=bk.value)
+
+    def pcg_bytes(self):
+        """Algorithmic HBM bytes of one CG iteration and of one linear solve's
+        work outside the CG loop."""
+        a, b = C.c_double(), C.c_double()
+        if self.lib.osqp_amd_pcg_bytes(self._solver, C.byref(a), C.byref(b)):
+            raise RuntimeError("osqp_amd_pcg_bytes failed")
+        return dict(per_cg_iter=a.value, per_solve=b.value)
 
     def close(self):
         if self._solver:

@@ -101,6 +101,52 @@ def test_basic_qp2_update(precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("polishing,expected", [(0, 0), (1, 1)])
+def test_basic_qp2_polish(polishing, expected, precision):
+    """tests/basic_qp2/test_basic_qp2.cpp:10-70: eps 1e-6, polish_refine_iter 4;
+    polish status NOT_PERFORMED (0) without polishing, SUCCESS (1) with it; x
+    and y within TESTS_TOL of the stored solution."""
+    p = pr.basic_qp2()
+    s, r = gpu_solve(p, precision, eps_abs=1e-6, eps_rel=1e-6, polishing=polishing, polish_refine_iter=4)
+    assert r.info.status_val == 1
+    assert r.info.status_polish == expected
+    assert np.max(np.abs(r.x - p["x"])) / np.max(np.abs(p["x"])) < 1e-4
+    assert np.max(np.abs(r.y - p["y"])) / np.max(np.abs(p["y"])) < 1e-4
+    assert abs(r.info.obj_val - p["obj"]) / abs(p["obj"]) < 1e-4
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_basic_qp_polish(precision):
+    """tests/basic_qp/test_basic_qp.cpp:60-100: polishing on, scaling off, no
+    warm start; x, y, objective within TESTS_TOL; the polish reaches the exact
+    vertex (residuals below the ADMM ones)."""
+    p = pr.basic_qp()
+    s, r = gpu_solve(p, precision, polishing=1, scaling=0, warm_starting=0, max_iter=2000)
+    assert r.info.status_val == 1
+    assert r.info.status_polish == 1
+    assert np.max(np.abs(r.x - p["x"])) < 1e-4
+    assert np.max(np.abs(r.y - p["y"])) < 1e-4
+    assert abs(r.info.obj_val - p["obj"]) < 1e-4
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_polish_mpc(precision):
+    """Polishing on a problem with many active bounds: the polished point meets
+    the eps test at least as well as the ADMM point (polish keeps the better)."""
+    p = pr.mpc(15, seed=3)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    s0, r0 = gpu_solve(p, precision, polishing=0, **st)
+    s1, r1 = gpu_solve(p, precision, polishing=1, **st)
+    assert r0.info.status_val == r1.info.status_val == 1
+    assert r1.info.status_polish in (1, -1)
+    c = pr.kkt_certificate(p, r1.x, r1.y)
+    assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+    assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+    if r1.info.status_polish == 1:
+        assert r1.info.prim_res <= r0.info.prim_res + 1e-12 or r1.info.dual_res <= r0.info.dual_res + 1e-12
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("name", ["basic_lp", "no_active_set", "unconstrained"])
 def test_closed_forms(name, precision):
     p = getattr(pr, name)()
