@@ -160,6 +160,17 @@ OSQPInt    osqp_amd_linsys_time_kernels(LinSysSolver* s, OSQPInt reps, double* m
  * iteration and of the per-solve work outside the CG loop (cached A x path:
  * w = rho o (A x - b2), FP64 residual, correction, z = A x) of this linsys. */
 OSQPInt    osqp_amd_linsys_bytes(LinSysSolver* s, double* per_cg_iter, double* per_solve);
+/* Fused ADMM steps for the rho-vector case (no counterpart in the reference,
+ * which issues the same arithmetic as separate vector calls; results are
+ * bit-identical): compute_rhs (src/auxil.c:136-158) and update_x / update_z /
+ * update_y (src/auxil.c:172-229) in one pass each. */
+void       osqp_amd_admm_rhs(OSQPVectorf* xt, OSQPVectorf* zt, OSQPFloat sigma, const OSQPVectorf* x_prev,
+                             const OSQPVectorf* q, const OSQPVectorf* rho_inv, const OSQPVectorf* y,
+                             const OSQPVectorf* z_prev);
+void       osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x, const OSQPVectorf* xt,
+                                const OSQPVectorf* x_prev, OSQPVectorf* z, OSQPVectorf* y, OSQPVectorf* delta_y,
+                                const OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
+                                const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u);
 /* Last HIP error seen by the backend (0 if none); cleared on read. */
 OSQPInt    osqp_amd_last_error(void);
 

@@ -196,6 +196,46 @@ inline hipStream_t S() { return rt().stream; }
 
 }  // namespace
 
+// ---- fused ADMM steps (host core, rho vector case) ---------------------------
+// One pass over [0, n) and [n, n + m) each; the element arithmetic is that of
+// the unfused algebra calls, in the same order, so results are bit-identical.
+//   compute_rhs (auxil.c:136-158): xt = sigma x_prev + (-1) q;
+//                                  zt = (-1)(rhoinv o y) + 1 z_prev
+__global__ void k_admm_rhs(int n, int m, double sigma, double* xt, const double* xp, const double* q, double* zt,
+                           const double* rhoinv, const double* y, const double* zp) {
+  GRID_STRIDE(k, (long long)n + m) {
+    if (k < n) {
+      xt[k] = sigma * xp[k] + (-1.0) * q[k];
+    } else {
+      const long long i = k - n;
+      const double v = rhoinv[i] * y[i];
+      zt[i] = (-1.0) * v + 1.0 * zp[i];
+    }
+  }
+}
+//   update_x (auxil.c:172-184), update_z (:186-208), update_y (:210-229)
+__global__ void k_admm_update(int n, int m, double a, double* x, double* dx, const double* xt, const double* xp,
+                              double* z, double* y, double* dy, const double* zt, const double* zp,
+                              const double* rho, const double* rhoinv, const double* l, const double* u) {
+  GRID_STRIDE(k, (long long)n + m) {
+    if (k < n) {
+      const double v = a * xt[k] + (1.0 - a) * xp[k];
+      x[k] = v;
+      dx[k] = v - xp[k];
+    } else {
+      const long long i = k - n;
+      double v = rhoinv[i] * y[i];
+      v += a * zt[i] + (1.0 - a) * zp[i];  // add_scaled3's increment form (x == a, sca == 1)
+      v = cmin(cmax(v, l[i]), u[i]);
+      z[i] = v;
+      double d = a * zt[i] + (1.0 - a) * zp[i] + (-1.0) * v;
+      d = d * rho[i];
+      dy[i] = d;
+      y[i] = y[i] + d;
+    }
+  }
+}
+
 extern "C" {
 
 // ---- creation / views / copies ------------------------------------------------
@@ -432,6 +472,22 @@ OSQPInt OSQPVectorf_ew_bounds_type(OSQPVectori* iseq, const OSQPVectorf* l, cons
   HIPCHK(hipMemcpyAsync(R.h_scalars, R.d_scalars, sizeof(double), hipMemcpyDeviceToHost, R.stream));
   HIPCHK(hipStreamSynchronize(R.stream));
   return R.h_scalars[0] > 0.0 ? 1 : 0;
+}
+
+void osqp_amd_admm_rhs(OSQPVectorf* xt, OSQPVectorf* zt, OSQPFloat sigma, const OSQPVectorf* x_prev,
+                       const OSQPVectorf* q, const OSQPVectorf* rho_inv, const OSQPVectorf* y,
+                       const OSQPVectorf* z_prev) {
+  const int n = xt->length, m = zt->length;
+  EW_LAUNCH(k_admm_rhs, (long long)n + m, n, m, sigma, xt->d, x_prev->d, q->d, zt->d, rho_inv->d, y->d, z_prev->d);
+}
+
+void osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x, const OSQPVectorf* xt,
+                          const OSQPVectorf* x_prev, OSQPVectorf* z, OSQPVectorf* y, OSQPVectorf* delta_y,
+                          const OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
+                          const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u) {
+  const int n = x->length, m = z->length;
+  EW_LAUNCH(k_admm_update, (long long)n + m, n, m, alpha, x->d, delta_x->d, xt->d, x_prev->d, z->d, y->d,
+            delta_y->d, zt->d, z_prev->d, rho->d, rho_inv->d, l->d, u->d);
 }
 
 }  // extern "C"

@@ -120,11 +120,13 @@ void swap_vectors(OSQPVectorf** a, OSQPVectorf** b) {
 static void compute_rhs(OSQPSolver* solver) {
   OSQPWorkspace* work = solver->work;
   OSQPSettings* settings = solver->settings;
-  OSQPVectorf_add_scaled(work->xtilde_view, settings->sigma, work->x_prev, -1., work->data->q);
   if (settings->rho_is_vec) {
-    OSQPVectorf_ew_prod(work->ztilde_view, work->rho_inv_vec, work->y);
-    OSQPVectorf_add_scaled(work->ztilde_view, -1.0, work->ztilde_view, 1.0, work->z_prev);
+    /* add_scaled(xt, sigma, x_prev, -1, q); ew_prod(zt, rho_inv, y);
+     * add_scaled(zt, -1, zt, 1, z_prev) — one fused pass */
+    osqp_amd_admm_rhs(work->xtilde_view, work->ztilde_view, settings->sigma, work->x_prev, work->data->q,
+                      work->rho_inv_vec, work->y, work->z_prev);
   } else {
+    OSQPVectorf_add_scaled(work->xtilde_view, settings->sigma, work->x_prev, -1., work->data->q);
     OSQPVectorf_add_scaled(work->ztilde_view, 1.0, work->z_prev, -work->rho_inv, work->y);
   }
 }
@@ -133,6 +135,15 @@ static void compute_rhs(OSQPSolver* solver) {
 void update_xz_tilde(OSQPSolver* solver, OSQPInt admm_iter) {
   compute_rhs(solver);
   solver->work->linsys_solver->solve(solver->work->linsys_solver, solver->work->xz_tilde, admm_iter);
+}
+
+/* auxil.c:172-229 for the rho-vector case: update_x, update_z, update_y in
+ * one fused pass (same element arithmetic) */
+void update_xzy(OSQPSolver* solver) {
+  OSQPWorkspace* work = solver->work;
+  osqp_amd_admm_update(solver->settings->alpha, work->x, work->delta_x, work->xtilde_view, work->x_prev, work->z,
+                       work->y, work->delta_y, work->ztilde_view, work->z_prev, work->rho_vec, work->rho_inv_vec,
+                       work->data->l, work->data->u);
 }
 
 /* auxil.c:172-184 */

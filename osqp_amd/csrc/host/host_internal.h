@@ -80,6 +80,7 @@ void      update_xz_tilde(OSQPSolver* solver, OSQPInt admm_iter);
 void      update_x(OSQPSolver* solver);
 void      update_z(OSQPSolver* solver);
 void      update_y(OSQPSolver* solver);
+void      update_xzy(OSQPSolver* solver);
 void      compute_obj_val_dual_gap(const OSQPSolver* solver, const OSQPVectorf* x, const OSQPVectorf* y,
                                    OSQPFloat* prim_obj_val, OSQPFloat* dual_obj_val, OSQPFloat* duality_gap);
 OSQPInt   has_solution(const OSQPInfo* info);

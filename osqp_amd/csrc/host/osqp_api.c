@@ -319,9 +319,13 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
     swap_vectors(&(work->x), &(work->x_prev));
     swap_vectors(&(work->z), &(work->z_prev));
     update_xz_tilde(solver, iter);
-    update_x(solver);
-    update_z(solver);
-    update_y(solver);
+    if (settings->rho_is_vec) {
+      update_xzy(solver);
+    } else {
+      update_x(solver);
+      update_z(solver);
+      update_y(solver);
+    }
 
     if (work->first_run) temp_run_time = solver->info->setup_time + osqp_amd_toc(work);
     else temp_run_time = solver->info->update_time + osqp_amd_toc(work);

@@ -1016,7 +1016,7 @@ struct Solver {
     }
     for (int i = t.tid; i < m; i += t.T) {
       double v = rhoinv[i] * y[i];
-      v = 1.0 * v + a * zt[i] + (1.0 - a) * zp[i];
+      v += a * zt[i] + (1.0 - a) * zp[i];  // add_scaled3's increment form (x == a, sca == 1)
       v = dmin(dmax(v, l[i]), u[i]);
       zz[i] = v;
       double dyv = a * zt[i] + (1.0 - a) * zp[i] + (-1.0) * v;

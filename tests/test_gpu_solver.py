@@ -254,6 +254,27 @@ def test_multi_panel_qp(precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("n,m_d,dens", [(40, 400, 0.1), (200, 2000, 0.02)])
+def test_lasso_vs_oracle(n, m_d, dens, precision):
+    """The lasso configuration's QP (variables x, y, t; P = blkdiag(0, 2I, 0),
+    so P has rows without a diagonal) at reduced size: same status as the
+    oracle's PCG path, objective to 1e-3 relative, KKT certificate."""
+    p = pr.lasso(n, m_d, dens, seed=1)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000, scaling=10)
+    o = OracleSolver()
+    fx = dict(FIXTURE)
+    fx.update(st)
+    assert o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=2, **fx) == 0
+    o.solve()
+    s, r = gpu_solve(p, precision, **st)
+    assert r.info.status_val == o.info.status_val == 1
+    assert abs(r.info.obj_val - o.info.obj_val) <= 1e-3 * max(1.0, abs(o.info.obj_val))
+    c = pr.kkt_certificate(p, r.x, r.y)
+    assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+    assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("N", [10, 30])
 def test_mpc_vs_oracle(N, precision):
     p = pr.mpc(N, seed=N)
