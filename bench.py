@@ -73,7 +73,11 @@ def serve_mpc(n_req, seed):
     ids, arr_of, lat, status, iters = [], {}, [], {}, []
     nxt = 0
     pending = set()
+    t_log = time.perf_counter()
     while nxt < n_req or pending:
+        if time.perf_counter() - t_log > 5.0:
+            log(f"[serve] mpc: {nxt} admitted, {len(lat)} done")
+            t_log = time.perf_counter()
         now = time.perf_counter() - t_start
         while nxt < n_req and arrivals[nxt] <= now and len(pending) < 4096:
             p = reqs[nxt]
@@ -119,6 +123,104 @@ def serve_mpc(n_req, seed):
             "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
             "plain_sample": k_plain, "statuses_identical_on_sample": same,
             "engine": {"epochs": cnt["epochs"], "gpu_s": cnt["gpu_s"], "tuned_team_per_bucket": tuning}}
+
+
+def serve_portfolio(n_problems, n_req, seed):
+    """Portfolio stream: n ~ U{500..5000} assets, problems resident on the GPU;
+    each request draws a new mu, replaces q (osqp_update_data_vec semantics)
+    and re-solves warm-started.  Closed loop: an idle problem takes its next
+    request as soon as its previous result is seen.  The plain FP64 engine,
+    one resident solver per problem doing update(q) + solve, on a sample of
+    the same requests, is the GPU comparison."""
+    from osqp_amd import OSQP
+    from osqp_amd import problems as pr
+    from osqp_amd.serve import BatchEngine
+    st = dict(eps_abs=1e-4, eps_rel=1e-4)
+    rng = np.random.default_rng(seed + 29)
+    sizes = rng.integers(500, 5001, n_problems)
+    t0 = time.time()
+    probs = [pr.portfolio(int(nn), seed=seed * 1000 + k) for k, nn in enumerate(sizes)]
+    reqs = [(int(rng.integers(n_problems)), None) for _ in range(n_req)]
+    reqs = [(k, pr.portfolio_q(rng.standard_normal(probs[k]["P"].shape[0] - probs[k]["k"]), probs[k]["k"]))
+            for k, _ in reqs]
+    log(f"[serve] generated {n_problems} portfolios, {n_req} requests in {time.time() - t0:.1f}s")
+    eng = BatchEngine(precision=32, max_active=4096, **st)
+    pid = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=True) for p in probs]
+    t0 = time.perf_counter()
+    ep = 0
+    while eng.pending():
+        eng.step()
+        ep += 1
+        if ep % 20 == 0:
+            log(f"[serve] portfolio initial solves: {n_problems - eng.pending()}/{n_problems} done, "
+                f"{time.perf_counter() - t0:.1f}s")
+    for i in pid:
+        eng.poll(i)
+    log(f"[serve] portfolio initial solves done in {time.perf_counter() - t0:.1f}s")
+    queue = {k: [] for k in range(n_problems)}
+    for j, (k, q) in enumerate(reqs):
+        queue[k].append(j)
+    t_start = time.perf_counter()
+    issued, lat, status, busy = {}, [], {}, {}
+    for k in range(n_problems):
+        if queue[k]:
+            j = queue[k].pop(0)
+            eng.resolve(pid[k], reqs[j][1])
+            issued[pid[k]] = (j, time.perf_counter())
+    t_log = time.perf_counter()
+    while issued:
+        if time.perf_counter() - t_log > 5.0:
+            log(f"[serve] portfolio: {len(lat)}/{n_req} requests done")
+            t_log = time.perf_counter()
+        if time.perf_counter() - t_start > 120.0:
+            # bound the section's run time: stop issuing, drain what is in flight
+            for k in queue:
+                queue[k] = []
+        for i in eng.step():
+            r = eng.poll(i)
+            j, t_iss = issued.pop(i)
+            lat.append(time.perf_counter() - t_iss)
+            status[j] = r.info.status_val
+            k = pid.index(i)
+            if queue[k]:
+                jn = queue[k].pop(0)
+                eng.resolve(i, reqs[jn][1])
+                issued[i] = (jn, time.perf_counter())
+    elapsed = time.perf_counter() - t_start
+    eng.close()
+    n_done = len(lat)
+    qps = n_done / elapsed
+    # plain FP64: one resident solver per problem (setup outside the timing)
+    sample = [j for j in range(len(reqs)) if j in status][:24]
+    k_plain = len(sample)
+    solvers = {}
+    for j in sample:
+        k = reqs[j][0]
+        if k not in solvers:
+            s = OSQP(precision=64)
+            p = probs[k]
+            assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
+            s.solve()
+            solvers[k] = s
+    same = 0
+    ts = time.perf_counter()
+    for j in sample:
+        k, q = reqs[j]
+        solvers[k].update(q=q)
+        r = solvers[k].solve()
+        same += int(r.info.status_val == status[j])
+    plain_qps = k_plain / (time.perf_counter() - ts)
+    for s in solvers.values():
+        s.close()
+    log(f"[serve] portfolio batched FP32: {qps:.0f} QPs/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
+        f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms; plain FP64: {plain_qps:.1f} QPs/s")
+    return {"workload": f"portfolio stream, {n_problems} resident problems n~U{{500..5000}}, {n_done} mu updates "
+                        f"with warm-started re-solves, eps 1e-4, closed loop",
+            "qps": qps, "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                       "p99": float(np.percentile(lat, 99) * 1e3)},
+            "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
+            "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
+            "plain_sample": k_plain, "statuses_identical_on_sample": same}
 
 
 def main():
@@ -254,6 +356,7 @@ def main():
 
     # ---- served MPC stream: batched FP32-PCG engine vs plain FP64 engine -------
     serving = None if args.no_serving else serve_mpc(args.mpc_requests, args.seed)
+    serving_pf = None if args.no_serving else serve_portfolio(64, 512, args.seed)
 
     out = {
         "metric": METRIC,
@@ -289,6 +392,7 @@ def main():
         "gpu_fp64_plain": fp64,
         "cpu_baseline": cpu,
         "serving_mpc": serving,
+        "serving_portfolio": serving_pf,
     }
     print(json.dumps(out), flush=True)
 

@@ -767,6 +767,14 @@ OSQPInt osqp_batch_add(OSQPBatch* b, const OSQPCscMatrix* P, const OSQPFloat* q,
   p->resident = resident ? 1 : 0;
   build_host(&p->h, P, q, A, l, u, x0, y0);
   p->L = make_layout(p->h, E->bits / 8);
+  if (p->L.total > E->arena_size || p->L.st_total + p->L.rs_total > E->pool_size) {
+    // could never be admitted: larger than the whole arena or staging pool
+    fprintf(stderr, "osqp_batch: problem needs %zu bytes of device arena (of %zu) and %zu of staging (of %zu)\n",
+            p->L.total, E->arena_size, p->L.st_total + p->L.rs_total, E->pool_size);
+    E->next_id--;
+    delete p;
+    return OSQP_DATA_VALIDATION_ERROR;
+  }
   p->bucket = bucket_of(p->h);
   p->state = kQueued;
   p->t_req = now_s();
