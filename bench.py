@@ -263,12 +263,14 @@ def main():
     log(f"[bench] FP32-PCG setup {time.time() - t0:.2f}s on {device_name()}")
     if args.profile_kernels:
         kt = eng.time_kernels(reps=20)
+        kt64 = eng.time_fp64_kernels(reps=20)
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(5):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"profile_kernels": kt, "copy_bytes_each_way": a.numel() * 4}), flush=True)
+        print(json.dumps({"profile_kernels": kt, "fp64_kernels": kt64, "copy_bytes_each_way": a.numel() * 4}),
+              flush=True)
         eng.close()
         return
     for w in range(args.warmup):
@@ -315,6 +317,16 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     log(f"[bench] kernels: Av {kt['ms_av']:.4f}ms {kt['bytes_av'] / kt['ms_av'] / 1e6:.0f} GB/s; "
         f"Kv {kt['ms_kv']:.4f}ms {kt['bytes_kv'] / kt['ms_kv'] / 1e6:.0f} GB/s")
+    # the defect correction's FP64 panel SpMVs (once per ADMM iteration each)
+    kt64 = eng.time_fp64_kernels(reps=20)
+    fp64_kernels = None
+    if kt64:
+        fp64_kernels = {"A_x64": {"ms_per_launch": kt64["ms_a"], "bytes_per_launch": kt64["bytes_a"],
+                                  "GBs": kt64["bytes_a"] / kt64["ms_a"] / 1e6},
+                        "K_x64": {"ms_per_launch": kt64["ms_k"], "bytes_per_launch": kt64["bytes_k"],
+                                  "GBs": kt64["bytes_k"] / kt64["ms_k"] / 1e6}}
+        log(f"[bench] FP64 residual kernels: A {kt64['ms_a']:.4f}ms {fp64_kernels['A_x64']['GBs']:.0f} GB/s; "
+            f"K {kt64['ms_k']:.4f}ms {fp64_kernels['K_x64']['GBs']:.0f} GB/s")
     eng.close()
 
     # ---- plain FP64 backend on the same QP (GPU comparison) -------------------
@@ -388,7 +400,8 @@ def main():
                      "ms_per_launch": dom_ms, "bytes_per_launch": dom_bytes,
                      "other_kernel": {"name": "k_Av_rho" if dom == "k_Kv" else "k_Kv",
                                       "ms_per_launch": min(kt["ms_kv"], kt["ms_av"]),
-                                      "bytes_per_launch": kt["bytes_av"] if dom == "k_Kv" else kt["bytes_kv"]}},
+                                      "bytes_per_launch": kt["bytes_av"] if dom == "k_Kv" else kt["bytes_kv"]},
+                     "fp64_residual_kernels": fp64_kernels},
         "gpu_fp64_plain": fp64,
         "cpu_baseline": cpu,
         "serving_mpc": serving,

@@ -288,6 +288,11 @@ OSQPInt osqp_amd_solver_stats(OSQPSolver* solver, OSQPInt* pcg_iters_total,
 OSQPInt osqp_amd_time_pcg_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_av, double* ms_kv,
                                   double* bytes_av, double* bytes_kv);
 
+/* Times the FP64 panel SpMVs of the PCG's defect correction (see
+ * osqp_amd_linsys_time_fp64_kernels in osqp_amd_algebra.h). */
+OSQPInt osqp_amd_time_fp64_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_a, double* ms_k,
+                                   double* bytes_a, double* bytes_k);
+
 /* Algorithmic HBM bytes of this solver's PCG work: one CG iteration, and the
  * per-solve work around the CG (residual, correction, z = A x) on the cached
  * path (see osqp_amd_linsys_bytes in osqp_amd_algebra.h). */

@@ -156,6 +156,11 @@ void       osqp_amd_linsys_counters(LinSysSolver* s, OSQPInt* cg_iters, OSQPInt*
  * launches timed with HIP events, and the algorithmic bytes of one launch. */
 OSQPInt    osqp_amd_linsys_time_kernels(LinSysSolver* s, OSQPInt reps, double* ms_av, double* ms_kv,
                                         double* bytes_av, double* bytes_kv);
+/* The same for the FP64 panel SpMVs of the defect correction (z = A x and
+ * the residual's [P A'] [x; w]); returns 1 when those operators are not
+ * panelled.  Extension: the reference has one precision per build. */
+OSQPInt    osqp_amd_linsys_time_fp64_kernels(LinSysSolver* s, OSQPInt reps, double* ms_a, double* ms_k,
+                                             double* bytes_a, double* bytes_k);
 /* Algorithmic bytes (each array element touched once per pass) of one CG
  * iteration and of the per-solve work outside the CG loop (cached A x path:
  * w = rho o (A x - b2), FP64 residual, correction, z = A x) of this linsys. */

@@ -7,9 +7,12 @@
 //
 // MI355X design:
 //   * Device-resident control.  The loop state (r'y, ||r||, alpha, beta, eps,
-//     iteration count, active flag) lives in a device struct; one-block
-//     "scalar" kernels finish each reduction; every CG kernel returns at once
-//     when `active` is cleared.  The host never waits on the GPU inside a
+//     iteration count, active flag) lives in a device struct.  The kernel that
+//     consumes a reduction finishes it itself, identically in every block
+//     (alpha inside the update, beta and the stop test inside the
+//     search-direction update), so a CG iteration is six launches: two panel
+//     SpMVs, their two combines, the update and the loop head.  Every CG
+//     kernel returns at once when `active` is cleared.  The host never waits on the GPU inside a
 //     solve: it queues CG iterations ahead and reads a per-iteration flag the
 //     loop-head kernel writes to host-coherent memory, so it stops queueing a
 //     couple of iterations after convergence.  The reference host-syncs every
@@ -62,7 +65,7 @@ struct PcgState {
   double eps_prev;     // tolerance schedule state
   double red_factor;   // tolerance schedule state
   double rhs_norm;
-  double rTy;          // r'y of the current iterate
+  double rty[2];       // r'y of the iterate after j CG iterations, in slot j & 1
   double rnorm;        // ||r||_inf of the CG residual
   double pKp, alpha, beta;
   int iter;            // CG iterations of the current solve
@@ -297,23 +300,11 @@ __global__ void k_comb_z(int m, int n, int np, const double* partial, double* b,
   }
 }
 
-// w = rho o (Ax - b2) from the cached A x of the previous solve (m rows)
-__global__ void k_w_from_ax(const PcgState* st, int m, const double* ax, const double* b2, const double* rho,
-                            double rho_sc, double* w) {
-  if (!st->active) return;
-  GRID_STRIDE(i, m) w[i] = rho_at(rho, rho_sc, (int)i) * (ax[i] - b2[i]);
-}
-
-// ---- tolerance (cuda_pcg_interface.cu:32-64) + loop reset, one block ---------
-// rhs_norm is needed only for the first ADMM iteration and in polishing; the
-// cached path passes nparts = 0.
-__global__ void k_tolerance(PcgState* st, const double* partials, int nparts, int admm_iter, double spr, double sdr,
-                            int polishing, double tol_fraction, int reduction_threshold) {
-  __shared__ double lds[kBlock / kWave];
-  double mx = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) mx = partials[i] > mx ? partials[i] : mx;
-  double rhs_norm = block_max(mx, lds);
-  if (threadIdx.x != 0) return;
+// ---- tolerance (cuda_pcg_interface.cu:32-64) + loop reset ---------------------
+// One thread.  rhs_norm matters only at the first ADMM iteration and in
+// polishing.
+__device__ void set_tolerance(PcgState* st, double rhs_norm, int admm_iter, double spr, double sdr, int polishing,
+                              double tol_fraction, int reduction_threshold) {
   // zero-iteration bookkeeping of the previous solve (cuda_pcg_interface.cu:266-268)
   if (st->solves > 0) {
     if (st->iter == 0) st->zero_iters++;
@@ -345,24 +336,55 @@ __global__ void k_tolerance(PcgState* st, const double* partials, int nparts, in
   st->solves++;
 }
 
-// one block: scale = ||r64||; tolerance on the scaled correction system
-__global__ void k_dc_scale(PcgState* st, const double* part_rn, int nparts) {
-  __shared__ double lds[kBlock / kWave];
+// Every block of the kernels below that consume a reduction forms its total
+// itself, from the producer's per-block partials, in one fixed order (thread i
+// takes partials i, i + 256, ...; then the block tree), so all blocks agree to
+// the bit and no one-block kernel sits between producer and consumer.  Block 0
+// alone records the scalars in the state.
+__device__ inline double parts_max(const double* part, int nparts, double* lds) {
   double mx = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) mx = part_rn[i] > mx ? part_rn[i] : mx;
-  double rn = block_max(mx, lds);
-  if (threadIdx.x != 0) return;
-  double s = (rn > 0.0 && isfinite(rn)) ? rn : 1.0;
-  st->scale = s;
-  st->eps_cmp = st->eps / s;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) mx = part[i] > mx ? part[i] : mx;
+  return block_max(mx, lds);
+}
+__device__ inline double parts_sum(const double* part, int nparts, double* lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+  return block_sum(s, lds);
 }
 
-// r = r64 / scale; y = M^-1 r; p = -y; d = 0; partials r'y, max|r|
-template <typename V>
-__global__ void k_dc_init_cg(const PcgState* st, int n, const double* r64, const V* dinv, V* r, V* y, V* p, V* d,
-                             double* part_rty, double* part_rn) {
+// explicit-rhs path: rhs_norm from the rhs kernel's partials; one block
+__global__ void k_tolerance(PcgState* st, const double* partials, int nparts, int admm_iter, double spr, double sdr,
+                            int polishing, double tol_fraction, int reduction_threshold) {
   __shared__ double lds[kBlock / kWave];
-  const double inv = 1.0 / st->scale;
+  const double rhs_norm = parts_max(partials, nparts, lds);
+  if (threadIdx.x == 0)
+    set_tolerance(st, rhs_norm, admm_iter, spr, sdr, polishing, tol_fraction, reduction_threshold);
+}
+
+// cached path: the tolerance (block 0) and w = rho o (A x - b2) from the
+// cached A x of the previous solve (m rows)
+__global__ void k_tol_w(PcgState* st, int admm_iter, double spr, double sdr, double tol_fraction,
+                        int reduction_threshold, int m, const double* ax, const double* b2, const double* rho,
+                        double rho_sc, double* w) {
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    set_tolerance(st, 0.0, admm_iter, spr, sdr, 0, tol_fraction, reduction_threshold);
+  GRID_STRIDE(i, m) w[i] = rho_at(rho, rho_sc, (int)i) * (ax[i] - b2[i]);
+}
+
+// scale = ||r64||_inf from the residual kernel's partials (tolerance on the
+// scaled correction system: eps / scale); r = r64 / scale; y = M^-1 r; p = -y;
+// d = 0; partials r'y, max|r|.  `part_res` must not alias the outputs.
+template <typename V>
+__global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int n, const double* r64, const V* dinv,
+                             V* r, V* y, V* p, V* d, double* part_rty, double* part_rn) {
+  __shared__ double lds[kBlock / kWave];
+  const double rn0 = parts_max(part_res, nres, lds);
+  const double scale = (rn0 > 0.0 && isfinite(rn0)) ? rn0 : 1.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->scale = scale;
+    st->eps_cmp = st->eps / scale;
+  }
+  const double inv = 1.0 / scale;
   Acc<V> dot;
   double mx = 0.0;
   GRID_STRIDE(i, n) {
@@ -397,58 +419,58 @@ __global__ void k_dc_apply(const PcgState* st, int n, const V* d, double* x64, d
   }
 }
 
-// ---- loop head / alpha: one block -----------------------------------------------
-// `flags` (host-coherent, may be null): flags[it] = 2 * solve + active, read by
-// the host to stop queueing iterations.
-__global__ void k_loop_head(PcgState* st, const double* part_rty, const double* part_rn, int nparts, int max_iter,
-                            int first, int guard, int* flags, int it) {
-  __shared__ double lds[kBlock / kWave];
-  if (!st->active) {
-    if (flags && threadIdx.x == 0) flags[it] = 2 * st->solves;
-    return;
-  }
-  double s = 0.0, mx = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-    s += part_rty[i];
-    mx = part_rn[i] > mx ? part_rn[i] : mx;
-  }
-  double rty = block_sum(s, lds);
-  double rn = block_max(mx, lds);
-  if (threadIdx.x != 0) return;
-  if (!first) {
-    st->beta = rty / st->rTy;
-    st->iter++;
-    st->total_iters++;
-  }
-  st->rTy = rty;
-  st->rnorm = rn;
-  int go = (rn > st->eps_cmp && st->iter < max_iter) ? 1 : 0;
+// ---- CG loop control ------------------------------------------------------------
+// `flags` (host-coherent, may be null): flags[j] = 2 * solve + active after the
+// loop head of CG iteration j, read by the host to stop queueing iterations.
+// Loop state written inside a kernel is read by that kernel's other blocks
+// only where the value cannot have changed (r'y lives in two slots by
+// iteration parity; the iteration count is the launch argument j).
+
+// stop test at the head of CG iteration j (one thread)
+__device__ inline int loop_go(const PcgState* st, double rty, double rn, int j, int max_iter, int guard) {
+  int go = (rn > st->eps_cmp && j < max_iter) ? 1 : 0;
   // the mixed path stops on a non-finite or non-positive r'y (FP32 breakdown)
   if (guard && !(rty > 0.0 && isfinite(rty))) go = 0;
-  st->active = go;
-  if (flags) flags[it] = 2 * st->solves + go;
+  return go;
 }
 
-__global__ void k_alpha(PcgState* st, const double* part, int nparts, int guard) {
+// head of CG iteration 0, after the defect-correction init; one block
+__global__ void k_loop_first(PcgState* st, const double* part_rty, const double* part_rn, int nparts, int max_iter,
+                             int guard, int* flags) {
   __shared__ double lds[kBlock / kWave];
-  if (!st->active) return;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
-  double pkp = block_sum(s, lds);
-  if (threadIdx.x == 0) {
-    st->pKp = pkp;
-    st->alpha = st->rTy / pkp;
-    if (guard && !(pkp > 0.0 && isfinite(pkp))) st->active = 0;
+  if (!st->active) {
+    if (flags && threadIdx.x == 0) flags[0] = 2 * st->solves;
+    return;
   }
+  const double rty = parts_sum(part_rty, nparts, lds);
+  const double rn = parts_max(part_rn, nparts, lds);
+  if (threadIdx.x != 0) return;
+  st->rty[0] = rty;
+  st->rnorm = rn;
+  const int go = loop_go(st, rty, rn, 0, max_iter, guard);
+  st->active = go;
+  if (flags) flags[0] = 2 * st->solves + go;
 }
 
-// d += alpha p; r += alpha Kp; y = M^-1 r; partials r'y, max|r|
+// CG iteration j: alpha = r'y / p'Kp from the operator's p'Kp partials; then
+// d += alpha p; r += alpha Kp; y = M^-1 r; partials r'y, max|r|.  On an FP32
+// breakdown (p'Kp not positive and finite) every block skips the update and
+// block 0 stops the loop.
 template <typename V>
-__global__ void k_update(const PcgState* st, int n, V* x, V* r, V* y, const V* p, const V* Kp, const V* dinv,
-                         double* part_rty, double* part_rn) {
+__global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, int guard, int n, V* x, V* r, V* y,
+                         const V* p, const V* Kp, const V* dinv, double* part_rty, double* part_rn) {
   if (!st->active) return;
   __shared__ double lds[kBlock / kWave];
-  const V alpha = (V)st->alpha;
+  const double pkp = parts_sum(part_pkp, npkp, lds);
+  const double alpha64 = st->rty[j & 1] / pkp;
+  const bool broke = guard && !(pkp > 0.0 && isfinite(pkp));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pKp = pkp;
+    st->alpha = alpha64;
+    if (broke) st->active = 0;
+  }
+  if (broke) return;
+  const V alpha = (V)alpha64;
   Acc<V> dot;
   double mx = 0.0;
   GRID_STRIDE(i, n) {
@@ -469,11 +491,31 @@ __global__ void k_update(const PcgState* st, int n, V* x, V* r, V* y, const V* p
   }
 }
 
-// p = beta p - y
+// head of CG iteration j >= 1: r'y and ||r|| from the update's partials,
+// beta = r'y / r'y_prev, the stop test; while the loop goes on, p = beta p - y
 template <typename V>
-__global__ void k_pupdate(const PcgState* st, int n, V* p, const V* y) {
-  if (!st->active) return;
-  const V beta = (V)st->beta;
+__global__ void k_head_pupdate(PcgState* st, const double* part_rty, const double* part_rn, int nparts, int j,
+                               int max_iter, int guard, int* flags, int n, V* p, const V* y) {
+  __shared__ double lds[kBlock / kWave];
+  if (!st->active) {
+    if (flags && blockIdx.x == 0 && threadIdx.x == 0) flags[j] = 2 * st->solves;
+    return;
+  }
+  const double rty = parts_sum(part_rty, nparts, lds);
+  const double rn = parts_max(part_rn, nparts, lds);
+  const double beta64 = rty / st->rty[(j - 1) & 1];
+  const int go = loop_go(st, rty, rn, j, max_iter, guard);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->beta = beta64;
+    st->iter = j;
+    st->total_iters++;
+    st->rty[j & 1] = rty;
+    st->rnorm = rn;
+    st->active = go;
+    if (flags) flags[j] = 2 * st->solves + go;
+  }
+  if (!go) return;
+  const V beta = (V)beta64;
   GRID_STRIDE(i, n) p[i] = beta * p[i] - y[i];
 }
 
@@ -874,31 +916,29 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
                        s->reduction_threshold);
     // r64 = K x64 - rhs64 with the FP64 values (the exact K)
     if (m) op_av_rho<double>(s, (const double*)s->x64, s->w64);
-    nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)s->rhs64, s->r64, part1);
+    nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)s->rhs64, s->r64, part2);
   } else {
-    hipLaunchKernelGGL(k_tolerance, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)nullptr, 0, admm_iter,
-                       spr, sdr, s->polishing, s->tol_fraction,
-                       s->reduction_threshold);
-    // w64 = rho o (A x64 - b2) from the cached A x64;  r64 = sigma x64 + P x64 + A' w64 - b1
-    hipLaunchKernelGGL(k_w_from_ax, dim3(gM), dim3(kBlock), 0, S(), (const PcgState*)s->st, m,
-                       (const double*)s->ax64, (const double*)(b->d + n), rho, s->rho_sc, s->w64);
-    nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)b->d, s->r64, part1);
+    // tolerance; w64 = rho o (A x64 - b2) from the cached A x64;
+    // r64 = sigma x64 + P x64 + A' w64 - b1
+    hipLaunchKernelGGL(k_tol_w, dim3(gM), dim3(kBlock), 0, S(), s->st, admm_iter, spr, sdr, s->tol_fraction,
+                       s->reduction_threshold, m, (const double*)s->ax64, (const double*)(b->d + n), rho, s->rho_sc,
+                       s->w64);
+    nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)b->d, s->r64, part2);
   }
-  hipLaunchKernelGGL(k_dc_scale, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part1, nres);
-  hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
+  hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nres, n,
                      (const double*)s->r64, dinv, r, y, p, d, part0, part1);
-  hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
-                     gE, s->max_iter, 1, guard, s->d_flags, 0);
+  hipLaunchKernelGGL(k_loop_first, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
+                     gE, s->max_iter, guard, s->d_flags);
+  // one CG iteration: the operator (two SpMVs with their combines), the
+  // update, and the next loop head fused with the search-direction update
   for (int it = 0; it < s->max_iter; ++it) {
     if (host_should_stop(s, it)) break;
     if (m) op_av_rho<V>(s, (const V*)p, w);
     const int nk = op_kv<V>(s, (const V*)p, (const V*)w, (const double*)nullptr, Kp, part2);
-    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part2, nk, guard);
-    hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, d, r, y,
-                       (const V*)p, (const V*)Kp, dinv, part0, part1);
-    hipLaunchKernelGGL(k_loop_head, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0,
-                       (const double*)part1, gE, s->max_iter, 0, guard, s->d_flags, it + 1);
-    hipLaunchKernelGGL(k_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, p, (const V*)y);
+    hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nk, it, guard, n,
+                       d, r, y, (const V*)p, (const V*)Kp, dinv, part0, part1);
+    hipLaunchKernelGGL(k_head_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part0,
+                       (const double*)part1, gE, it + 1, s->max_iter, guard, s->d_flags, n, p, (const V*)y);
   }
   hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
                      (const V*)d, s->x64, b->d);
@@ -1171,6 +1211,49 @@ OSQPInt osqp_amd_linsys_time_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_
   s->host_solves = 0;
   HIPCHK(hipStreamSynchronize(S()));
   if (s->h_flags) memset(s->h_flags, 0, sizeof(int) * (size_t)s->flags_len);
+  return 0;
+}
+
+// Times the FP64 panel SpMVs of the defect correction (z = A x64 and the
+// residual's [P A'] [x64; w64]): `reps` launches of each on the backend
+// stream.  Returns 1 when the operators are not panelled.
+OSQPInt osqp_amd_linsys_time_fp64_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_a, double* ms_k,
+                                          double* bytes_a, double* bytes_k) {
+  HipPcg* s = (HipPcg*)ls;
+  if (!s || reps <= 0) return 1;
+  const PanelMatrix* PA = panels_A<double>(s);
+  const PanelMatrix* PK = panels_K<double>(s);
+  if (!PA || !PK) return 1;
+  const int n = s->n;
+  hipEvent_t e0, e1, e2;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventCreate(&e2));
+  update_precond(s);
+  // inputs: x64 (the warm start) and w64 = 1
+  hipLaunchKernelGGL(k_recip_to<double>, dim3(grid_for(s->m, kBlock, 4096)), dim3(kBlock), 0, S(), s->m,
+                     (const double*)s->dtmp, s->w64, 1);
+  HIPCHK(hipStreamSynchronize(S()));
+  HIPCHK(hipEventRecord(e0, S()));
+  for (int r = 0; r < reps; ++r)
+    panel_spmv(*PA, nullptr, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
+  HIPCHK(hipEventRecord(e1, S()));
+  for (int r = 0; r < reps; ++r)
+    panel_spmv(*PK, nullptr, (const double*)s->x64, n, (const double*)s->w64, (double*)s->partK);
+  HIPCHK(hipEventRecord(e2, S()));
+  HIPCHK(hipEventSynchronize(e2));
+  float t01 = 0.f, t12 = 0.f;
+  HIPCHK(hipEventElapsedTime(&t01, e0, e1));
+  HIPCHK(hipEventElapsedTime(&t12, e1, e2));
+  if (ms_a) *ms_a = (double)t01 / reps;
+  if (ms_k) *ms_k = (double)t12 / reps;
+  if (bytes_a) *bytes_a = panel_spmv_bytes(*PA, 8);
+  if (bytes_k) *bytes_k = panel_spmv_bytes(*PK, 8);
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  HIPCHK(hipEventDestroy(e2));
+  // w64 and the A x cache are rebuilt by the next solve
+  s->ax_valid = 0;
   return 0;
 }
 

@@ -696,6 +696,12 @@ OSQPInt osqp_amd_time_pcg_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_a
   return osqp_amd_linsys_time_kernels(solver->work->linsys_solver, reps, ms_av, ms_kv, bytes_av, bytes_kv);
 }
 
+OSQPInt osqp_amd_time_fp64_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_a, double* ms_k, double* bytes_a,
+                                   double* bytes_k) {
+  if (!solver || !solver->work || !solver->work->linsys_solver) return 1;
+  return osqp_amd_linsys_time_fp64_kernels(solver->work->linsys_solver, reps, ms_a, ms_k, bytes_a, bytes_k);
+}
+
 OSQPInt osqp_amd_pcg_bytes(OSQPSolver* solver, double* per_cg_iter, double* per_solve) {
   if (!solver || !solver->work || !solver->work->linsys_solver) return 1;
   return osqp_amd_linsys_bytes(solver->work->linsys_solver, per_cg_iter, per_solve);

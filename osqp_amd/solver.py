@@ -64,6 +64,8 @@ def load_library():
         lib.osqp_amd_solver_stats.restype = C.c_int
         lib.osqp_amd_time_pcg_kernels.argtypes = [C.POINTER(T.OSQPSolver), C.c_int] + [C.POINTER(C.c_double)] * 4
         lib.osqp_amd_time_pcg_kernels.restype = C.c_int
+        lib.osqp_amd_time_fp64_kernels.argtypes = [C.POINTER(T.OSQPSolver), C.c_int] + [C.POINTER(C.c_double)] * 4
+        lib.osqp_amd_time_fp64_kernels.restype = C.c_int
         lib.osqp_amd_pcg_bytes.argtypes = [C.POINTER(T.OSQPSolver), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.osqp_amd_pcg_bytes.restype = C.c_int
         lib.osqp_algebra_device_name.argtypes = [C.c_char_p, C.c_int]
@@ -197,6 +199,15 @@ class OSQP:
         if rc:
             raise RuntimeError("osqp_amd_time_pcg_kernels failed")
         return dict(ms_av=a.value, ms_kv=k.value, bytes_av=ba.value, bytes_kv=bk.value)
+
+    def time_fp64_kernels(self, reps=20):
+        """Mean ms and algorithmic bytes of one launch of the defect
+        correction's FP64 panel SpMVs (A x and [P A'] [x; w]); None when the
+        operators are not panelled."""
+        a, k, ba, bk = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        if self.lib.osqp_amd_time_fp64_kernels(self._solver, reps, C.byref(a), C.byref(k), C.byref(ba), C.byref(bk)):
+            return None
+        return dict(ms_a=a.value, ms_k=k.value, bytes_a=ba.value, bytes_k=bk.value)
 
     def pcg_bytes(self):
         """Algorithmic HBM bytes of one CG iteration and of one linear solve's
