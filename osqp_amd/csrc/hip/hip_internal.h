@@ -22,9 +22,10 @@ struct Runtime {
   int device = -1;
   int refcount = 0;
   hipStream_t stream = nullptr;
-  double* h_scalars = nullptr;     // pinned host slots for reduction results
-  double* d_partials = nullptr;    // per-block partials (2 * kMaxReduceBlocks doubles)
-  double* d_scalars = nullptr;     // device result slots
+  double* h_scalars = nullptr;     // host-coherent slots for reduction results
+  double* dh_scalars = nullptr;    // their device address
+  double* d_partials = nullptr;    // per-block partials (4 * kMaxReduceBlocks doubles)
+  unsigned* d_ticket = nullptr;    // finished-block counter of the running reduction
   int last_error = 0;
   int num_cu = 0;
   char device_name[256] = {0};
@@ -188,6 +189,8 @@ inline int rows_grid(long long rows, int G, int cap) {
 
 constexpr int kRowGridCap = 2048;  // 8192 wavefronts: every SIMD of 256 CUs 8-deep
 
+struct PanelMatrix;
+
 }  // namespace osqp_hip
 
 // ---- concrete types behind the opaque handles -----------------------------
@@ -231,4 +234,9 @@ struct OSQPMatrix_ {
   // row-block partitions (nonzero-balanced) of the CSR copy and of A'
   int rb_n, trb_n;
   int *rb_rows, *trb_rows;
+  // FP64 column panels of A and of A' (general matrices; matrix_panels64)
+  struct osqp_hip::PanelMatrix* pan64[2];
+  int pan_state[2];   // 0: not tried, 1: built, -1: not used
+  double* pan_part;   // per-(panel, row) partials of the panel products
+  size_t pan_part_len;
 };

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "hip_internal.h"
+#include "panel.h"
 #include "spmv_rb.h"
 
 using namespace osqp_hip;
@@ -173,7 +174,96 @@ void spmv(const int* rp, const int* ci, const double* val, int rows, long long n
   LAUNCHCHK();
 }
 
+// y = alpha * (sum of a row's panel partials) + beta * y (the row-block
+// kernel's epilogue)
+template <int G>
+__global__ void k_pan_axpy(int R, int np, const double* partial, double alpha, double beta, double* y) {
+  COMB_ROWS(i, j, R, G) {
+    const double s = panel_sum<double, G>(partial, np, R, i, j);
+    if (j == 0) {
+      double v = alpha * s;
+      if (beta != 0.0) v += beta * y[i];
+      y[i] = v;
+    }
+  }
+}
+
+// Products of matrices with at least this many nonzeros go through the
+// matrix's FP64 panels (built with the matrix); smaller ones stay on the
+// one-launch row-block kernel, where launch latency, not traffic, is the cost.
+constexpr long long kPanelAxpyMinNnz = 1LL << 20;
+
+// FP64 panels of M (t = 0) or M' (t = 1) from the host CSR structure of that
+// operator; marks the slot tried whatever the outcome.
+void build_matrix_panels(OSQPMatrix* M, int t, const int* rp, const int* ci) {
+  const int R = t ? M->n : M->m, C = t ? M->m : M->n;
+  M->pan_state[t] = -1;
+  if (M->is_triu || R <= 0 || C <= 0 || M->nnz <= 0 || !panel_worth(M->nnz, R, C, 64)) return;
+  PanelMatrix* P = (PanelMatrix*)calloc(1, sizeof(PanelMatrix));
+  if (!P) return;
+  if (panel_build(P, R, C, 64, rp, ci)) {
+    panel_free(P);
+    free(P);
+    return;
+  }
+  const size_t need = (size_t)P->np * (size_t)R;
+  if (need > M->pan_part_len) {
+    double* b = (double*)dev_alloc(sizeof(double) * need);
+    if (!b) {
+      panel_free(P);
+      free(P);
+      return;
+    }
+    dev_free(M->pan_part);
+    M->pan_part = b;
+    M->pan_part_len = need;
+  }
+  M->pan64[t] = P;
+  M->pan_state[t] = 1;
+}
+
+// y = alpha M x + beta y (t = 0) or alpha M' x + beta y (t = 1) over the
+// panels; false when the product stays on the row-block kernel
+bool panel_axpy(const OSQPMatrix* M, int t, const double* x, double* y, double alpha, double beta) {
+  if (M->is_triu || !x || alpha == 0.0 || M->nnz < kPanelAxpyMinNnz) return false;
+  const PanelMatrix* P = matrix_panels64(M, t);
+  if (!P || P->ntask <= 0) return false;
+  panel_spmv(*P, nullptr, x, P->C, (const double*)nullptr, M->pan_part);
+  const int R = P->R, G = comb_group(P->np, R);
+  const dim3 g(grid_for((long long)R * G, kBlock, 8192));
+  switch (G) {
+    case 8: hipLaunchKernelGGL(k_pan_axpy<8>, g, dim3(kBlock), 0, S(), R, P->np, (const double*)M->pan_part, alpha, beta, y); break;
+    case 4: hipLaunchKernelGGL(k_pan_axpy<4>, g, dim3(kBlock), 0, S(), R, P->np, (const double*)M->pan_part, alpha, beta, y); break;
+    case 2: hipLaunchKernelGGL(k_pan_axpy<2>, g, dim3(kBlock), 0, S(), R, P->np, (const double*)M->pan_part, alpha, beta, y); break;
+    default: hipLaunchKernelGGL(k_pan_axpy<1>, g, dim3(kBlock), 0, S(), R, P->np, (const double*)M->pan_part, alpha, beta, y); break;
+  }
+  LAUNCHCHK();
+  return true;
+}
+
 }  // namespace
+
+namespace osqp_hip {
+
+const PanelMatrix* matrix_panels64(const OSQPMatrix* Mc, int t) {
+  OSQPMatrix* M = (OSQPMatrix*)Mc;
+  if (M->is_triu) return nullptr;
+  if (M->pan_state[t] == 0) {
+    const int R = t ? M->n : M->m;
+    std::vector<int> hrp((size_t)R + 1), hci((size_t)(M->nnz > 0 ? M->nnz : 1));
+    download(hrp.data(), t ? M->trp : M->rp, sizeof(int) * hrp.size());
+    if (M->nnz) download(hci.data(), t ? M->tci : M->ci, sizeof(int) * (size_t)M->nnz);
+    build_matrix_panels(M, t, hrp.data(), hci.data());
+  }
+  PanelMatrix* P = M->pan64[t];
+  if (P && P->ver != M->version) {
+    panel_refresh_values(P, t ? M->tval : M->val);
+    P->ver = M->version;
+  }
+  return P;
+}
+
+}  // namespace osqp_hip
 
 // ---- host-side structure builders (also used by the batched engine) ----------
 namespace osqp_hip {
@@ -293,6 +383,12 @@ OSQPMatrix* OSQPMatrix_new_from_csc(const OSQPCscMatrix* A, OSQPInt is_triu) {
     M->tlanes = lanes_for_rows(nnz, n);
     M->rb_rows = upload_row_blocks(rp.data(), m, &M->rb_n);
     M->trb_rows = upload_row_blocks(A->p, n, &M->trb_n);
+    // large operators: FP64 panels of A and A' now, from the host structure
+    // (values follow on first use)
+    if (nnz >= kPanelAxpyMinNnz) {
+      build_matrix_panels(M, 0, rp.data(), ci.data());
+      build_matrix_panels(M, 1, A->p, A->i);
+    }
   } else {
     build_full_from_triu(n, A->p, A->i, rp, ci, pos, mirror, diag);
     const int full = rp[(size_t)n];
@@ -325,6 +421,12 @@ void OSQPMatrix_free(OSQPMatrix* M) {
   dev_free(M->csc2csr); dev_free(M->csc2mirror); dev_free(M->diag_pos);
   dev_free(M->val32); dev_free(M->tval32);
   dev_free(M->rb_rows); dev_free(M->trb_rows);
+  for (int t = 0; t < 2; ++t)
+    if (M->pan64[t]) {
+      panel_free(M->pan64[t]);
+      free(M->pan64[t]);
+    }
+  dev_free(M->pan_part);
   free(M);
 }
 
@@ -392,6 +494,7 @@ void OSQPMatrix_rmult_diag(OSQPMatrix* A, const OSQPVectorf* R) {
 
 void OSQPMatrix_Axpy(const OSQPMatrix* A, const OSQPVectorf* x, OSQPVectorf* y, OSQPFloat alpha, OSQPFloat beta) {
   int rows = A->is_triu ? A->n : A->m;
+  if (panel_axpy(A, 0, x ? x->d : nullptr, y->d, alpha, beta)) return;
   spmv(A->rp, A->ci, A->val, rows, A->nnz, A->rb_rows, A->rb_n, x ? x->d : nullptr, y->d, alpha, beta);
 }
 
@@ -399,6 +502,7 @@ void OSQPMatrix_Atxpy(const OSQPMatrix* A, const OSQPVectorf* x, OSQPVectorf* y,
   if (A->is_triu) {
     spmv(A->rp, A->ci, A->val, A->n, A->nnz, A->rb_rows, A->rb_n, x->d, y->d, alpha, beta);
   } else {
+    if (panel_axpy(A, 1, x ? x->d : nullptr, y->d, alpha, beta)) return;
     spmv(A->trp, A->tci, A->tval, A->n, A->nnz_in, A->trb_rows, A->trb_n, x ? x->d : nullptr, y->d, alpha, beta);
   }
 }

@@ -74,4 +74,47 @@ void panel_spmv(const PanelMatrix& P, const int* active, const double* x0, int n
 // descriptors
 double panel_spmv_bytes(const PanelMatrix& P, int value_bytes);
 
+// FP64 panels of a general (non-triu) matrix M (transpose = 0: m x n) or of
+// its transpose (transpose = 1: n x m), owned by M: built on the first call
+// when they pay (panel_worth), refreshed to M's current values on every call.
+// Null when they do not pay, for triu matrices, or when the build failed.
+const PanelMatrix* matrix_panels64(const OSQPMatrix* M, int transpose);
+
+// ---- combining the per-panel partials of a row -------------------------------
+// A group of G consecutive lanes serves one row: lane j adds the partials of
+// panels [j np / G, (j + 1) np / G) in panel order (loads four at a time), and
+// a fixed xor butterfly adds the G sums (every lane of the group ends with the
+// same bits).  Lane 0 of the group runs the epilogue.  Deterministic.
+template <typename TX, int G>
+__device__ __forceinline__ TX panel_sum(const TX* __restrict__ partial, int np, int R, int row, int j) {
+  const int p0 = (np * j) / G, p1 = (np * (j + 1)) / G;
+  TX s = 0;
+  int p = p0;
+  for (; p + 4 <= p1; p += 4) {
+    const TX a0 = partial[(long long)p * R + row], a1 = partial[(long long)(p + 1) * R + row];
+    const TX a2 = partial[(long long)(p + 2) * R + row], a3 = partial[(long long)(p + 3) * R + row];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
+  }
+  for (; p < p1; ++p) s += partial[(long long)p * R + row];
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) s += __shfl_xor(s, off, G);
+  return s;
+}
+
+// lanes per row: enough for about 256K threads, at least 4 panels per lane
+inline int comb_group(int np, int rows) {
+  int G = 1;
+  while (G < 8 && (long long)rows * G < 262144 && np >= 4 * G) G *= 2;
+  return G;
+}
+
+#define COMB_ROWS(i, j, R, G)                                                               \
+  for (long long _t = (long long)blockIdx.x * blockDim.x + threadIdx.x, _stride = (long long)gridDim.x * blockDim.x; \
+       _t < (long long)(R) * (G); _t += _stride)                                           \
+    if (const int i = (int)(_t / (G)), j = (int)(_t % (G)); true)
+
+
 }  // namespace osqp_hip

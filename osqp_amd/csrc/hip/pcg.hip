@@ -212,42 +212,6 @@ __global__ void k_kfill(int n, const int* prp, const int* pci, const double* pva
   }
 }
 
-// ---- panel combines: sum the per-panel partials of a row ---------------------
-// A group of G consecutive lanes serves one row: lane j adds the partials of
-// panels [j np / G, (j + 1) np / G) in panel order (loads four at a time), and
-// a fixed xor butterfly adds the G sums (every lane of the group ends with the
-// same bits).  Lane 0 of the group runs the epilogue.  Deterministic.
-template <typename TX, int G>
-__device__ __forceinline__ TX panel_sum(const TX* __restrict__ partial, int np, int R, int row, int j) {
-  const int p0 = (np * j) / G, p1 = (np * (j + 1)) / G;
-  TX s = 0;
-  int p = p0;
-  for (; p + 4 <= p1; p += 4) {
-    const TX a0 = partial[(long long)p * R + row], a1 = partial[(long long)(p + 1) * R + row];
-    const TX a2 = partial[(long long)(p + 2) * R + row], a3 = partial[(long long)(p + 3) * R + row];
-    s += a0;
-    s += a1;
-    s += a2;
-    s += a3;
-  }
-  for (; p < p1; ++p) s += partial[(long long)p * R + row];
-#pragma unroll
-  for (int off = 1; off < G; off <<= 1) s += __shfl_xor(s, off, G);
-  return s;
-}
-
-// lanes per row: enough for about 256K threads, at least 4 panels per lane
-inline int comb_group(int np, int rows) {
-  int G = 1;
-  while (G < 8 && (long long)rows * G < 262144 && np >= 4 * G) G *= 2;
-  return G;
-}
-
-#define COMB_ROWS(i, j, R, G)                                                               \
-  for (long long _t = (long long)blockIdx.x * blockDim.x + threadIdx.x, _stride = (long long)gridDim.x * blockDim.x; \
-       _t < (long long)(R) * (G); _t += _stride)                                           \
-    if (const int i = (int)(_t / (G)), j = (int)(_t % (G)); true)
-
 // w = rho o (A v)
 template <typename TX, int G>
 __global__ void k_comb_av(const PcgState* st, int m, int np, const TX* partial, const double* rho, double rho_sc,
@@ -600,16 +564,19 @@ struct HipPcg {
   // column-panelled copies of A (m x n) and K (n x (n+m)) for the hot SpMVs:
   // FP32 ones for the mixed CG, FP64 ones for the residual and z = A x (and
   // for everything in FP64 mode).  An operator whose panels do not pay
-  // (panel_worth) has none (ntask == 0) and uses the row-block CSR kernels.
-  PanelMatrix pA32, pK32, pA64, pK64;
+  // (panel_worth) has none (ntask == 0 / null) and uses the row-block CSR
+  // kernels.  The FP64 panels of A are the matrix's own (matrix_panels64,
+  // shared with the residual products of the termination checks).
+  PanelMatrix pA32, pK32, pK64;
+  const PanelMatrix* pA64;
   void *partA, *partK;  // per-panel partial sums (sized for the largest use)
 };
 
 // the panel copy of A / K for value type V, or null when CSR kernels are used
 template <typename V>
 const PanelMatrix* panels_A(const HipPcg* s) {
-  const PanelMatrix* P = sizeof(V) == 4 ? &s->pA32 : &s->pA64;
-  return P->ntask > 0 ? P : nullptr;
+  const PanelMatrix* P = sizeof(V) == 4 ? &s->pA32 : s->pA64;
+  return P && P->ntask > 0 ? P : nullptr;
 }
 template <typename V>
 const PanelMatrix* panels_K(const HipPcg* s) {
@@ -648,13 +615,12 @@ void refresh_kvals(HipPcg* s) {
 }
 
 void refresh_panels(HipPcg* s) {
-  PanelMatrix* As[2] = {&s->pA32, &s->pA64};
   PanelMatrix* Ks[2] = {&s->pK32, &s->pK64};
-  for (PanelMatrix* P : As)
-    if (P->ntask > 0 && P->ver != s->A->version) {
-      panel_refresh_values(P, s->A->val);
-      P->ver = s->A->version;
-    }
+  if (s->pA32.ntask > 0 && s->pA32.ver != s->A->version) {
+    panel_refresh_values(&s->pA32, s->A->val);
+    s->pA32.ver = s->A->version;
+  }
+  if (s->pA64) s->pA64 = matrix_panels64(s->A, 0);  // refreshes the values
   const long long kv = s->P->version * 1000003LL + s->A->version;
   for (PanelMatrix* P : Ks)
     if (P->ntask > 0 && P->ver != kv) {
@@ -687,17 +653,18 @@ int build_panels(HipPcg* s) {
       bytesK = b > bytesK ? b : bytesK;
     }
   }
-  if (needA) {
+  if (needA && s->bits == 32 && panel_worth(nnzA, m, n, 32)) {
     std::vector<int> arp((size_t)m + 1), aci((size_t)(nnzA > 0 ? nnzA : 1));
     download(arp.data(), s->A->rp, sizeof(int) * arp.size());
     if (nnzA) download(aci.data(), s->A->ci, sizeof(int) * (size_t)nnzA);
-    for (int bits : prec) {
-      if ((bits == 32 && s->bits != 32) || !panel_worth(nnzA, m, n, bits)) continue;
-      PanelMatrix* P = bits == 32 ? &s->pA32 : &s->pA64;
-      if (panel_build(P, m, n, bits, arp.data(), aci.data())) return 1;
-      const size_t b = (size_t)P->np * (size_t)m * (bits / 8);
-      bytesA = b > bytesA ? b : bytesA;
-    }
+    if (panel_build(&s->pA32, m, n, 32, arp.data(), aci.data())) return 1;
+    bytesA = (size_t)s->pA32.np * (size_t)m * sizeof(float);
+  }
+  if (needA && panel_worth(nnzA, m, n, 64)) {
+    s->pA64 = matrix_panels64(s->A, 0);
+    if (!s->pA64) return 1;
+    const size_t b = (size_t)s->pA64->np * (size_t)m * sizeof(double);
+    bytesA = b > bytesA ? b : bytesA;
   }
   if (bytesA && !(s->partA = dev_alloc(bytesA))) return 1;
   if (bytesK && !(s->partK = dev_alloc(bytesK))) return 1;
@@ -1017,7 +984,7 @@ void pcg_free(LinSysSolver* self) {
   dev_free(s->w); dev_free(s->dinv); dev_free(s->dtmp);
   dev_free(s->st); dev_free(s->parts);
   dev_free(s->krp); dev_free(s->kci); dev_free(s->kval64); dev_free(s->kval32); dev_free(s->k_rows);
-  panel_free(&s->pA32); panel_free(&s->pK32); panel_free(&s->pA64); panel_free(&s->pK64);
+  panel_free(&s->pA32); panel_free(&s->pK32); panel_free(&s->pK64);
   dev_free(s->partA); dev_free(s->partK);
   if (s->h_st) (void)hipHostFree(s->h_st);
   if (s->h_flags) (void)hipHostFree(s->h_flags);

@@ -130,9 +130,15 @@ OSQPInt osqp_algebra_init_libs(OSQPInt device) {
   if (HIPCHK(hipSetDevice(device))) return 1;
   // default priority stream (no priority streams on this platform)
   if (HIPCHK(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking))) return 1;
-  if (HIPCHK(hipHostMalloc((void**)&g_rt.h_scalars, 64 * sizeof(double), hipHostMallocDefault))) return 1;
+  // reduction results go from the last block of the reducing kernel straight
+  // to host-coherent memory (no device-to-host copy after it)
+  if (HIPCHK(hipHostMalloc((void**)&g_rt.h_scalars, 64 * sizeof(double),
+                           hipHostMallocMapped | hipHostMallocCoherent)))
+    return 1;
+  if (HIPCHK(hipHostGetDevicePointer((void**)&g_rt.dh_scalars, g_rt.h_scalars, 0))) return 1;
   if (HIPCHK(hipMalloc((void**)&g_rt.d_partials, 4 * kMaxReduceBlocks * sizeof(double)))) return 1;
-  if (HIPCHK(hipMalloc((void**)&g_rt.d_scalars, 64 * sizeof(double)))) return 1;
+  if (HIPCHK(hipMalloc((void**)&g_rt.d_ticket, 64 * sizeof(unsigned)))) return 1;
+  if (HIPCHK(hipMemset(g_rt.d_ticket, 0, 64 * sizeof(unsigned)))) return 1;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     g_rt.num_cu = prop.multiProcessorCount;
@@ -157,12 +163,13 @@ void osqp_algebra_free_libs(void) {
   HIPCHK(hipStreamSynchronize(g_rt.stream));
   release_cache();
   (void)hipFree(g_rt.d_partials);
-  (void)hipFree(g_rt.d_scalars);
+  (void)hipFree(g_rt.d_ticket);
   (void)hipHostFree(g_rt.h_scalars);
   (void)hipStreamDestroy(g_rt.stream);
   g_rt.d_partials = nullptr;
-  g_rt.d_scalars = nullptr;
+  g_rt.d_ticket = nullptr;
   g_rt.h_scalars = nullptr;
+  g_rt.dh_scalars = nullptr;
   g_rt.stream = nullptr;
   g_rt.device = -1;
 }

@@ -134,7 +134,34 @@ __device__ inline double red_elem(int op, const RedArgs& r, long long i) {
   return 0.0;
 }
 
-__global__ void k_reduce_partials(int op, RedArgs r, int n, double* partials) {
+// The last block of a reducing kernel to finish (counted on `ticket`) adds
+// the per-block partials in index order -- the same order whatever the block
+// timing, so results are deterministic -- writes the result to host-visible
+// memory and re-arms the ticket for the next reduction on the stream.
+__device__ void finish_reduce(bool mx, double t, double* partials, unsigned* ticket, double* out, double* lds) {
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t;
+    __threadfence();  // the partial is visible device-wide before the ticket
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    if (last) __threadfence();  // every other block's partial is visible here
+  }
+  __syncthreads();
+  if (!last) return;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+    const double v = partials[i];
+    if (mx) acc = v > acc ? v : acc;
+    else acc += v;
+  }
+  const double r = mx ? block_max(acc, lds) : block_sum(acc, lds);
+  if (threadIdx.x == 0) {
+    *out = r;
+    *ticket = 0;
+  }
+}
+
+__global__ void k_reduce(int op, RedArgs r, int n, double* partials, unsigned* ticket, double* out) {
   __shared__ double lds[kBlock / kWave];
   const bool mx = is_max_op(op);
   double acc = 0.0;
@@ -143,39 +170,26 @@ __global__ void k_reduce_partials(int op, RedArgs r, int n, double* partials) {
     if (mx) acc = v > acc ? v : acc;
     else acc += v;
   }
-  double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  const double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
+  finish_reduce(mx, t, partials, ticket, out, lds);
 }
 
-__global__ void k_reduce_final(int op, const double* partials, int nparts, double* out) {
-  __shared__ double lds[kBlock / kWave];
-  const bool mx = is_max_op(op);
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-    double v = partials[i];
-    if (mx) acc = v > acc ? v : acc;
-    else acc += v;
-  }
-  double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
-  if (threadIdx.x == 0) *out = t;
-}
-
+// one kernel, then the stream is synchronised and the host reads the result
 double reduce(int op, const RedArgs& r, int n) {
   if (n <= 0) return 0.0;
   Runtime& R = rt();
   int g = reduce_grid(n);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(g), dim3(kBlock), 0, R.stream, op, r, n, R.d_partials);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, R.stream, op, R.d_partials, g, R.d_scalars);
+  hipLaunchKernelGGL(k_reduce, dim3(g), dim3(kBlock), 0, R.stream, op, r, n, R.d_partials, R.d_ticket,
+                     R.dh_scalars);
   LAUNCHCHK();
-  HIPCHK(hipMemcpyAsync(R.h_scalars, R.d_scalars, sizeof(double), hipMemcpyDeviceToHost, R.stream));
   HIPCHK(hipStreamSynchronize(R.stream));
-  return R.h_scalars[0];
+  return ((volatile double*)R.h_scalars)[0];
 }
 
 // ew_bounds_type: classify constraints (vector.c:885-921; loose checked
 // before equality, as the builtin does) and report whether any type changed.
 __global__ void k_bounds_type(int* iseq, const double* l, const double* u, double tol, double infval, int n,
-                              double* partials) {
+                              double* partials, unsigned* ticket, double* out) {
   __shared__ double lds[kBlock / kWave];
   double changed = 0.0;
   GRID_STRIDE(i, n) {
@@ -186,8 +200,8 @@ __global__ void k_bounds_type(int* iseq, const double* l, const double* u, doubl
     iseq[i] = t;
     if (t != old) changed = 1.0;
   }
-  double tt = block_max(changed, lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = tt;
+  const double tt = block_max(changed, lds);
+  finish_reduce(true, tt, partials, ticket, out, lds);
 }
 
 inline int egrid(int n) { return grid_for(n, kBlock, 8192); }
@@ -465,13 +479,10 @@ OSQPInt OSQPVectorf_ew_bounds_type(OSQPVectori* iseq, const OSQPVectorf* l, cons
   Runtime& R = rt();
   int g = reduce_grid(n);
   hipLaunchKernelGGL(k_bounds_type, dim3(g), dim3(kBlock), 0, R.stream, iseq->d, l->d, u->d, tol, infval, n,
-                     R.d_partials);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, R.stream, (int)R_NORM_INF, R.d_partials, g,
-                     R.d_scalars);
+                     R.d_partials, R.d_ticket, R.dh_scalars);
   LAUNCHCHK();
-  HIPCHK(hipMemcpyAsync(R.h_scalars, R.d_scalars, sizeof(double), hipMemcpyDeviceToHost, R.stream));
   HIPCHK(hipStreamSynchronize(R.stream));
-  return R.h_scalars[0] > 0.0 ? 1 : 0;
+  return ((volatile double*)R.h_scalars)[0] > 0.0 ? 1 : 0;
 }
 
 void osqp_amd_admm_rhs(OSQPVectorf* xt, OSQPVectorf* zt, OSQPFloat sigma, const OSQPVectorf* x_prev,

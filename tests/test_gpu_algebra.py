@@ -177,7 +177,9 @@ def _bound(Mabs, xabs, rowlen):
     return 4.0 * EPS * np.maximum(rowlen, 1) * (Mabs @ xabs) + 1e-300
 
 
-@pytest.mark.parametrize("m,n,d", [(50, 30, 0.2), (3000, 800, 0.01), (20000, 2000, 0.005)])
+# the last case has 1.2e6 nonzeros: A x and A' y go through the matrix's FP64
+# column panels (products of matrices with >= 2^20 nonzeros)
+@pytest.mark.parametrize("m,n,d", [(50, 30, 0.2), (3000, 800, 0.01), (20000, 2000, 0.005), (200000, 20000, 3e-4)])
 def test_spmv(algebra, m, n, d):
     A, Pf, rng = _rand_mats(m + n, m, n, d)
     L = algebra.lib()
@@ -201,6 +203,37 @@ def test_spmv(algebra, m, n, d):
     L.OSQPMatrix_Axpy(MA.h, vx.h, vn.h, 1.0, 0.0)
     assert np.all(np.isfinite(vn.get()))
     for v in (vx, vy, vyy, vo, vp, vn, MA, MP):
+        v.free()
+
+
+def test_panel_spmv_follows_value_updates(algebra):
+    """A large operator (panel path) after update_values and diagonal scaling:
+    the panel copies are refreshed before the next product."""
+    m, n = 200000, 20000
+    A, _, rng = _rand_mats(7, m, n, 3e-4)
+    L = algebra.lib()
+    MA = Mat(A)
+    x, yv = rng.standard_normal(n), rng.standard_normal(m)
+    vx, vy, vyv, vo = Vec(x), Vec(n=m), Vec(yv), Vec(n=n)
+    L.OSQPMatrix_Axpy(MA.h, vx.h, vy.h, 1.0, 0.0)
+    np.testing.assert_allclose(vy.get(), A @ x, rtol=1e-12, atol=1e-12)
+    idx = np.array([0, 11, A.nnz - 1], dtype=np.int32)
+    newv = np.array([7.0, -3.0, 0.25])
+    L.OSQPMatrix_update_values(MA.h, newv.ctypes.data, idx.ctypes.data, 3)
+    A2 = A.copy()
+    A2.data[idx] = newv
+    L.OSQPMatrix_Axpy(MA.h, vx.h, vy.h, 1.0, 0.0)
+    np.testing.assert_allclose(vy.get(), A2 @ x, rtol=1e-12, atol=1e-12)
+    dl, dr = rng.uniform(0.5, 2.0, m), rng.uniform(0.5, 2.0, n)
+    vdl, vdr = Vec(dl), Vec(dr)
+    L.OSQPMatrix_lmult_diag(MA.h, vdl.h)
+    L.OSQPMatrix_rmult_diag(MA.h, vdr.h)
+    A3 = sp.diags(dl) @ A2 @ sp.diags(dr)
+    L.OSQPMatrix_Atxpy(MA.h, vyv.h, vo.h, 1.0, 0.0)
+    np.testing.assert_allclose(vo.get(), A3.T @ yv, rtol=1e-11, atol=1e-11)
+    L.OSQPMatrix_Axpy(MA.h, vx.h, vy.h, 1.0, 0.0)
+    np.testing.assert_allclose(vy.get(), A3 @ x, rtol=1e-11, atol=1e-11)
+    for v in (MA, vx, vy, vyv, vo, vdl, vdr):
         v.free()
 
 
