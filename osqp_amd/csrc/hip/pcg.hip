@@ -735,8 +735,11 @@ void update_precond(HipPcg* s) {
 // 2 * solve + active; while it still holds an older solve's value the GPU has
 // not got there yet and the host waits (yielding), bounded by a deadline after
 // which it simply keeps queueing (the kernels exit on the device flag anyway).
+// Iteration 0 is always queued.  Short iterations (small operators) run two
+// ahead; a skipped iteration costs six launches, less than waiting for the
+// host.  Long ones use lookahead 0 with one launch queued ahead (solve_impl).
 bool host_should_stop(HipPcg* s, int it) {
-  if (!s->h_flags || it < s->lookahead) return false;
+  if (!s->h_flags || it < (s->lookahead > 0 ? s->lookahead : 1)) return false;
   volatile int* f = (volatile int*)s->h_flags;
   const int want = 2 * s->host_solves;
   struct timespec t0, t;
@@ -761,33 +764,46 @@ const V* csr_vals_K(HipPcg* s) {
 }
 const int* active_flag(HipPcg* s) { return (const int*)((const char*)s->st + offsetof(PcgState, active)); }
 
-// w = rho o (A v)  (skipped while the CG state is inactive)
+// w = rho o (A v)  (skipped while the CG state is inactive), in two launches
+// where A is panelled: the panel SpMV (av_spmv) and the combine (av_finish);
+// the row-block CSR kernel does both in av_spmv.
 template <typename V>
-void op_av_rho(HipPcg* s, const V* v, V* w) {
-  const int m = s->m;
+void av_spmv(HipPcg* s, const V* v, V* w) {
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
   if (const PanelMatrix* P = panels_A<V>(s)) {
     panel_spmv(*P, active_flag(s), v, s->n, (const V*)nullptr, (V*)s->partA);
-    const int G = comb_group(P->np, m);
-    const dim3 g(grid_for((long long)m * G, kBlock, 8192));
-    if (G == 8)
-      hipLaunchKernelGGL((k_comb_av<V, 8>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                         (const V*)s->partA, rho, s->rho_sc, w);
-    else if (G == 4)
-      hipLaunchKernelGGL((k_comb_av<V, 4>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                         (const V*)s->partA, rho, s->rho_sc, w);
-    else if (G == 2)
-      hipLaunchKernelGGL((k_comb_av<V, 2>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                         (const V*)s->partA, rho, s->rho_sc, w);
-    else
-      hipLaunchKernelGGL((k_comb_av<V, 1>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                         (const V*)s->partA, rho, s->rho_sc, w);
   } else {
     const int g = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
     hipLaunchKernelGGL((k_rb_Av_rho<V, V>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st,
                        (const int*)s->A->rb_rows, s->A->rb_n, (const int*)s->A->rp, (const int*)s->A->ci,
                        csr_vals_A<V>(s), v, rho, s->rho_sc, w);
   }
+}
+template <typename V>
+void av_finish(HipPcg* s, V* w) {
+  const int m = s->m;
+  const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
+  const PanelMatrix* P = panels_A<V>(s);
+  if (!P) return;
+  const int G = comb_group(P->np, m);
+  const dim3 g(grid_for((long long)m * G, kBlock, 8192));
+  if (G == 8)
+    hipLaunchKernelGGL((k_comb_av<V, 8>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                       (const V*)s->partA, rho, s->rho_sc, w);
+  else if (G == 4)
+    hipLaunchKernelGGL((k_comb_av<V, 4>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                       (const V*)s->partA, rho, s->rho_sc, w);
+  else if (G == 2)
+    hipLaunchKernelGGL((k_comb_av<V, 2>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                       (const V*)s->partA, rho, s->rho_sc, w);
+  else
+    hipLaunchKernelGGL((k_comb_av<V, 1>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
+                       (const V*)s->partA, rho, s->rho_sc, w);
+}
+template <typename V>
+void op_av_rho(HipPcg* s, const V* v, V* w) {
+  av_spmv<V>(s, v, w);
+  av_finish<V>(s, w);
 }
 
 // Kv = sigma v + [P A'] [v; w] (- sub) and block partials (v'Kv, or
@@ -897,15 +913,26 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   hipLaunchKernelGGL(k_loop_first, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
                      gE, s->max_iter, guard, s->d_flags);
   // one CG iteration: the operator (two SpMVs with their combines), the
-  // update, and the next loop head fused with the search-direction update
+  // update, and the next loop head fused with the search-direction update.
+  // Without speculation (lookahead 0) the first launch of iteration it + 1
+  // (A's panel SpMV) is queued before the host waits for iteration it + 1's
+  // go, so the GPU has that SpMV to run while the host reacts; a stop skips
+  // just that one launch.
+  const bool ahead = s->lookahead == 0 && m > 0;
+  if (ahead) av_spmv<V>(s, (const V*)p, w);
   for (int it = 0; it < s->max_iter; ++it) {
     if (host_should_stop(s, it)) break;
-    if (m) op_av_rho<V>(s, (const V*)p, w);
+    if (ahead) {
+      av_finish<V>(s, w);
+    } else if (m) {
+      op_av_rho<V>(s, (const V*)p, w);
+    }
     const int nk = op_kv<V>(s, (const V*)p, (const V*)w, (const double*)nullptr, Kp, part2);
     hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nk, it, guard, n,
                        d, r, y, (const V*)p, (const V*)Kp, dinv, part0, part1);
     hipLaunchKernelGGL(k_head_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part0,
                        (const double*)part1, gE, it + 1, s->max_iter, guard, s->d_flags, n, p, (const V*)y);
+    if (ahead && it + 1 < s->max_iter) av_spmv<V>(s, (const V*)p, w);
   }
   hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
                      (const V*)d, s->x64, b->d);
@@ -1095,6 +1122,10 @@ OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** sp, const OSQPMatrix* P, 
     return OSQP_LINSYS_SOLVER_INIT_ERROR;
   }
   update_precond(s);
+  // no speculative CG iterations where one moves more than 64 MB
+  double cg_bytes = 0.0;
+  osqp_amd_linsys_bytes(&s->base, &cg_bytes, nullptr);
+  s->lookahead = cg_bytes > 64e6 ? 0 : 2;
   return 0;
 }
 

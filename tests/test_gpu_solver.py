@@ -253,6 +253,35 @@ def test_multi_panel_qp(precision):
     assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
 
 
+def test_large_qp_unspeculated_cg():
+    """A QP whose CG iteration moves more than 64 MB (6e6 nonzeros in A), where
+    the host queues no CG iteration before the previous one has decided to go
+    on: both engines solve it, the FP32 engine's point matches the FP64 one and
+    meets the FP64 KKT certificate, and a warm re-solve after a q update
+    agrees again."""
+    p = pr.random_qp(20000, 200000, 1e-4, 1.5e-3, seed=5)
+    assert p["A"].nnz > 5_000_000
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000)
+    out = {}
+    for prec in (64, 32):
+        s, r = gpu_solve(p, prec, **st)
+        assert r.info.status_val == 1, r.info.status
+        c = pr.kkt_certificate(p, r.x, r.y)
+        assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+        assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+        st_ = s.stats()
+        assert st_.pcg_iters > 0 and st_.linsys_solves >= r.info.iter
+        q2 = p["q"] + 0.1 * np.random.default_rng(1).standard_normal(p["q"].shape[0])
+        s.update(q=q2)
+        r2 = s.solve()
+        assert r2.info.status_val == 1
+        out[prec] = (r, r2)
+        s.close()
+    for k in (0, 1):
+        a, b = out[32][k], out[64][k]
+        assert abs(a.info.obj_val - b.info.obj_val) <= 1e-3 * max(1.0, abs(b.info.obj_val))
+
+
 @pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("n,m_d,dens", [(40, 400, 0.1), (200, 2000, 0.02)])
 def test_lasso_vs_oracle(n, m_d, dens, precision):
