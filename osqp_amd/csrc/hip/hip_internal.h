@@ -26,6 +26,7 @@ struct Runtime {
   double* dh_scalars = nullptr;    // their device address
   double* d_partials = nullptr;    // per-block partials (4 * kMaxReduceBlocks doubles)
   unsigned* d_ticket = nullptr;    // finished-block counter of the running reduction
+  double red_seq = 0.0;            // sequence number of the last reduction
   int last_error = 0;
   int num_cu = 0;
   char device_name[256] = {0};

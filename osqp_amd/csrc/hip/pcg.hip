@@ -744,12 +744,17 @@ bool host_should_stop(HipPcg* s, int it) {
   const int want = 2 * s->host_solves;
   struct timespec t0, t;
   clock_gettime(CLOCK_MONOTONIC, &t0);
-  for (;;) {
+  // busy polling for the first millisecond (the answer usually comes within
+  // microseconds), yielding the CPU after that
+  for (long k = 1;; ++k) {
     int v = f[it - s->lookahead];
     if (v >= want) return (v & 1) == 0;
-    clock_gettime(CLOCK_MONOTONIC, &t);
-    if ((t.tv_sec - t0.tv_sec) > 5) return false;
-    sched_yield();
+    if ((k & 255) == 0) {
+      clock_gettime(CLOCK_MONOTONIC, &t);
+      const double el = (t.tv_sec - t0.tv_sec) + 1e-9 * (t.tv_nsec - t0.tv_nsec);
+      if (el > 5.0) return false;
+      if (el > 1e-3) sched_yield();
+    }
   }
 }
 

@@ -136,6 +136,8 @@ OSQPInt osqp_algebra_init_libs(OSQPInt device) {
                            hipHostMallocMapped | hipHostMallocCoherent)))
     return 1;
   if (HIPCHK(hipHostGetDevicePointer((void**)&g_rt.dh_scalars, g_rt.h_scalars, 0))) return 1;
+  memset(g_rt.h_scalars, 0, 64 * sizeof(double));
+  g_rt.red_seq = 0.0;
   if (HIPCHK(hipMalloc((void**)&g_rt.d_partials, 4 * kMaxReduceBlocks * sizeof(double)))) return 1;
   if (HIPCHK(hipMalloc((void**)&g_rt.d_ticket, 64 * sizeof(unsigned)))) return 1;
   if (HIPCHK(hipMemset(g_rt.d_ticket, 0, 64 * sizeof(unsigned)))) return 1;

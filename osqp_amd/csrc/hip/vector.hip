@@ -7,6 +7,7 @@
 // block), never atomics; max-type reductions are therefore bit-exact against
 // the CPU and sums differ only by summation order.
 #include <math.h>
+#include <time.h>
 
 #include "hip_internal.h"
 
@@ -137,8 +138,10 @@ __device__ inline double red_elem(int op, const RedArgs& r, long long i) {
 // The last block of a reducing kernel to finish (counted on `ticket`) adds
 // the per-block partials in index order -- the same order whatever the block
 // timing, so results are deterministic -- writes the result to host-visible
-// memory and re-arms the ticket for the next reduction on the stream.
-__device__ void finish_reduce(bool mx, double t, double* partials, unsigned* ticket, double* out, double* lds) {
+// memory (out[0], then the sequence number `seq` in out[1]) and re-arms the
+// ticket for the next reduction on the stream.
+__device__ void finish_reduce(bool mx, double t, double* partials, unsigned* ticket, double* out, double seq,
+                              double* lds) {
   __shared__ int last;
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = t;
@@ -156,12 +159,14 @@ __device__ void finish_reduce(bool mx, double t, double* partials, unsigned* tic
   }
   const double r = mx ? block_max(acc, lds) : block_sum(acc, lds);
   if (threadIdx.x == 0) {
-    *out = r;
     *ticket = 0;
+    out[0] = r;
+    __threadfence_system();  // the result reaches the host before its sequence number
+    ((volatile double*)out)[1] = seq;
   }
 }
 
-__global__ void k_reduce(int op, RedArgs r, int n, double* partials, unsigned* ticket, double* out) {
+__global__ void k_reduce(int op, RedArgs r, int n, double* partials, unsigned* ticket, double* out, double seq) {
   __shared__ double lds[kBlock / kWave];
   const bool mx = is_max_op(op);
   double acc = 0.0;
@@ -171,25 +176,45 @@ __global__ void k_reduce(int op, RedArgs r, int n, double* partials, unsigned* t
     else acc += v;
   }
   const double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
-  finish_reduce(mx, t, partials, ticket, out, lds);
+  finish_reduce(mx, t, partials, ticket, out, seq, lds);
 }
 
-// one kernel, then the stream is synchronised and the host reads the result
+// The host waits for reduction `seq` by watching its sequence slot (a stream
+// synchronisation sleeps and wakes tens of microseconds late, and the
+// termination checks run a dozen reductions back to back).  After half a
+// second it synchronises the stream instead, which also reports a failed
+// kernel.
+double wait_result(Runtime& R, double seq) {
+  volatile double* h = R.h_scalars;
+  struct timespec t0, t;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (long k = 1;; ++k) {
+    if (h[1] == seq) return h[0];
+    if ((k & 1023) == 0) {
+      clock_gettime(CLOCK_MONOTONIC, &t);
+      if ((t.tv_sec - t0.tv_sec) + 1e-9 * (t.tv_nsec - t0.tv_nsec) > 0.5) break;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(R.stream));
+  return h[0];
+}
+
+// one kernel; the host reads the result from host-coherent memory
 double reduce(int op, const RedArgs& r, int n) {
   if (n <= 0) return 0.0;
   Runtime& R = rt();
   int g = reduce_grid(n);
+  const double seq = ++R.red_seq;
   hipLaunchKernelGGL(k_reduce, dim3(g), dim3(kBlock), 0, R.stream, op, r, n, R.d_partials, R.d_ticket,
-                     R.dh_scalars);
-  LAUNCHCHK();
-  HIPCHK(hipStreamSynchronize(R.stream));
-  return ((volatile double*)R.h_scalars)[0];
+                     R.dh_scalars, seq);
+  if (LAUNCHCHK()) return 0.0;
+  return wait_result(R, seq);
 }
 
 // ew_bounds_type: classify constraints (vector.c:885-921; loose checked
 // before equality, as the builtin does) and report whether any type changed.
 __global__ void k_bounds_type(int* iseq, const double* l, const double* u, double tol, double infval, int n,
-                              double* partials, unsigned* ticket, double* out) {
+                              double* partials, unsigned* ticket, double* out, double seq) {
   __shared__ double lds[kBlock / kWave];
   double changed = 0.0;
   GRID_STRIDE(i, n) {
@@ -201,7 +226,7 @@ __global__ void k_bounds_type(int* iseq, const double* l, const double* u, doubl
     if (t != old) changed = 1.0;
   }
   const double tt = block_max(changed, lds);
-  finish_reduce(true, tt, partials, ticket, out, lds);
+  finish_reduce(true, tt, partials, ticket, out, seq, lds);
 }
 
 inline int egrid(int n) { return grid_for(n, kBlock, 8192); }
@@ -478,11 +503,11 @@ OSQPInt OSQPVectorf_ew_bounds_type(OSQPVectori* iseq, const OSQPVectorf* l, cons
   if (n <= 0) return 0;
   Runtime& R = rt();
   int g = reduce_grid(n);
+  const double seq = ++R.red_seq;
   hipLaunchKernelGGL(k_bounds_type, dim3(g), dim3(kBlock), 0, R.stream, iseq->d, l->d, u->d, tol, infval, n,
-                     R.d_partials, R.d_ticket, R.dh_scalars);
-  LAUNCHCHK();
-  HIPCHK(hipStreamSynchronize(R.stream));
-  return ((volatile double*)R.h_scalars)[0] > 0.0 ? 1 : 0;
+                     R.d_partials, R.d_ticket, R.dh_scalars, seq);
+  if (LAUNCHCHK()) return 0;
+  return wait_result(R, seq) > 0.0 ? 1 : 0;
 }
 
 void osqp_amd_admm_rhs(OSQPVectorf* xt, OSQPVectorf* zt, OSQPFloat sigma, const OSQPVectorf* x_prev,
