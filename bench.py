@@ -79,7 +79,11 @@ def serve_mpc(n_req, seed):
             log(f"[serve] mpc: {nxt} admitted, {len(lat)} done")
             t_log = time.perf_counter()
         now = time.perf_counter() - t_start
-        while nxt < n_req and arrivals[nxt] <= now and len(pending) < 4096:
+        # take at most 256 arrivals between two steps, so the engine works on
+        # admitted requests while the host takes in the next ones
+        taken = 0
+        while nxt < n_req and arrivals[nxt] <= now and len(pending) < 4096 and taken < 256:
+            taken += 1
             p = reqs[nxt]
             pid = eng.add(p["P"], p["q"], p["A"], p["l"], p["u"])
             arr_of[pid] = (nxt, arrivals[nxt])

@@ -152,8 +152,11 @@ class CscHolder:
 
     def __init__(self, M):
         import scipy.sparse as sp
-        M = sp.csc_matrix(M)
-        M.sort_indices()
+        # a CSC matrix with sorted int32 indices is used as it is (no copy)
+        if not sp.issparse(M) or M.format != "csc":
+            M = sp.csc_matrix(M)
+        if not M.has_sorted_indices:
+            M = M.sorted_indices()
         self.m, self.n = M.shape
         self.p = np.ascontiguousarray(M.indptr, dtype=np.int32)
         self.i = np.ascontiguousarray(M.indices, dtype=np.int32)

@@ -168,32 +168,68 @@ class RangeAlloc {
   std::map<size_t, size_t> free_, used_;
 };
 
+// row groups of one CSR matrix (the BCsr layout, built on the host)
+struct HostGroups {
+  int ng = 0;
+  std::vector<int> longs, empty, gk, grow, gci;
+};
+
+// Rows of 1..kLongRow entries are packed in row order into groups of
+// kGroupSlots entries; a row that does not fit the rest of the current group
+// starts the next one (the rest is padding), so no row crosses a group.
+void build_groups(const std::vector<int>& rp, const std::vector<int>& ci, int rows, HostGroups* g) {
+  *g = HostGroups();
+  // pass 1: group count
+  int ng = 0, fill = 0;
+  for (int r = 0; r < rows; ++r) {
+    const int len = rp[(size_t)r + 1] - rp[r];
+    if (len > kLongRow) {
+      g->longs.push_back(r);
+    } else if (len == 0) {
+      g->empty.push_back(r);
+    } else {
+      if (ng == 0 || fill + len > kGroupSlots) {
+        ng++;
+        fill = 0;
+      }
+      fill += len;
+    }
+  }
+  g->ng = ng;
+  const size_t total = (size_t)ng * kGroupSlots;
+  g->gk.assign(total, -1);
+  g->grow.assign(total, -1);
+  g->gci.assign(total, 0);
+  // pass 2: the same packing, writing the slots
+  int grp = -1;
+  fill = 0;
+  for (int r = 0; r < rows; ++r) {
+    const int s = rp[r], e = rp[(size_t)r + 1], len = e - s;
+    if (len > kLongRow || len == 0) continue;
+    if (grp < 0 || fill + len > kGroupSlots) {
+      grp++;
+      fill = 0;
+    }
+    int* gk = g->gk.data() + (size_t)grp * kGroupSlots;
+    int* gr = g->grow.data() + (size_t)grp * kGroupSlots;
+    int* gc = g->gci.data() + (size_t)grp * kGroupSlots;
+    for (int k = s; k < e; ++k) {
+      const int pos = fill++;  // the group's entry number: lane pos / 8, element pos % 8
+      const int slot = (pos % kGroupPer) * kWave + pos / kGroupPer;
+      gk[slot] = k;
+      gr[slot] = r;
+      gc[slot] = ci[(size_t)k];
+    }
+  }
+}
+
 // host copy of a problem in CSR form (values raw, unscaled)
 struct HostData {
   int n = 0, m = 0;
-  std::vector<int> Prp, Pci, Pdiag, Arp, Aci, Trp, Tci, Plong, Along, Tlong;
+  std::vector<int> Prp, Pci, Pdiag, Arp, Aci, Trp, Tci;
   std::vector<double> Pval, Aval, Tval, q, l, u, x0, y0;
-  int LP = 1, LA = 1, LT = 1;
+  HostGroups GP, GA, GT;
 };
-
-int lane_group(const std::vector<int>& rp, int rows, std::vector<int>* longs) {
-  long long nnz_short = 0;
-  int rows_short = 0;
-  longs->clear();
-  for (int r = 0; r < rows; ++r) {
-    const int len = rp[r + 1] - rp[r];
-    if (len > kLongRow) {
-      longs->push_back(r);
-    } else {
-      nnz_short += len;
-      rows_short++;
-    }
-  }
-  const double avg = rows_short ? (double)nnz_short / rows_short : 1.0;
-  int L = 1;
-  while (L < 64 && L < avg) L <<= 1;
-  return L;
-}
 
 // A (CSC, m x n) -> A CSR and A' CSR (= the CSC arrays); P (CSC upper
 // triangle) -> full symmetric CSR with sorted columns and diagonal positions
@@ -274,13 +310,18 @@ void build_host(HostData* h, const OSQPCscMatrix* P, const OSQPFloat* q, const O
   }
   (void)nnzPt;
   h->q.assign(q, q + n);
-  h->l.assign(l, l + m);
-  h->u.assign(u, u + m);
+  // bounds beyond +-OSQP_INFTY are clamped to it, as osqp_setup does
+  h->l.resize((size_t)m);
+  h->u.resize((size_t)m);
+  for (int i = 0; i < m; ++i) {
+    h->l[(size_t)i] = l[i] < -kInfty ? -kInfty : (l[i] > kInfty ? kInfty : l[i]);
+    h->u[(size_t)i] = u[i] < -kInfty ? -kInfty : (u[i] > kInfty ? kInfty : u[i]);
+  }
   if (x0) h->x0.assign(x0, x0 + n);
   if (y0) h->y0.assign(y0, y0 + m);
-  h->LP = lane_group(h->Prp, n, &h->Plong);
-  h->LA = lane_group(h->Arp, m, &h->Along);
-  h->LT = lane_group(h->Trp, n, &h->Tlong);
+  build_groups(h->Prp, h->Pci, n, &h->GP);
+  build_groups(h->Arp, h->Aci, m, &h->GA);
+  build_groups(h->Trp, h->Tci, n, &h->GT);
 }
 
 // byte layout of a problem's HBM block and of its pinned images
@@ -288,7 +329,10 @@ struct Layout {
   size_t desc;  // BProb at offset 0
   // static image (copied from the pinned stage at setup): offsets from block start
   size_t s_begin, s_end;
-  size_t Prp, Pci, Pval, Pdiag, Plong, Arp, Aci, Aval, Along, Trp, Tci, Tval, Tlong, q, l, u;
+  size_t Prp, Pci, Pval, Pdiag, Arp, Aci, Aval, Trp, Tci, Tval, q, l, u;
+  struct Groups {
+    size_t longs, empty, gk, grow, gci;
+  } gP, gA, gT;
   // dynamic
   size_t ctype, D, Dinv, E, Einv, rho, rhoinv, xa, xb, za, zb, y, xt, zt, dx, dy, Ax, Px, Aty, tn, tm, tn2;
   size_t x64, r64, rhs64, w64, ax64, prec, cd, cp, cKp, cr, cy, cw, cdinv;
@@ -315,15 +359,22 @@ Layout make_layout(const HostData& h, int vbytes) {
   L.Pci = take(4 * h.Pci.size());
   L.Pval = take(8 * h.Pval.size());
   L.Pdiag = take(4 * n);
-  L.Plong = take(4 * h.Plong.size());
   L.Arp = take(4 * (m + 1));
   L.Aci = take(4 * h.Aci.size());
   L.Aval = take(8 * h.Aval.size());
-  L.Along = take(4 * h.Along.size());
   L.Trp = take(4 * (n + 1));
   L.Tci = take(4 * h.Tci.size());
   L.Tval = take(8 * h.Tval.size());
-  L.Tlong = take(4 * h.Tlong.size());
+  auto groups = [&](Layout::Groups* o, const HostGroups& g) {
+    o->longs = take(4 * g.longs.size());
+    o->empty = take(4 * g.empty.size());
+    o->gk = take(4 * g.gk.size());
+    o->grow = take(4 * g.grow.size());
+    o->gci = take(4 * g.gci.size());
+  };
+  groups(&L.gP, h.GP);
+  groups(&L.gA, h.GA);
+  groups(&L.gT, h.GT);
   L.q = take(8 * n);
   L.l = take(8 * m);
   L.u = take(8 * m);
@@ -462,6 +513,9 @@ struct Engine {
   long long epochs = 0, admm_iters = 0;
   double gpu_s = 0.0;
   bool failed = false;
+  bool inflight = false;          // an epoch is running on the GPU
+  std::vector<int> resumed;       // resident problems re-solved since the last launch
+  bool launched[kBuckets] = {};   // the buckets it covers
 };
 
 namespace {
@@ -523,15 +577,22 @@ bool admit(Engine* E, HProb* p) {
   put(img, L.Pci - b, h.Pci);
   put(img, L.Pval - b, h.Pval);
   put(img, L.Pdiag - b, h.Pdiag);
-  put(img, L.Plong - b, h.Plong);
   put(img, L.Arp - b, h.Arp);
   put(img, L.Aci - b, h.Aci);
   put(img, L.Aval - b, h.Aval);
-  put(img, L.Along - b, h.Along);
   put(img, L.Trp - b, h.Trp);
   put(img, L.Tci - b, h.Tci);
   put(img, L.Tval - b, h.Tval);
-  put(img, L.Tlong - b, h.Tlong);
+  auto put_groups = [&](const Layout::Groups& o, const HostGroups& g) {
+    put(img, o.longs - b, g.longs);
+    put(img, o.empty - b, g.empty);
+    put(img, o.gk - b, g.gk);
+    put(img, o.grow - b, g.grow);
+    put(img, o.gci - b, g.gci);
+  };
+  put_groups(L.gP, h.GP);
+  put_groups(L.gA, h.GA);
+  put_groups(L.gT, h.GT);
   put(img, L.q - b, h.q);
   put(img, L.l - b, h.l);
   put(img, L.u - b, h.u);
@@ -545,19 +606,24 @@ bool admit(Engine* E, HProb* p) {
   P->set = E->set;
   P->st.phase = kPhaseSetup;
   P->st.rho = E->set.rho0;
-  auto csr = [&](BCsr* M, int rows, int Lg, const std::vector<int>& longs, size_t rp, size_t ci, size_t val,
-                 size_t lg) {
+  auto csr = [&](BCsr* M, int rows, const HostGroups& g, const Layout::Groups& o, size_t rp, size_t ci,
+                 size_t val) {
     M->rows = rows;
-    M->L = Lg;
-    M->nlong = (int)longs.size();
+    M->ng = g.ng;
+    M->nlong = (int)g.longs.size();
+    M->nempty = (int)g.empty.size();
     M->rp = (int*)(dev + rp);
     M->ci = (int*)(dev + ci);
     M->val = (double*)(dev + val);
-    M->longs = (int*)(dev + lg);
+    M->longs = (int*)(dev + o.longs);
+    M->empty = (int*)(dev + o.empty);
+    M->gk = (int*)(dev + o.gk);
+    M->grow = (int*)(dev + o.grow);
+    M->gci = (int*)(dev + o.gci);
   };
-  csr(&P->P, h.n, h.LP, h.Plong, L.Prp, L.Pci, L.Pval, L.Plong);
-  csr(&P->A, h.m, h.LA, h.Along, L.Arp, L.Aci, L.Aval, L.Along);
-  csr(&P->At, h.n, h.LT, h.Tlong, L.Trp, L.Tci, L.Tval, L.Tlong);
+  csr(&P->P, h.n, h.GP, L.gP, L.Prp, L.Pci, L.Pval);
+  csr(&P->A, h.m, h.GA, L.gA, L.Arp, L.Aci, L.Aval);
+  csr(&P->At, h.n, h.GT, L.gT, L.Trp, L.Tci, L.Tval);
   P->Pdiag = (int*)(dev + L.Pdiag);
   P->ctype = (int*)(dev + L.ctype);
 #define DPTR(f) P->f = (double*)(dev + L.f)
@@ -800,62 +866,27 @@ OSQPInt osqp_batch_resolve(OSQPBatch* b, OSQPInt id, const OSQPFloat* q_new) {
   p->qnew = q_new ? 1 : 0;
   p->state = kActive;
   p->t_req = now_s();
-  E->buckets[p->bucket].active.push_back(p->id);
+  // joins its bucket at the next launch (an epoch may be in flight)
+  E->resumed.push_back(p->id);
   E->n_active++;
   return 0;
 }
 
-OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
-  if (completed) *completed = 0;
-  if (!b) return OSQP_MEM_ALLOC_ERROR;
-  Engine* E = &b->e;
-  if (E->failed) return OSQP_MEM_ALLOC_ERROR;
-  E->last_done.clear();
-  // admission in arrival order while slots and memory allow
-  size_t kept = 0;
-  for (size_t k = 0; k < E->queue.size(); ++k) {
-    HProb* p = E->probs[E->queue[k]];
-    if (E->n_active < E->max_active && admit(E, p)) {
-      p->state = kActive;
-      E->buckets[p->bucket].active.push_back(p->id);
-      E->n_active++;
-    } else {
-      E->queue[kept++] = E->queue[k];
-    }
-  }
-  E->queue.resize(kept);
-  // launch every bucket with work
-  bool launched[kBuckets] = {false};
-  for (int k = 0; k < kBuckets; ++k) {
-    Bucket& B = E->buckets[k];
-    if (B.active.empty()) continue;
-    if (!ensure_table(E, B, B.active.size())) return OSQP_MEM_ALLOC_ERROR;
-    B.tab->count = (int)B.active.size();
-    B.tab->epoch = ++B.epoch;
-    B.tab->iters = kEpochIters;
-    for (size_t i = 0; i < B.active.size(); ++i) {
-      HProb* p = E->probs[B.active[i]];
-      BEntry& en = B.tab->e[i];
-      en.dev = (BProb*)(E->arena + p->dev_off);
-      en.init = p->cmd == 1 ? (const BProb*)(E->pool + p->stage_off) : nullptr;
-      en.cmd = p->cmd;
-      en.iters_done = 0;
-      en.upd_q = nullptr;
-      if (p->cmd == 2 && p->qnew) en.upd_q = (const double*)(E->pool + p->res_off + p->L.rs_q);
-    }
-    const int c = B.trial < kNumCands ? B.trial : B.best;
-    const int T = B.fixT ? B.fixT : kCands[c].T, tpc = B.fixT ? B.fixTpc : kCands[c].tpc;
-    if (!ensure_graph(E, B, T, tpc)) return OSQP_MEM_ALLOC_ERROR;
-    if (!ok(E, hipEventRecord(B.e0, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
-    if (!ok(E, hipGraphLaunch(B.exec, E->stream), "graph launch")) return OSQP_MEM_ALLOC_ERROR;
-    if (!ok(E, hipEventRecord(B.e1, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
-    launched[k] = true;
-  }
-  if (!ok(E, hipStreamSynchronize(E->stream), "epoch")) return OSQP_MEM_ALLOC_ERROR;
+}  // extern "C"
+
+namespace osqp_serve {
+namespace {
+
+// Ends the epoch in flight, if any: its results, the tuner, the counters.
+// Returns the number of solves that ended in it.
+int finish_epoch(Engine* E) {
+  if (!E->inflight) return 0;
+  E->inflight = false;
+  if (!ok(E, hipStreamSynchronize(E->stream), "epoch")) return -1;
   const double t_now = now_s();
   int done = 0;
   for (int k = 0; k < kBuckets; ++k) {
-    if (!launched[k]) continue;
+    if (!E->launched[k]) continue;
     Bucket& B = E->buckets[k];
     float ms = 0.f;
     hipEventElapsedTime(&ms, B.e0, B.e1);
@@ -902,6 +933,76 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
     }
   }
   E->epochs++;
+  return done;
+}
+
+}  // namespace
+}  // namespace osqp_serve
+
+extern "C" {
+
+// One step of the pipelined engine: queued problems are admitted (their
+// memory reserved and images written) while the epoch launched by the
+// previous step still runs, then that epoch is finished, then the next one is
+// launched and the call returns without waiting for it.  So the host work
+// between two calls (adds, polls, resolves, the caller's own) overlaps the GPU.
+OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
+  if (completed) *completed = 0;
+  if (!b) return OSQP_MEM_ALLOC_ERROR;
+  Engine* E = &b->e;
+  if (E->failed) return OSQP_MEM_ALLOC_ERROR;
+  E->last_done.clear();
+  // admission in arrival order while slots and memory allow (the tables of
+  // the epoch in flight are not touched)
+  std::vector<int> fresh;
+  size_t kept = 0;
+  for (size_t k = 0; k < E->queue.size(); ++k) {
+    HProb* p = E->probs[E->queue[k]];
+    if (E->n_active + (int)fresh.size() < E->max_active && admit(E, p)) {
+      fresh.push_back(p->id);
+    } else {
+      E->queue[kept++] = E->queue[k];
+    }
+  }
+  E->queue.resize(kept);
+  const int done = finish_epoch(E);
+  if (done < 0) return OSQP_MEM_ALLOC_ERROR;
+  for (int id : fresh) {
+    HProb* p = E->probs[id];
+    p->state = kActive;
+    E->buckets[p->bucket].active.push_back(id);
+    E->n_active++;
+  }
+  for (int id : E->resumed) E->buckets[E->probs[id]->bucket].active.push_back(id);
+  E->resumed.clear();
+  // launch every bucket with work
+  for (int k = 0; k < kBuckets; ++k) {
+    E->launched[k] = false;
+    Bucket& B = E->buckets[k];
+    if (B.active.empty()) continue;
+    if (!ensure_table(E, B, B.active.size())) return OSQP_MEM_ALLOC_ERROR;
+    B.tab->count = (int)B.active.size();
+    B.tab->epoch = ++B.epoch;
+    B.tab->iters = kEpochIters;
+    for (size_t i = 0; i < B.active.size(); ++i) {
+      HProb* p = E->probs[B.active[i]];
+      BEntry& en = B.tab->e[i];
+      en.dev = (BProb*)(E->arena + p->dev_off);
+      en.init = p->cmd == 1 ? (const BProb*)(E->pool + p->stage_off) : nullptr;
+      en.cmd = p->cmd;
+      en.iters_done = 0;
+      en.upd_q = nullptr;
+      if (p->cmd == 2 && p->qnew) en.upd_q = (const double*)(E->pool + p->res_off + p->L.rs_q);
+    }
+    const int c = B.trial < kNumCands ? B.trial : B.best;
+    const int T = B.fixT ? B.fixT : kCands[c].T, tpc = B.fixT ? B.fixTpc : kCands[c].tpc;
+    if (!ensure_graph(E, B, T, tpc)) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipEventRecord(B.e0, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipGraphLaunch(B.exec, E->stream), "graph launch")) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipEventRecord(B.e1, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
+    E->launched[k] = true;
+    E->inflight = true;
+  }
   if (completed) *completed = done;
   return 0;
 }

@@ -10,9 +10,12 @@
 //
 // Results do not depend on the team size (threads per workgroup) or on the
 // other problems in flight:
-//   * a CSR row is reduced by one group of L lanes (L fixed per matrix from its
-//     row lengths) or, for rows longer than 64, by one wavefront, in a fixed
-//     order, whatever group or wavefront takes the row;
+//   * a matrix's rows of at most 64 entries are packed, in row order, into
+//     fixed groups of 64 lanes x 8 entries (no row crosses a group); a group is
+//     reduced by one wavefront in a fixed order (each lane adds its 8 entries
+//     in sequence, a fixed-shape scan joins the rows that cross lanes),
+//     whatever wavefront takes it; longer rows are reduced by one wavefront
+//     each, also in a fixed order;
 //   * a sum / max over a vector is formed over fixed 256-element blocks (each
 //     block by one wavefront: 4 elements per lane in index order, then a fixed
 //     butterfly) and the block results are added in block order.
@@ -32,6 +35,8 @@ using osqp_hip::kWave;
 constexpr int kRedBlock = 256;     // elements per reduction block
 constexpr int kMaxRedBlocks = 512; // vectors up to 131072 entries
 constexpr int kLongRow = 64;       // rows longer than this go to whole wavefronts
+constexpr int kGroupPer = 8;       // entries per lane of a row group
+constexpr int kGroupSlots = kWave * kGroupPer;  // entries per row group
 constexpr int kMaxTeamWaves = 16;
 
 // phases of a resident problem
@@ -61,15 +66,24 @@ constexpr double kRhoEqOverIneq = 1e3;
 constexpr double kZeroDeadzone = 1e-15;
 constexpr double kCgTolMin = 1e-7;
 
-// CSR matrix with its lane-group size and long-row list
+// CSR matrix with its row groups (rows of 1..kLongRow entries), long-row and
+// empty-row lists.  Slot (g, lane, e) of the groups sits at
+// g * kGroupSlots + e * kWave + lane and holds the group's entry lane * 8 + e
+// (row order, then column order), or padding (gk = grow = -1) after the
+// group's last entry.
 struct BCsr {
   int rows;
-  int L;        // lanes per short row (power of two, 1..64)
+  int ng;       // row groups
   int nlong;    // rows longer than kLongRow
+  int nempty;   // rows without entries
   int* rp;
   int* ci;
   double* val;
   int* longs;
+  int* empty;
+  int* gk;      // ng * kGroupSlots: CSR position of the slot's entry, -1 padding
+  int* grow;    // its row, -1 padding
+  int* gci;     // its column (0 for padding)
 };
 
 // scalar state that survives between epochs (stored with the problem)
@@ -206,37 +220,109 @@ __device__ __forceinline__ double team_dot(const Team& t, int n, F f) {
   return s;
 }
 
-// y-rows of a CSR product: for each row r, epi(r, sum_k elem(val_k, col_k))
-// with the row reduced by L lanes (short rows) or a wavefront (long rows).
-// The matrix is taken by value so its pointers live in registers.  Caller syncs.
+// y-rows of a CSR product: for each row r, epi(r, sum_k elem(val_k, col_k)),
+// epi called exactly once per row by one lane.  Row groups go to wavefronts
+// (one group per wavefront at a time):
+//   1. each lane loads its 8 slots, forms the products and adds them in order,
+//      row by row: a row wholly inside the lane is finished there; the lane's
+//      first row (head) and last row (tail) may continue across lanes;
+//   2. C_l, the part of lane l's head row from lanes before l, follows
+//      C_l = cont_l ? ts_{l-1} + (single_{l-1} ? C_{l-1} : 0) : 0 (ts: tail sum,
+//      single: the lane holds one row), an affine recurrence solved by a
+//      6-step shuffle scan of fixed shape;
+//   3. the lane where a row ends adds C_l to its part and runs the epilogue.
+// Long rows take one wavefront each; empty rows get epi(r, 0).  The matrix is
+// taken by value so its pointers live in registers.  Caller syncs.
 template <typename TX, typename Elem, typename Epi>
 __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem, Epi epi) {
+  const int* __restrict__ gk = M.gk;
+  const int* __restrict__ grow = M.grow;
+  const int* __restrict__ gci = M.gci;
+  const double* __restrict__ val = M.val;
+  const int lane = t.lane;
+  for (int g = t.wv; g < M.ng; g += t.W) {
+    const int base = g * kGroupSlots + lane;
+    int rr[kGroupPer], kk[kGroupPer], cc[kGroupPer];
+#pragma unroll
+    for (int e = 0; e < kGroupPer; ++e) {
+      rr[e] = grow[base + e * kWave];
+      kk[e] = gk[base + e * kWave];
+      cc[e] = gci[base + e * kWave];
+    }
+    TX v[kGroupPer];
+#pragma unroll
+    for (int e = 0; e < kGroupPer; ++e) v[e] = kk[e] >= 0 ? (TX)elem(val[kk[e]], cc[e]) : (TX)0;
+    const int hr = rr[0];
+    int tr = rr[0];
+    TX hs = 0, ts = 0;
+    bool multi = false;
+#pragma unroll
+    for (int e = 0; e < kGroupPer; ++e) {
+      const int r = rr[e];
+      if (r >= 0) {
+        if (r != tr) {
+          if (multi) epi(tr, ts);  // a row wholly inside this lane
+          else hs = ts;            // the head row's part
+          multi = true;
+          tr = r;
+          ts = 0;
+        }
+        ts += v[e];
+      }
+    }
+    if (!multi) hs = ts;
+    // lane l's head continues lane l-1's tail
+    const int prev_tr = __shfl_up(tr, 1, kWave);
+    const TX prev_ts = __shfl_up(ts, 1, kWave);
+    const int prev_multi = __shfl_up((int)multi, 1, kWave);
+    const bool cont = lane > 0 && hr >= 0 && hr == prev_tr;
+    int a = (cont && !prev_multi) ? 1 : 0;
+    TX b = cont ? prev_ts : (TX)0;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const TX bp = __shfl_up(b, d, kWave);
+      const int ap = __shfl_up(a, d, kWave);
+      if (lane >= d) {
+        if (a) b = b + bp;
+        a = a & ap;
+      }
+    }
+    const int next_cont = __shfl_down((int)cont, 1, kWave);
+    const bool ends_here = lane == kWave - 1 || !next_cont;
+    if (hr >= 0) {
+      if (multi) {
+        epi(hr, b + hs);
+        if (ends_here) epi(tr, ts);
+      } else if (ends_here) {
+        epi(hr, b + ts);
+      }
+    }
+  }
+  for (int j = t.tid; j < M.nempty; j += t.T) epi(M.empty[j], (TX)0);
+  // long rows: lane l adds entries s + l, s + l + 64, ... in order; the loads
+  // of 8 consecutive strides are issued before their products are added
   const int* __restrict__ rp = M.rp;
   const int* __restrict__ ci = M.ci;
-  const double* __restrict__ val = M.val;
-  const int L = M.L, rows = M.rows;
-  const int G = t.T / L;
-  const int g = t.tid / L, gl = t.tid % L;
-  for (int r0 = 0; r0 < rows; r0 += G) {
-    const int r = r0 + g;
-    const bool own = r < rows;
-    int s = 0, e = 0;
-    if (own) {
-      s = rp[r];
-      e = rp[r + 1];
-    }
-    const bool lng = (e - s) > kLongRow;
-    TX acc = 0;
-    if (own && !lng)
-      for (int k = s + gl; k < e; k += L) acc += elem(val[k], ci[k]);
-    for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
-    if (own && !lng && gl == 0) epi(r, acc);
-  }
   for (int j = t.wv; j < M.nlong; j += t.W) {
     const int r = M.longs[j];
     const int s = rp[r], e = rp[r + 1];
     TX acc = 0;
-    for (int k = s + t.lane; k < e; k += kWave) acc += elem(val[k], ci[k]);
+    int k = s + t.lane;
+    for (; k + (kGroupPer - 1) * kWave < e; k += kGroupPer * kWave) {
+      double vv[kGroupPer];
+      int cc[kGroupPer];
+#pragma unroll
+      for (int i = 0; i < kGroupPer; ++i) {
+        vv[i] = val[k + i * kWave];
+        cc[i] = ci[k + i * kWave];
+      }
+      TX pr[kGroupPer];
+#pragma unroll
+      for (int i = 0; i < kGroupPer; ++i) pr[i] = (TX)elem(vv[i], cc[i]);
+#pragma unroll
+      for (int i = 0; i < kGroupPer; ++i) acc += pr[i];
+    }
+    for (; k < e; k += kWave) acc += elem(val[k], ci[k]);
     acc = osqp_hip::wave_sum(acc);
     if (t.lane == 0) epi(r, acc);
   }

@@ -83,9 +83,7 @@ class BatchEngine:
     # ---- requests -------------------------------------------------------------
     def add(self, P, q, A, l, u, x0=None, y0=None, resident=False):
         Ph, Ah = T.CscHolder(P), T.CscHolder(A)
-        q = T.as_f64(q)
-        l = np.clip(T.as_f64(l), -T.OSQP_INFTY, T.OSQP_INFTY)
-        u = np.clip(T.as_f64(u), -T.OSQP_INFTY, T.OSQP_INFTY)
+        q, l, u = T.as_f64(q), T.as_f64(l), T.as_f64(u)  # l, u are clamped to +-OSQP_INFTY by the engine
         x0, y0 = T.as_f64(x0), T.as_f64(y0)
         pid = C.c_int(0)
         rc = self.lib.osqp_batch_add(self._b, Ph.ptr, T.dptr(q), Ah.ptr, T.dptr(l), T.dptr(u), T.dptr(x0),
