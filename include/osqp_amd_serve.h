@@ -13,7 +13,10 @@
  *   - osqp_batch_step runs one epoch: every active problem advances up to a
  *     fixed number of ADMM iterations in one persistent kernel per size bucket
  *     (one hipGraph each); a workgroup that finishes a problem takes the next
- *     one in the same launch; problems added meanwhile join the next epoch;
+ *     one in the same launch; problems added meanwhile join the next epoch.
+ *     Steps are pipelined: a step launches its epoch and returns, and the next
+ *     step finishes it, so the caller's work between steps overlaps the GPU
+ *     and a solve is reported by the step after the epoch it ended in;
  *   - a resident problem keeps its iterates and rho after a solve; a new q
  *     (osqp_update_data_vec semantics) and a warm-started re-solve are
  *     requested with osqp_batch_resolve.
@@ -72,8 +75,11 @@ OSQPInt osqp_batch_add(OSQPBatch* batch, const OSQPCscMatrix* P, const OSQPFloat
  * -1 when the problem is unknown, not resident or busy. */
 OSQPInt osqp_batch_resolve(OSQPBatch* batch, OSQPInt id, const OSQPFloat* q_new);
 
-/* One epoch; *completed (may be null) = solves that ended in it.  Returns 0,
- * or OSQP_MEM_ALLOC_ERROR when a HIP call failed. */
+/* Admits queued problems, finishes the epoch launched by the previous step
+ * and launches the next one without waiting for it.  *completed (may be
+ * null) = solves that ended in the finished epoch.  Returns 0, or
+ * OSQP_MEM_ALLOC_ERROR when a HIP call failed.  Loop with
+ * `while (osqp_batch_pending(b)) osqp_batch_step(b, &done);`. */
 OSQPInt osqp_batch_step(OSQPBatch* batch, OSQPInt* completed);
 
 /* Ids of the solves that ended in the last osqp_batch_step (at most cap of
