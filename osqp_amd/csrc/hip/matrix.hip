@@ -36,10 +36,15 @@ inline int egrid(long long n) { return grid_for(n, kBlock, 16384); }
 __global__ void k_scale(double* v, double sc, long long n) { GRID_STRIDE(i, n) v[i] *= sc; }
 
 // per-entry row index of a CSR (expanded once per call; rows are short)
+// val[k] *= L[r] over the entries of row r; one wavefront per row, so the
+// row's entries are read and written in coalesced strides
 __global__ void k_lmult_csr(double* val, const int* rp, const double* L, int rows) {
-  GRID_STRIDE(r, rows) {
-    double d = L[r];
-    for (int k = rp[r]; k < rp[r + 1]; ++k) val[k] *= d;
+  const int lane = threadIdx.x & (kWave - 1);
+  const long long nw = (long long)gridDim.x * blockDim.x / kWave;
+  for (long long r = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < rows; r += nw) {
+    const double d = L[r];
+    const int e = rp[r + 1];
+    for (int k = rp[r] + lane; k < e; k += kWave) val[k] *= d;
   }
 }
 
@@ -472,7 +477,9 @@ void OSQPMatrix_lmult_diag(OSQPMatrix* A, const OSQPVectorf* L) {
   if (A->is_triu) {
     hipLaunchKernelGGL(k_lmult_sym, dim3(egrid(A->n)), dim3(kBlock), 0, S(), A->val, A->rp, A->ci, L->d, A->n);
   } else {
-    if (A->m > 0) hipLaunchKernelGGL(k_lmult_csr, dim3(egrid(A->m)), dim3(kBlock), 0, S(), A->val, A->rp, L->d, A->m);
+    if (A->m > 0)
+      hipLaunchKernelGGL(k_lmult_csr, dim3(grid_for((long long)A->m * kWave, kBlock, 16384)), dim3(kBlock), 0, S(),
+                         A->val, A->rp, L->d, A->m);
     if (A->nnz_in > 0)
       hipLaunchKernelGGL(k_rmult_csr, dim3(egrid(A->nnz_in)), dim3(kBlock), 0, S(), A->tval, A->tci, L->d,
                          (long long)A->nnz_in);
@@ -487,7 +494,8 @@ void OSQPMatrix_rmult_diag(OSQPMatrix* A, const OSQPVectorf* R) {
     if (A->nnz > 0)
       hipLaunchKernelGGL(k_rmult_csr, dim3(egrid(A->nnz)), dim3(kBlock), 0, S(), A->val, A->ci, R->d, (long long)A->nnz);
     if (A->n > 0)
-      hipLaunchKernelGGL(k_lmult_csr, dim3(egrid(A->n)), dim3(kBlock), 0, S(), A->tval, A->trp, R->d, A->n);
+      hipLaunchKernelGGL(k_lmult_csr, dim3(grid_for((long long)A->n * kWave, kBlock, 16384)), dim3(kBlock), 0, S(),
+                         A->tval, A->trp, R->d, A->n);
   }
   touched(A);
 }

@@ -168,6 +168,59 @@ def test_resident_resolve_follows_update_data_vec():
         assert abs(a.info.iter - b.info.iter) <= max(100, 0.25 * b.info.iter)
 
 
+def test_pipelined_steps_with_resolve_in_flight():
+    """Steps are pipelined: a solve is reported by the step after its epoch.
+    A resident problem re-solved while another problem's epoch is in flight
+    joins the next launch; both results equal the ones of running alone."""
+    short, long_ = pr.mpc(12, seed=21), pr.portfolio(600, seed=22)
+    q2 = short["q"] + 0.05  # a new linear term for the short problem
+    ref = []
+    for p in (short, long_):
+        eng = BatchEngine(precision=32, **ST)
+        ref.append(run_all(eng, [p])[0])
+        eng.close()
+    eng = BatchEngine(precision=32, **ST)
+    ref_short2 = None
+    a = eng.add(short["P"], short["q"], short["A"], short["l"], short["u"], resident=True)
+    b = eng.add(long_["P"], long_["q"], long_["A"], long_["l"], long_["u"])
+    assert eng.step() == []  # the first step only launches
+    got_a = got_b = None
+    resolved = False
+    for _ in range(100000):
+        if not eng.pending():
+            break
+        for i in eng.step():
+            if i == a and not resolved:
+                got_a = eng.poll(a)
+                eng.resolve(a, q2)  # the long problem's epoch is in flight
+                resolved = True
+            elif i == a:
+                ref_short2 = eng.poll(a)
+            elif i == b:
+                got_b = eng.poll(b)
+    eng.close()
+    assert resolved and ref_short2 is not None and got_b is not None
+    same(got_a, ref[0])
+    same(got_b, ref[1])
+    # the re-solve equals a resident re-solve in an engine of its own
+    eng = BatchEngine(precision=32, **ST)
+    a2 = eng.add(short["P"], short["q"], short["A"], short["l"], short["u"], resident=True)
+    run_until_done(eng)
+    eng.poll(a2)
+    eng.resolve(a2, q2)
+    run_until_done(eng)
+    alone2 = eng.poll(a2)
+    eng.close()
+    same(ref_short2, alone2)
+
+
+def run_until_done(eng):
+    for _ in range(100000):
+        if not eng.pending():
+            return
+        eng.step()
+
+
 def test_admission_by_slots_and_memory():
     probs = [pr.mpc(N, seed=N) for N in range(10, 30)]
     eng = BatchEngine(precision=32, max_active=3, arena_bytes=8 << 20, **ST)
