@@ -70,7 +70,7 @@ def serve_mpc(n_req, seed):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    ids, arr_of, lat, status, iters = [], {}, [], {}, []
+    ids, arr_of, lat, status, iters, objs = [], {}, [], {}, [], {}
     nxt = 0
     pending = set()
     t_log = time.perf_counter()
@@ -96,6 +96,7 @@ def serve_mpc(n_req, seed):
             k, a = arr_of[pid]
             lat.append(t_seen - a)
             status[k] = r.info.status_val
+            objs[k] = r.info.obj_val
             iters.append(r.info.iter)
             pending.discard(pid)
     elapsed = time.perf_counter() - t_start
@@ -106,6 +107,7 @@ def serve_mpc(n_req, seed):
     # plain FP64 engine, one request at a time (setup + solve), on a sample
     k_plain = 24
     same = 0
+    plain = []
     ts = time.perf_counter()
     for k in range(k_plain):
         p = reqs[k]
@@ -113,8 +115,10 @@ def serve_mpc(n_req, seed):
         assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
         r = s.solve()
         same += int(r.info.status_val == status[k])
+        plain.append(r)
         s.close()
     plain_qps = k_plain / (time.perf_counter() - ts)
+    obj_err = max(abs(objs[k] - plain[k].info.obj_val) / max(1.0, abs(plain[k].info.obj_val)) for k in range(k_plain))
     log(f"[serve] batched FP32: {qps:.0f} QPs/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
         f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms; plain FP64: {plain_qps:.1f} QPs/s; "
         f"statuses identical on {same}/{k_plain}")
@@ -126,6 +130,7 @@ def serve_mpc(n_req, seed):
             "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
             "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
             "plain_sample": k_plain, "statuses_identical_on_sample": same,
+            "max_obj_rel_err_vs_fp64_on_sample": obj_err,
             "engine": {"epochs": cnt["epochs"], "gpu_s": cnt["gpu_s"], "tuned_team_per_bucket": tuning}}
 
 
@@ -165,7 +170,7 @@ def serve_portfolio(n_problems, n_req, seed):
     for j, (k, q) in enumerate(reqs):
         queue[k].append(j)
     t_start = time.perf_counter()
-    issued, lat, status, busy = {}, [], {}, {}
+    issued, lat, status, busy, objs = {}, [], {}, {}, {}
     for k in range(n_problems):
         if queue[k]:
             j = queue[k].pop(0)
@@ -185,6 +190,7 @@ def serve_portfolio(n_problems, n_req, seed):
             j, t_iss = issued.pop(i)
             lat.append(time.perf_counter() - t_iss)
             status[j] = r.info.status_val
+            objs[j] = r.info.obj_val
             k = pid.index(i)
             if queue[k]:
                 jn = queue[k].pop(0)
@@ -207,12 +213,14 @@ def serve_portfolio(n_problems, n_req, seed):
             s.solve()
             solvers[k] = s
     same = 0
+    obj_err = 0.0
     ts = time.perf_counter()
     for j in sample:
         k, q = reqs[j]
         solvers[k].update(q=q)
         r = solvers[k].solve()
         same += int(r.info.status_val == status[j])
+        obj_err = max(obj_err, abs(objs[j] - r.info.obj_val) / max(1.0, abs(r.info.obj_val)))
     plain_qps = k_plain / (time.perf_counter() - ts)
     for s in solvers.values():
         s.close()
@@ -224,7 +232,39 @@ def serve_portfolio(n_problems, n_req, seed):
                                        "p99": float(np.percentile(lat, 99) * 1e3)},
             "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
             "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
-            "plain_sample": k_plain, "statuses_identical_on_sample": same}
+            "plain_sample": k_plain, "statuses_identical_on_sample": same,
+            "max_obj_rel_err_vs_fp64_on_sample": obj_err}
+
+
+def lasso_section(seed):
+    """The lasso configuration at full size (n = 20000, m_d = 200000, D of
+    density 1e-3): one cold solve with the FP32-PCG engine and one with the
+    plain FP64 engine, the error of the first against the second, and the FP64
+    KKT certificate of the FP32 point."""
+    from osqp_amd import OSQP
+    from osqp_amd import problems as pr
+    t0 = time.time()
+    p = pr.lasso(20000, 200000, 1e-3, seed=seed)
+    log(f"[lasso] generated n={p['P'].shape[0]} m={p['A'].shape[0]} nnz(A)={p['A'].nnz} in {time.time() - t0:.1f}s")
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, verbose=0)
+    out = {}
+    for prec in (32, 64):
+        s = OSQP(precision=prec)
+        assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
+        ts = time.perf_counter()
+        r = s.solve()
+        out[prec] = (r, time.perf_counter() - ts)
+        s.close()
+    (r32, t32), (r64, t64) = out[32], out[64]
+    c = pr.kkt_certificate(p, r32.x, r32.y)
+    log(f"[lasso] FP32-PCG {t32 * 1e3:.0f} ms ({r32.info.status}, {r32.info.iter} it); FP64 {t64 * 1e3:.0f} ms")
+    return {"workload": f"lasso n=20000 m_d=200000 density 1e-3 (QP n={p['P'].shape[0]} m={p['A'].shape[0]} "
+                        f"nnz(A)={p['A'].nnz}), eps 1e-4, cold solve",
+            "fp32_pcg": {"qps": 1.0 / t32, "status": r32.info.status, "iter": r32.info.iter},
+            "fp64_plain": {"qps": 1.0 / t64, "status": r64.info.status, "iter": r64.info.iter},
+            "err_vs_fp64": {"x_inf": float(np.max(np.abs(r32.x - r64.x))), "y_inf": float(np.max(np.abs(r32.y - r64.y))),
+                            "obj_rel": abs(r32.info.obj_val - r64.info.obj_val) / max(1.0, abs(r64.info.obj_val))},
+            "kkt_certificate": {"prim": c["prim"], "dual": c["dual"]}}
 
 
 def main():
@@ -371,6 +411,7 @@ def main():
         log(f"[bench] cpu oracle: {k_cpu} ADMM iterations in {tc:.1f}s")
 
     # ---- served MPC stream: batched FP32-PCG engine vs plain FP64 engine -------
+    lasso = None if args.no_serving else lasso_section(args.seed)
     serving = None if args.no_serving else serve_mpc(args.mpc_requests, args.seed)
     serving_pf = None if args.no_serving else serve_portfolio(64, 512, args.seed)
 
@@ -408,6 +449,7 @@ def main():
                      "fp64_residual_kernels": fp64_kernels},
         "gpu_fp64_plain": fp64,
         "cpu_baseline": cpu,
+        "lasso": lasso,
         "serving_mpc": serving,
         "serving_portfolio": serving_pf,
     }
