@@ -75,6 +75,7 @@ __device__ void run_entry(const Team& t, BProb& P, const BEntry& E, int iters, i
 template <typename V>
 __global__ __launch_bounds__(1024) void k_epoch(BTable* tab, int* counters) {
   __shared__ double red[kMaxRedBlocks];
+  __shared__ double red2[kMaxRedBlocks];
   __shared__ BProb P;
   __shared__ int slot, done_iters;
   Team t;
@@ -84,6 +85,7 @@ __global__ __launch_bounds__(1024) void k_epoch(BTable* tab, int* counters) {
   t.wv = threadIdx.x / kWave;
   t.W = blockDim.x / kWave;
   t.red = red;
+  t.red2 = red2;
   const int ep = tab->epoch;
   const int count = tab->count;
   const int iters = tab->iters;

@@ -148,7 +148,8 @@ struct BProb {
 // ---- team primitives -------------------------------------------------------
 struct Team {
   int tid, T, lane, wv, W;
-  double* red;  // LDS, kMaxRedBlocks doubles
+  double* red;   // LDS, kMaxRedBlocks doubles
+  double* red2;  // LDS, kMaxRedBlocks doubles (a second reduction in the same pass)
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
 
@@ -218,6 +219,46 @@ __device__ __forceinline__ double team_dot(const Team& t, int n, F f) {
   for (int b = 0; b < nb; ++b) s += t.red[b];
   t.sync();
   return s;
+}
+
+// One pass over the same fixed 256-element blocks as team_dot and
+// team_reduce: body(i) runs once for every element (it may update it) and
+// returns the pair (dot term, max term); the result is (team_dot of the
+// first, team_reduce<true> of the second) with the bits of the two separate
+// reductions.  Four barriers fewer than body loop + team_dot + team_reduce.
+template <typename V, typename F>
+__device__ __forceinline__ void team_dot_max(const Team& t, int n, F body, double* dot, double* mx) {
+  const int nb = (n + kRedBlock - 1) / kRedBlock;
+  for (int b = t.wv; b < nb; b += t.W) {
+    KahanAcc<V> acc;
+    double m = 0.0;
+#pragma unroll
+    for (int j = 0; j < kRedBlock / kWave; ++j) {
+      const int i = b * kRedBlock + j * kWave + t.lane;
+      if (i < n) {
+        V a;
+        double c;
+        body(i, a, c);
+        acc.add(a);
+        m = dmax(m, c);
+      }
+    }
+    const double v = osqp_hip::wave_sum(acc.value());
+    m = osqp_hip::wave_max(m);
+    if (t.lane == 0) {
+      t.red[b] = v;
+      t.red2[b] = m;
+    }
+  }
+  t.sync();
+  double s = 0.0, x = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    s += t.red[b];
+    x = dmax(x, t.red2[b]);
+  }
+  t.sync();
+  *dot = s;
+  *mx = x;
 }
 
 // y-rows of a CSR product: for each row r, epi(r, sum_k elem(val_k, col_k)),
@@ -580,18 +621,21 @@ struct Solver {
     const double scale = (rn > 0.0 && isfinite(rn)) ? rn : 1.0;
     const double eps_cmp = eps / scale;
     const double inv = 1.0 / scale;
-    // CG on K d = -r64 / scale
-    for (int i = t.tid; i < n; i += t.T) {
-      const V ri = (V)(r64[i] * inv);
-      const V yi = dinv[i] * ri;
-      r[i] = ri;
-      y[i] = yi;
-      pp[i] = -yi;
-      d[i] = (V)0;
-    }
-    t.sync();
-    double rty = team_dot<V>(t, n, [=](int i) { return y[i] * r[i]; });
-    double rnorm = team_reduce<true>(t, n, [=](int i) { return fabs((double)r[i]); });
+    // CG on K d = -r64 / scale (init fused with r'y and max|r|)
+    double rty, rnorm;
+    team_dot_max<V>(
+        t, n,
+        [=](int i, V& a, double& c) {
+          const V ri = (V)(r64[i] * inv);
+          const V yi = dinv[i] * ri;
+          r[i] = ri;
+          y[i] = yi;
+          pp[i] = -yi;
+          d[i] = (V)0;
+          a = yi * ri;
+          c = fabs((double)ri);
+        },
+        &rty, &rnorm);
     int it = 0;
     bool go = rnorm > eps_cmp && it < cg_max;
     if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
@@ -615,15 +659,19 @@ struct Solver {
       const double pkp = team_dot<V>(t, n, [=](int i) { return pp[i] * Kp[i]; });
       if (guard && !(pkp > 0.0 && isfinite(pkp))) break;
       const V alpha = (V)(rty / pkp);
-      for (int i = t.tid; i < n; i += t.T) {
-        d[i] += alpha * pp[i];
-        const V ri = r[i] + alpha * Kp[i];
-        r[i] = ri;
-        y[i] = dinv[i] * ri;
-      }
-      t.sync();
-      const double rty_new = team_dot<V>(t, n, [=](int i) { return y[i] * r[i]; });
-      rnorm = team_reduce<true>(t, n, [=](int i) { return fabs((double)r[i]); });
+      double rty_new;
+      team_dot_max<V>(
+          t, n,
+          [=](int i, V& a, double& c) {
+            d[i] += alpha * pp[i];
+            const V ri = r[i] + alpha * Kp[i];
+            const V yi = dinv[i] * ri;
+            r[i] = ri;
+            y[i] = yi;
+            a = yi * ri;
+            c = fabs((double)ri);
+          },
+          &rty_new, &rnorm);
       const double beta = rty_new / rty;
       it++;
       rty = rty_new;
