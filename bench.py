@@ -236,6 +236,41 @@ def serve_portfolio(n_problems, n_req, seed):
             "max_obj_rel_err_vs_fp64_on_sample": obj_err}
 
 
+def random_qp_small_section(seed):
+    """The random-QP configuration that runs on the CPU today (n = 500,
+    m = 5000, density 0.15, P = M M' + 1e-2 I, default settings, eps 1e-4):
+    GPU solves with both engines, and the CPU oracle's direct (LDL') path --
+    the reference's default -- timed on one core; errors against the oracle."""
+    from oracle import OracleSolver
+    from osqp_amd import OSQP
+    from osqp_amd import problems as pr
+    p = pr.random_qp(500, 5000, 0.15, 0.15, seed=seed)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4, verbose=0)
+    o = OracleSolver()
+    ts = time.perf_counter()
+    assert o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=1, **st) == 0
+    o.solve()
+    t_cpu = time.perf_counter() - ts
+    xo, yo, fo = np.array(o.x), np.array(o.y), o.info.obj_val
+    out = {"workload": f"random QP n=500 m=5000 density 0.15 (nnz(A)={p['A'].nnz}), eps 1e-4, setup + cold solve",
+           "cpu_oracle_direct": {"qps": 1.0 / t_cpu, "cores": 1, "status": o.info.status.decode()
+                                 if isinstance(o.info.status, bytes) else str(o.info.status), "iter": o.info.iter}}
+    for prec in (32, 64):
+        s = OSQP(precision=prec)
+        ts = time.perf_counter()
+        assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
+        r = s.solve()
+        el = time.perf_counter() - ts
+        s.close()
+        out["gpu_fp32_pcg" if prec == 32 else "gpu_fp64_plain"] = {
+            "qps": 1.0 / el, "status": r.info.status, "iter": r.info.iter,
+            "err_vs_oracle": {"x_inf": float(np.max(np.abs(r.x - xo))), "y_inf": float(np.max(np.abs(r.y - yo))),
+                              "obj_rel": abs(r.info.obj_val - fo) / max(1.0, abs(fo))}}
+    log(f"[random-qp-small] CPU oracle {t_cpu * 1e3:.0f} ms; GPU FP32 {1e3 / out['gpu_fp32_pcg']['qps']:.0f} ms, "
+        f"obj err {out['gpu_fp32_pcg']['err_vs_oracle']['obj_rel']:.1e}")
+    return out
+
+
 def lasso_section(seed):
     """The lasso configuration at full size (n = 20000, m_d = 200000, D of
     density 1e-3): one cold solve with the FP32-PCG engine and one with the
@@ -411,6 +446,7 @@ def main():
         log(f"[bench] cpu oracle: {k_cpu} ADMM iterations in {tc:.1f}s")
 
     # ---- served MPC stream: batched FP32-PCG engine vs plain FP64 engine -------
+    small = None if args.no_serving else random_qp_small_section(args.seed)
     lasso = None if args.no_serving else lasso_section(args.seed)
     serving = None if args.no_serving else serve_mpc(args.mpc_requests, args.seed)
     serving_pf = None if args.no_serving else serve_portfolio(64, 512, args.seed)
@@ -449,6 +485,7 @@ def main():
                      "fp64_residual_kernels": fp64_kernels},
         "gpu_fp64_plain": fp64,
         "cpu_baseline": cpu,
+        "random_qp_small": small,
         "lasso": lasso,
         "serving_mpc": serving,
         "serving_portfolio": serving_pf,

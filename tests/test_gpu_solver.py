@@ -232,6 +232,27 @@ def test_random_qp_vs_oracle(seed, precision):
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
+def test_random_qp_config_full_size(precision):
+    """The random-QP configuration at full size (n = 500, m = 5000, density
+    0.15, P = M M' + 1e-2 I; default settings, eps 1e-4) against the oracle's
+    direct (LDL') path, the reference's default: same status, objective within
+    1e-4 relative, and the GPU point passes the FP64 KKT certificate."""
+    p = pr.random_qp(500, 5000, 0.15, 0.15, seed=0)
+    st = dict(eps_abs=1e-4, eps_rel=1e-4)
+    o = OracleSolver()
+    fx = dict(eps_abs=1e-4, eps_rel=1e-4, verbose=0)
+    assert o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=1, **fx) == 0
+    o.solve()
+    s, r = gpu_solve(p, precision, rho=0.1, alpha=1.6, max_iter=4000, scaling=10, **st)
+    assert r.info.status_val == o.info.status_val == 1
+    assert abs(r.info.obj_val - o.info.obj_val) <= 1e-4 * abs(o.info.obj_val)
+    c = pr.kkt_certificate(p, r.x, r.y)
+    assert c["prim"] <= 10 * (1e-4 + 1e-4 * c["scale_p"])
+    assert c["dual"] <= 10 * (1e-4 + 1e-4 * c["scale_d"])
+    s.close()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
 def test_multi_panel_qp(precision):
     """A QP wide enough for the column-panelled SpMV to use several panels in
     both precisions: A is 30000 x 9000 with 30 nonzeros per row, K = [P A']
