@@ -37,7 +37,7 @@ constexpr int kMaxRedBlocks = 512; // vectors up to 131072 entries
 constexpr int kLongRow = 64;       // rows longer than this go to whole wavefronts
 constexpr int kGroupPer = 8;       // entries per lane of a row group
 constexpr int kGroupSlots = kWave * kGroupPer;  // entries per row group
-constexpr int kMaxTeamWaves = 16;
+constexpr int kMaxTeamWaves = 8;   // teams of up to 512 threads
 
 // phases of a resident problem
 enum : int { kPhaseIdle = 0, kPhaseSetup = 1, kPhaseUpdate = 2, kPhaseRun = 3, kPhaseDone = 4 };
