@@ -80,6 +80,7 @@ template <typename V>
 __global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters) {
   __shared__ double red[kMaxRedBlocks];
   __shared__ double red2[kMaxRedBlocks];
+  extern __shared__ double dyn_lds[];
   __shared__ BProb P;
   __shared__ int slot, done_iters;
   Team t;
@@ -90,6 +91,8 @@ __global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters) {
   t.W = blockDim.x / kWave;
   t.red = red;
   t.red2 = red2;
+  t.lds = (char*)dyn_lds;
+  t.lds_bytes = kTeamLdsBytes;
   const int ep = tab->epoch;
   const int count = tab->count;
   const int iters = tab->iters;
@@ -337,7 +340,7 @@ struct Layout {
   size_t s_begin, s_end;
   size_t Prp, Pci, Pval, Pdiag, Arp, Aci, Aval, Trp, Tci, Tval, q, l, u;
   struct Groups {
-    size_t longs, empty, gk, grow, gci;
+    size_t longs, empty, gk, grow, gci, gval;
   } gP, gA, gT;
   // dynamic
   size_t ctype, D, Dinv, E, Einv, rho, rhoinv, xa, xb, za, zb, y, xt, zt, dx, dy, Ax, Px, Aty, tn, tm, tn2;
@@ -386,6 +389,9 @@ Layout make_layout(const HostData& h, int vbytes) {
   L.u = take(8 * m);
   o = align_up(o, 64);
   L.s_end = o;
+  L.gP.gval = take(8 * h.GP.gk.size());
+  L.gA.gval = take(8 * h.GA.gk.size());
+  L.gT.gval = take(8 * h.GT.gk.size());
   L.ctype = take(4 * m);
   L.D = take(8 * n);
   L.Dinv = take(8 * n);
@@ -626,6 +632,7 @@ bool admit(Engine* E, HProb* p) {
     M->gk = (int*)(dev + o.gk);
     M->grow = (int*)(dev + o.grow);
     M->gci = (int*)(dev + o.gci);
+    M->gval = (double*)(dev + o.gval);
   };
   csr(&P->P, h.n, h.GP, L.gP, L.Prp, L.Pci, L.Pval);
   csr(&P->A, h.m, h.GA, L.gA, L.Arp, L.Aci, L.Aval);
@@ -718,9 +725,9 @@ bool ensure_graph(Engine* E, Bucket& B, int T, int tpc) {
   if (!ok(E, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
   const dim3 grid((unsigned)(tpc * E->cus)), block((unsigned)T);
   if (E->bits == 32)
-    hipLaunchKernelGGL(k_epoch<float>, grid, block, 0, E->stream, B.tab, B.counters);
+    hipLaunchKernelGGL(k_epoch<float>, grid, block, kTeamLdsBytes, E->stream, B.tab, B.counters);
   else
-    hipLaunchKernelGGL(k_epoch<double>, grid, block, 0, E->stream, B.tab, B.counters);
+    hipLaunchKernelGGL(k_epoch<double>, grid, block, kTeamLdsBytes, E->stream, B.tab, B.counters);
   if (!ok(E, hipStreamEndCapture(E->stream, &B.graph), "end capture")) return false;
   if (!ok(E, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "graph instantiate")) return false;
   B.g_T = T;

@@ -38,6 +38,7 @@ constexpr int kLongRow = 64;       // rows longer than this go to whole wavefron
 constexpr int kGroupPer = 8;       // entries per lane of a row group
 constexpr int kGroupSlots = kWave * kGroupPer;  // entries per row group
 constexpr int kMaxTeamWaves = 8;   // teams of up to 512 threads
+constexpr int kTeamLdsBytes = 32768;  // dynamic LDS per team: CG vectors of problems with 6n + m small
 
 // phases of a resident problem
 enum : int { kPhaseIdle = 0, kPhaseSetup = 1, kPhaseUpdate = 2, kPhaseRun = 3, kPhaseDone = 4 };
@@ -84,6 +85,7 @@ struct BCsr {
   int* gk;      // ng * kGroupSlots: CSR position of the slot's entry, -1 padding
   int* grow;    // its row, -1 padding
   int* gci;     // its column (0 for padding)
+  double* gval; // its value, copied from val after scaling (0 for padding)
 };
 
 // scalar state that survives between epochs (stored with the problem)
@@ -148,6 +150,8 @@ struct BProb {
 // ---- team primitives -------------------------------------------------------
 struct Team {
   int tid, T, lane, wv, W;
+  char* lds;       // dynamic LDS of the workgroup (small problems' CG vectors)
+  int lds_bytes;
   double* red;   // LDS, kMaxRedBlocks doubles
   double* red2;  // LDS, kMaxRedBlocks doubles (a second reduction in the same pass)
   __device__ __forceinline__ void sync() const { __syncthreads(); }
@@ -276,23 +280,24 @@ __device__ __forceinline__ void team_dot_max(const Team& t, int n, F body, doubl
 // taken by value so its pointers live in registers.  Caller syncs.
 template <typename TX, typename Elem, typename Epi>
 __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem, Epi epi) {
-  const int* __restrict__ gk = M.gk;
   const int* __restrict__ grow = M.grow;
   const int* __restrict__ gci = M.gci;
+  const double* __restrict__ gval = M.gval;
   const double* __restrict__ val = M.val;
   const int lane = t.lane;
   for (int g = t.wv; g < M.ng; g += t.W) {
     const int base = g * kGroupSlots + lane;
-    int rr[kGroupPer], kk[kGroupPer], cc[kGroupPer];
+    int rr[kGroupPer], cc[kGroupPer];
+    double vv[kGroupPer];
 #pragma unroll
     for (int e = 0; e < kGroupPer; ++e) {
       rr[e] = grow[base + e * kWave];
-      kk[e] = gk[base + e * kWave];
       cc[e] = gci[base + e * kWave];
+      vv[e] = gval[base + e * kWave];
     }
     TX v[kGroupPer];
 #pragma unroll
-    for (int e = 0; e < kGroupPer; ++e) v[e] = kk[e] >= 0 ? (TX)elem(val[kk[e]], cc[e]) : (TX)0;
+    for (int e = 0; e < kGroupPer; ++e) v[e] = rr[e] >= 0 ? (TX)elem(vv[e], cc[e]) : (TX)0;
     const int hr = rr[0];
     int tr = rr[0];
     TX hs = 0, ts = 0;
@@ -541,6 +546,20 @@ struct Solver {
     update_precond();
   }
 
+  // the row groups' values in slot order, from the (scaled) CSR values
+  __device__ __forceinline__ void fill_group_values() {
+    const BCsr* Ms[3] = {&p.P, &p.A, &p.At};
+    for (int k = 0; k < 3; ++k) {
+      const BCsr& M = *Ms[k];
+      const int* __restrict__ gk = M.gk;
+      const double* __restrict__ val = M.val;
+      double* __restrict__ gval = M.gval;
+      const int slots = M.ng * kGroupSlots;
+      for (int i = t.tid; i < slots; i += t.T) gval[i] = gk[i] >= 0 ? val[gk[i]] : 0.0;
+    }
+    t.sync();
+  }
+
   // Jacobi preconditioner: 1 / (sigma + diag(P) + sum_j rho_j A_ji^2)
   __device__ __forceinline__ void update_precond() {
     V* __restrict__ dinv = (V*)p.cdinv;
@@ -567,13 +586,21 @@ struct Solver {
     const int n = p.n, m = p.m;
     const double sg = p.set.sigma;
     const int cg_max = p.set.cg_max_iter;
-    V* __restrict__ d = (V*)p.cd;
-    V* __restrict__ pp = (V*)p.cp;
-    V* __restrict__ Kp = (V*)p.cKp;
-    V* __restrict__ r = (V*)p.cr;
-    V* __restrict__ y = (V*)p.cy;
-    V* __restrict__ w = (V*)p.cw;
+    // a small problem keeps its CG vectors in the workgroup's dynamic LDS
+    // (d, p, Kp, r, y, M^-1 over n; w over m); a larger one in its arena block
+    V* lv = (size_t)(6 * n + m) * sizeof(V) <= (size_t)t.lds_bytes ? (V*)t.lds : nullptr;
+    V* __restrict__ d = lv ? lv : (V*)p.cd;
+    V* __restrict__ pp = lv ? lv + n : (V*)p.cp;
+    V* __restrict__ Kp = lv ? lv + 2 * n : (V*)p.cKp;
+    V* __restrict__ r = lv ? lv + 3 * n : (V*)p.cr;
+    V* __restrict__ y = lv ? lv + 4 * n : (V*)p.cy;
+    V* __restrict__ w = lv ? lv + 6 * n : (V*)p.cw;
     const V* __restrict__ dinv = (const V*)p.cdinv;
+    if (lv) {
+      V* dl = lv + 5 * n;
+      for (int i = t.tid; i < n; i += t.T) dl[i] = dinv[i];
+      dinv = dl;  // made visible by the barriers before its first use
+    }
     const double* __restrict__ rho = p.rho;
     double* __restrict__ x64 = p.x64;
     double* __restrict__ r64 = p.r64;
@@ -1028,6 +1055,7 @@ struct Solver {
     S.rho = p.set.rho0;
     if (p.set.scaling) scale_data();
     else no_scaling();
+    fill_group_values();
     set_rho_vec();
     update_precond();
     S.swapped = 0;

@@ -2,9 +2,10 @@
 
     eng = BatchEngine(precision=32, eps_abs=1e-4, eps_rel=1e-4)
     ids = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"]) for p in problems]
+    results = {}
     while eng.pending():
-        eng.step()                     # one epoch on the GPU
-    results = [eng.poll(i) for i in ids]
+        for i in eng.step():           # one epoch on the GPU (pipelined)
+            results[i] = eng.poll(i)   # polling frees the solve's memory
 
 Settings are OSQP's (keyword arguments as in OSQP.setup).  All work happens in
 libosqp_amd.so; this module marshals arrays only.
@@ -158,10 +159,10 @@ class BatchEngine:
         for _ in range(max_epochs):
             if not self.pending():
                 break
-            self.step()
-        for pid in ids:
-            out[pid] = self.poll(pid)
-        return [out[i] for i in ids]
+            # a finished solve holds its memory until polled: poll as they end
+            for pid in self.step():
+                out[pid] = self.poll(pid)
+        return [out.get(i) for i in ids]
 
     def close(self):
         if self._b:

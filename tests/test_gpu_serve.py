@@ -51,11 +51,13 @@ def mixed_problems():
 
 def run_all(eng, probs, resident=False):
     ids = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=resident) for p in probs]
+    got = {}
     for _ in range(100000):
         if not eng.pending():
             break
-        eng.step()
-    return [eng.poll(i) for i in ids]
+        for i in eng.step():
+            got[i] = eng.poll(i)
+    return [got.get(i) for i in ids]
 
 
 @pytest.mark.parametrize("precision", [64, 32])
@@ -227,13 +229,17 @@ def test_admission_by_slots_and_memory():
     ids = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"]) for p in probs[:10]]
     eng.step()
     assert 0 < eng.pending() <= 10
-    # requests arriving while others run are admitted as capacity frees up
+    # requests arriving while others run are admitted as capacity frees up;
+    # a finished solve keeps its memory until it is polled, so the loop polls
+    # what each step reports
     ids += [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"]) for p in probs[10:]]
+    got = {}
     for _ in range(100000):
         if not eng.pending():
             break
-        eng.step()
-    res = [eng.poll(i) for i in ids]
+        for i in eng.step():
+            got[i] = eng.poll(i)
+    res = [got.get(i) for i in ids]
     eng.close()
     for p, r in zip(probs, res):
         assert r is not None
