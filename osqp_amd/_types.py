@@ -54,7 +54,8 @@ OSQP_NAN = float(0x7FC00000)
 class OSQPCscMatrix(C.Structure):
     _fields_ = [
         ("m", OSQPInt), ("n", OSQPInt),
-        ("p", C.POINTER(OSQPInt)), ("i", C.POINTER(OSQPInt)), ("x", C.POINTER(OSQPFloat)),
+        # OSQPInt* p, OSQPInt* i, OSQPFloat* x (held as addresses)
+        ("p", C.c_void_p), ("i", C.c_void_p), ("x", C.c_void_p),
         ("nzmax", OSQPInt), ("nz", OSQPInt), ("owned", OSQPInt),
     ]
 
@@ -164,16 +165,17 @@ class CscHolder:
         if self.x.size == 0:
             self.i = np.zeros(1, np.int32)
             self.x = np.zeros(1, np.float64)
-        self.s = OSQPCscMatrix(
-            self.m, self.n,
-            self.p.ctypes.data_as(C.POINTER(OSQPInt)),
-            self.i.ctypes.data_as(C.POINTER(OSQPInt)),
-            self.x.ctypes.data_as(C.POINTER(OSQPFloat)),
-            int(M.nnz), -1, 0)
+        self.s = OSQPCscMatrix(self.m, self.n, addr(self.p), addr(self.i), addr(self.x), int(M.nnz), -1, 0)
 
     @property
     def ptr(self):
         return C.byref(self.s)
+
+
+def addr(a):
+    """Address of a numpy array's data as an int (None for None): the cheap
+    form for arguments declared c_void_p."""
+    return None if a is None else a.__array_interface__["data"][0]
 
 
 def dptr(a):

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "osqp_amd_serve.h"
@@ -177,161 +178,44 @@ class RangeAlloc {
   std::map<size_t, size_t> free_, used_;
 };
 
-// row groups of one CSR matrix (the BCsr layout, built on the host)
-struct HostGroups {
-  int ng = 0;
-  std::vector<int> longs, empty, gk, grow, gci;
+// Sizes of the row groups of one CSR matrix (the BCsr layout).  Rows of
+// 1..kLongRow entries are packed in row order into groups of kGroupSlots
+// entries; a row that does not fit the rest of the current group starts the
+// next one (the rest is padding), so no row crosses a group.
+struct GroupDims {
+  int ng = 0, nlong = 0, nempty = 0;
 };
 
-// Rows of 1..kLongRow entries are packed in row order into groups of
-// kGroupSlots entries; a row that does not fit the rest of the current group
-// starts the next one (the rest is padding), so no row crosses a group.
-void build_groups(const std::vector<int>& rp, const std::vector<int>& ci, int rows, HostGroups* g) {
-  *g = HostGroups();
-  // pass 1: group count
-  int ng = 0, fill = 0;
+GroupDims count_groups(const int* rp, int rows) {
+  GroupDims d;
+  int fill = 0;
   for (int r = 0; r < rows; ++r) {
-    const int len = rp[(size_t)r + 1] - rp[r];
+    const int len = rp[r + 1] - rp[r];
     if (len > kLongRow) {
-      g->longs.push_back(r);
+      d.nlong++;
     } else if (len == 0) {
-      g->empty.push_back(r);
+      d.nempty++;
     } else {
-      if (ng == 0 || fill + len > kGroupSlots) {
-        ng++;
+      if (d.ng == 0 || fill + len > kGroupSlots) {
+        d.ng++;
         fill = 0;
       }
       fill += len;
     }
   }
-  g->ng = ng;
-  const size_t total = (size_t)ng * kGroupSlots;
-  g->gk.assign(total, -1);
-  g->grow.assign(total, -1);
-  g->gci.assign(total, 0);
-  // pass 2: the same packing, writing the slots
-  int grp = -1;
-  fill = 0;
-  for (int r = 0; r < rows; ++r) {
-    const int s = rp[r], e = rp[(size_t)r + 1], len = e - s;
-    if (len > kLongRow || len == 0) continue;
-    if (grp < 0 || fill + len > kGroupSlots) {
-      grp++;
-      fill = 0;
-    }
-    int* gk = g->gk.data() + (size_t)grp * kGroupSlots;
-    int* gr = g->grow.data() + (size_t)grp * kGroupSlots;
-    int* gc = g->gci.data() + (size_t)grp * kGroupSlots;
-    for (int k = s; k < e; ++k) {
-      const int pos = fill++;  // the group's entry number: lane pos / 8, element pos % 8
-      const int slot = (pos % kGroupPer) * kWave + pos / kGroupPer;
-      gk[slot] = k;
-      gr[slot] = r;
-      gc[slot] = ci[(size_t)k];
-    }
-  }
+  return d;
 }
 
-// host copy of a problem in CSR form (values raw, unscaled)
+// Dimensions of a problem and its pinned stage image, built once at add time
+// in its final byte layout (so admission is one copy into the stage).
 struct HostData {
   int n = 0, m = 0;
-  std::vector<int> Prp, Pci, Pdiag, Arp, Aci, Trp, Tci;
-  std::vector<double> Pval, Aval, Tval, q, l, u, x0, y0;
-  HostGroups GP, GA, GT;
+  int nnzP = 0;  // the full symmetric P
+  int nnzA = 0;
+  GroupDims gP, gA, gT;
+  bool warm_x = false, warm_y = false;
+  std::unique_ptr<char[]> img;  // stage bytes [st_static, st_total): static image, x0, y0
 };
-
-// A (CSC, m x n) -> A CSR and A' CSR (= the CSC arrays); P (CSC upper
-// triangle) -> full symmetric CSR with sorted columns and diagonal positions
-void build_host(HostData* h, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
-                const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0) {
-  const int n = P->n, m = A->m;
-  h->n = n;
-  h->m = m;
-  // A' = CSC of A
-  const int nnzA = A->p[n];
-  h->Trp.assign(A->p, A->p + n + 1);
-  h->Tci.assign(A->i, A->i + nnzA);
-  h->Tval.assign(A->x, A->x + nnzA);
-  // A CSR (counting sort by row; columns ascending)
-  h->Arp.assign((size_t)m + 1, 0);
-  for (int k = 0; k < nnzA; ++k) h->Arp[(size_t)A->i[k] + 1]++;
-  for (int r = 0; r < m; ++r) h->Arp[(size_t)r + 1] += h->Arp[r];
-  h->Aci.resize((size_t)nnzA);
-  h->Aval.resize((size_t)nnzA);
-  {
-    std::vector<int> next(h->Arp.begin(), h->Arp.end() - 1);
-    for (int j = 0; j < n; ++j)
-      for (int k = A->p[j]; k < A->p[j + 1]; ++k) {
-        const int pos = next[A->i[k]]++;
-        h->Aci[pos] = j;
-        h->Aval[pos] = A->x[k];
-      }
-  }
-  // P full: row r = column r of the triangle (rows i <= r) then the entries
-  // (r, c) of the triangle's columns c > r, ascending
-  const int nnzPt = P->p[n];
-  std::vector<int> cnt((size_t)n, 0);
-  for (int c = 0; c < n; ++c)
-    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
-      const int r = P->i[k];
-      cnt[c]++;
-      if (r != c) cnt[r]++;
-    }
-  h->Prp.assign((size_t)n + 1, 0);
-  for (int r = 0; r < n; ++r) h->Prp[(size_t)r + 1] = h->Prp[r] + cnt[r];
-  const int nnzP = h->Prp[n];
-  h->Pci.resize((size_t)nnzP);
-  h->Pval.resize((size_t)nnzP);
-  h->Pdiag.assign((size_t)n, -1);
-  std::vector<int> next(h->Prp.begin(), h->Prp.end() - 1);
-  for (int c = 0; c < n; ++c)  // lower part of row c: column c of the triangle (entries (i, c), i <= c)
-    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
-      const int pos = next[c]++;
-      h->Pci[pos] = P->i[k];
-      h->Pval[pos] = P->x[k];
-      if (P->i[k] == c) h->Pdiag[c] = pos;
-    }
-  for (int c = 0; c < n; ++c)  // upper part of row r: (r, c) for c > r
-    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
-      const int r = P->i[k];
-      if (r == c) continue;
-      const int pos = next[r]++;
-      h->Pci[pos] = c;
-      h->Pval[pos] = P->x[k];
-    }
-  // a triangle column's rows are not required to be sorted: sort each row
-  for (int r = 0; r < n; ++r) {
-    const int s = h->Prp[r], e = h->Prp[(size_t)r + 1];
-    bool sorted = true;
-    for (int k = s + 1; k < e && sorted; ++k) sorted = h->Pci[k - 1] < h->Pci[k];
-    if (!sorted) {
-      std::vector<std::pair<int, double>> row;
-      for (int k = s; k < e; ++k) row.emplace_back(h->Pci[k], h->Pval[k]);
-      std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
-        return a.first < b.first;
-      });
-      for (int k = s; k < e; ++k) {
-        h->Pci[k] = row[k - s].first;
-        h->Pval[k] = row[k - s].second;
-        if (h->Pci[k] == r) h->Pdiag[r] = k;
-      }
-    }
-  }
-  (void)nnzPt;
-  h->q.assign(q, q + n);
-  // bounds beyond +-OSQP_INFTY are clamped to it, as osqp_setup does
-  h->l.resize((size_t)m);
-  h->u.resize((size_t)m);
-  for (int i = 0; i < m; ++i) {
-    h->l[(size_t)i] = l[i] < -kInfty ? -kInfty : (l[i] > kInfty ? kInfty : l[i]);
-    h->u[(size_t)i] = u[i] < -kInfty ? -kInfty : (u[i] > kInfty ? kInfty : u[i]);
-  }
-  if (x0) h->x0.assign(x0, x0 + n);
-  if (y0) h->y0.assign(y0, y0 + m);
-  build_groups(h->Prp, h->Pci, n, &h->GP);
-  build_groups(h->Arp, h->Aci, m, &h->GA);
-  build_groups(h->Trp, h->Tci, n, &h->GT);
-}
 
 // byte layout of a problem's HBM block and of its pinned images
 struct Layout {
@@ -365,33 +249,34 @@ Layout make_layout(const HostData& h, int vbytes) {
   o = align_up(o, 64);
   L.s_begin = o;
   L.Prp = take(4 * (n + 1));
-  L.Pci = take(4 * h.Pci.size());
-  L.Pval = take(8 * h.Pval.size());
+  L.Pci = take(4 * (size_t)h.nnzP);
+  L.Pval = take(8 * (size_t)h.nnzP);
   L.Pdiag = take(4 * n);
   L.Arp = take(4 * (m + 1));
-  L.Aci = take(4 * h.Aci.size());
-  L.Aval = take(8 * h.Aval.size());
+  L.Aci = take(4 * (size_t)h.nnzA);
+  L.Aval = take(8 * (size_t)h.nnzA);
   L.Trp = take(4 * (n + 1));
-  L.Tci = take(4 * h.Tci.size());
-  L.Tval = take(8 * h.Tval.size());
-  auto groups = [&](Layout::Groups* o, const HostGroups& g) {
-    o->longs = take(4 * g.longs.size());
-    o->empty = take(4 * g.empty.size());
-    o->gk = take(4 * g.gk.size());
-    o->grow = take(4 * g.grow.size());
-    o->gci = take(4 * g.gci.size());
+  L.Tci = take(4 * (size_t)h.nnzA);
+  L.Tval = take(8 * (size_t)h.nnzA);
+  auto groups = [&](Layout::Groups* o, const GroupDims& g) {
+    const size_t slots = (size_t)g.ng * kGroupSlots;
+    o->longs = take(4 * (size_t)g.nlong);
+    o->empty = take(4 * (size_t)g.nempty);
+    o->gk = take(4 * slots);
+    o->grow = take(4 * slots);
+    o->gci = take(4 * slots);
   };
-  groups(&L.gP, h.GP);
-  groups(&L.gA, h.GA);
-  groups(&L.gT, h.GT);
+  groups(&L.gP, h.gP);
+  groups(&L.gA, h.gA);
+  groups(&L.gT, h.gT);
   L.q = take(8 * n);
   L.l = take(8 * m);
   L.u = take(8 * m);
   o = align_up(o, 64);
   L.s_end = o;
-  L.gP.gval = take(8 * h.GP.gk.size());
-  L.gA.gval = take(8 * h.GA.gk.size());
-  L.gT.gval = take(8 * h.GT.gk.size());
+  L.gP.gval = take(8 * (size_t)h.gP.ng * kGroupSlots);
+  L.gA.gval = take(8 * (size_t)h.gA.ng * kGroupSlots);
+  L.gT.gval = take(8 * (size_t)h.gT.ng * kGroupSlots);
   L.ctype = take(4 * m);
   L.D = take(8 * n);
   L.Dinv = take(8 * n);
@@ -434,10 +319,10 @@ Layout make_layout(const HostData& h, int vbytes) {
   s += L.s_end - L.s_begin;
   s = align_up(s, 64);
   L.st_x0 = s;
-  s += h.x0.empty() ? 0 : 8 * n;
+  s += h.warm_x ? 8 * n : 0;
   s = align_up(s, 64);
   L.st_y0 = s;
-  s += h.y0.empty() ? 0 : 8 * m;
+  s += h.warm_y ? 8 * m : 0;
   L.st_total = align_up(s, 256);
   // pinned results: [x][y][BResult][q buffer]
   size_t r = 0;
@@ -450,6 +335,159 @@ Layout make_layout(const HostData& h, int vbytes) {
   L.rs_q = r;
   r = align_up(r + 8 * (n ? n : 1), 64);
   L.rs_total = align_up(r, 256);
+  return L;
+}
+
+// Pass 2 of the packing: the long and empty row lists and the slot arrays
+// (a group's entry number e sits in lane e / kGroupPer, element e % kGroupPer;
+// slot = element * kWave + lane), written into the image img whose first
+// byte is block offset base.
+void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, const Layout::Groups& o,
+                  char* img, size_t base) {
+  int* longs = (int*)(img + (o.longs - base));
+  int* empty = (int*)(img + (o.empty - base));
+  int* gk0 = (int*)(img + (o.gk - base));
+  int* gr0 = (int*)(img + (o.grow - base));
+  int* gc0 = (int*)(img + (o.gci - base));
+  const size_t slots = (size_t)d.ng * kGroupSlots;
+  std::fill(gk0, gk0 + slots, -1);
+  std::fill(gr0, gr0 + slots, -1);
+  std::fill(gc0, gc0 + slots, 0);
+  int nl = 0, ne = 0, grp = -1, fill = 0;
+  for (int r = 0; r < rows; ++r) {
+    const int s = rp[r], e = rp[r + 1], len = e - s;
+    if (len > kLongRow) {
+      longs[nl++] = r;
+      continue;
+    }
+    if (len == 0) {
+      empty[ne++] = r;
+      continue;
+    }
+    if (grp < 0 || fill + len > kGroupSlots) {
+      grp++;
+      fill = 0;
+    }
+    int* gk = gk0 + (size_t)grp * kGroupSlots;
+    int* gr = gr0 + (size_t)grp * kGroupSlots;
+    int* gc = gc0 + (size_t)grp * kGroupSlots;
+    for (int k = s; k < e; ++k) {
+      const int pos = fill++;
+      const int slot = (pos % kGroupPer) * kWave + pos / kGroupPer;
+      gk[slot] = k;
+      gr[slot] = r;
+      gc[slot] = ci[k];
+    }
+  }
+}
+
+// Builds a problem's stage image (values raw, unscaled): A (CSC, m x n) ->
+// A CSR and A' CSR (= the CSC arrays); P (CSC upper triangle) -> the full
+// symmetric CSR with sorted columns and diagonal positions; bounds clamped to
+// +-OSQP_INFTY as osqp_setup does; the row groups of all three; the warm start.
+Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
+                  const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0) {
+  const int n = P->n, m = A->m;
+  h->n = n;
+  h->m = m;
+  h->nnzA = A->p[n];
+  // row pointers first: their sizes fix the layout (per-thread scratch keeps
+  // a stream of adds free of allocations)
+  thread_local std::vector<int> arp, prp, next;
+  arp.assign((size_t)m + 1, 0);
+  for (int k = 0; k < h->nnzA; ++k) arp[(size_t)A->i[k] + 1]++;
+  for (int r = 0; r < m; ++r) arp[(size_t)r + 1] += arp[r];
+  prp.assign((size_t)n + 1, 0);
+  for (int c = 0; c < n; ++c)
+    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
+      const int r = P->i[k];
+      prp[(size_t)c + 1]++;
+      if (r != c) prp[(size_t)r + 1]++;
+    }
+  for (int r = 0; r < n; ++r) prp[(size_t)r + 1] += prp[r];
+  h->nnzP = prp[n];
+  h->gP = count_groups(prp.data(), n);
+  h->gA = count_groups(arp.data(), m);
+  h->gT = count_groups(A->p, n);
+  h->warm_x = x0 != nullptr;
+  h->warm_y = y0 != nullptr;
+  const Layout L = make_layout(*h, vbytes);
+  h->img.reset(new char[L.st_total - L.st_static]);
+  char* img = h->img.get();
+  const size_t base = L.s_begin;
+  auto at = [&](size_t off) { return img + (off - base); };
+  const size_t nnzA = (size_t)h->nnzA, nnzP = (size_t)h->nnzP;
+  // A' = the CSC arrays
+  memcpy(at(L.Trp), A->p, 4 * ((size_t)n + 1));
+  memcpy(at(L.Tci), A->i, 4 * nnzA);
+  memcpy(at(L.Tval), A->x, 8 * nnzA);
+  // A CSR by a counting sort on rows (columns ascending)
+  int* Arp = (int*)at(L.Arp);
+  int* Aci = (int*)at(L.Aci);
+  double* Aval = (double*)at(L.Aval);
+  memcpy(Arp, arp.data(), 4 * ((size_t)m + 1));
+  next.assign(arp.begin(), arp.end() - 1);
+  for (int j = 0; j < n; ++j)
+    for (int k = A->p[j]; k < A->p[j + 1]; ++k) {
+      const int pos = next[A->i[k]]++;
+      Aci[pos] = j;
+      Aval[pos] = A->x[k];
+    }
+  // P full: row r = column r of the triangle (rows i <= r) then the entries
+  // (r, c) of the triangle's columns c > r, ascending
+  int* Prp = (int*)at(L.Prp);
+  int* Pci = (int*)at(L.Pci);
+  double* Pval = (double*)at(L.Pval);
+  int* Pdiag = (int*)at(L.Pdiag);
+  memcpy(Prp, prp.data(), 4 * ((size_t)n + 1));
+  std::fill(Pdiag, Pdiag + n, -1);
+  next.assign(prp.begin(), prp.end() - 1);
+  for (int c = 0; c < n; ++c)  // lower part of row c: column c of the triangle (entries (i, c), i <= c)
+    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
+      const int pos = next[c]++;
+      Pci[pos] = P->i[k];
+      Pval[pos] = P->x[k];
+      if (P->i[k] == c) Pdiag[c] = pos;
+    }
+  for (int c = 0; c < n; ++c)  // upper part of row r: (r, c) for c > r
+    for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
+      const int r = P->i[k];
+      if (r == c) continue;
+      const int pos = next[r]++;
+      Pci[pos] = c;
+      Pval[pos] = P->x[k];
+    }
+  // a triangle column's rows are not required to be sorted: sort each row
+  for (int r = 0; r < n; ++r) {
+    const int s = Prp[r], e = Prp[r + 1];
+    bool sorted = true;
+    for (int k = s + 1; k < e && sorted; ++k) sorted = Pci[k - 1] < Pci[k];
+    if (!sorted) {
+      std::vector<std::pair<int, double>> row;
+      for (int k = s; k < e; ++k) row.emplace_back(Pci[k], Pval[k]);
+      std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+        return a.first < b.first;
+      });
+      for (int k = s; k < e; ++k) {
+        Pci[k] = row[k - s].first;
+        Pval[k] = row[k - s].second;
+        if (Pci[k] == r) Pdiag[r] = k;
+      }
+    }
+  }
+  (void)nnzP;
+  memcpy(at(L.q), q, 8 * (size_t)n);
+  double* lo = (double*)at(L.l);
+  double* up = (double*)at(L.u);
+  for (int i = 0; i < m; ++i) {
+    lo[i] = l[i] < -kInfty ? -kInfty : (l[i] > kInfty ? kInfty : l[i]);
+    up[i] = u[i] < -kInfty ? -kInfty : (u[i] > kInfty ? kInfty : u[i]);
+  }
+  write_groups(Prp, Pci, n, h->gP, L.gP, img, base);
+  write_groups(Arp, Aci, m, h->gA, L.gA, img, base);
+  write_groups(A->p, A->i, n, h->gT, L.gT, img, base);
+  if (x0) memcpy(img + (L.st_x0 - L.st_static), x0, 8 * (size_t)n);
+  if (y0) memcpy(img + (L.st_y0 - L.st_static), y0, 8 * (size_t)m);
   return L;
 }
 
@@ -470,7 +508,7 @@ constexpr int kBuckets = 6;
 constexpr int kEpochIters = 25;
 
 int bucket_of(const HostData& h) {
-  const long long w = (long long)h.n + h.m + (long long)h.Aci.size() + (long long)h.Pci.size();
+  const long long w = (long long)h.n + h.m + (long long)h.nnzA + (long long)h.nnzP;
   if (w <= (1 << 12)) return 0;
   if (w <= (1 << 14)) return 1;
   if (w <= (1 << 16)) return 2;
@@ -554,14 +592,9 @@ void free_problem_memory(Engine* E, HProb* p, bool dev, bool stage, bool res) {
   }
 }
 
-template <typename T>
-void put(char* base, size_t off, const std::vector<T>& v) {
-  if (!v.empty()) memcpy(base + off, v.data(), sizeof(T) * v.size());
-}
-
 // reserve memory and write the pinned images of a queued problem
 bool admit(Engine* E, HProb* p) {
-  const HostData& h = p->h;
+  HostData& h = p->h;
   const Layout& L = p->L;
   size_t d, s, r;
   if (!E->arena_alloc.alloc(L.total, &d)) return false;
@@ -581,35 +614,10 @@ bool admit(Engine* E, HProb* p) {
   char* dev = E->arena + d;
   char* st = E->pool + s;
   char* rs = E->pool + r;
-  // static image, laid out as in the HBM block (offsets from s_begin)
+  // static image (laid out as in the HBM block) and warm start, built at add
   char* img = st + L.st_static;
-  const size_t b = L.s_begin;
-  memset(img, 0, L.s_end - L.s_begin);
-  put(img, L.Prp - b, h.Prp);
-  put(img, L.Pci - b, h.Pci);
-  put(img, L.Pval - b, h.Pval);
-  put(img, L.Pdiag - b, h.Pdiag);
-  put(img, L.Arp - b, h.Arp);
-  put(img, L.Aci - b, h.Aci);
-  put(img, L.Aval - b, h.Aval);
-  put(img, L.Trp - b, h.Trp);
-  put(img, L.Tci - b, h.Tci);
-  put(img, L.Tval - b, h.Tval);
-  auto put_groups = [&](const Layout::Groups& o, const HostGroups& g) {
-    put(img, o.longs - b, g.longs);
-    put(img, o.empty - b, g.empty);
-    put(img, o.gk - b, g.gk);
-    put(img, o.grow - b, g.grow);
-    put(img, o.gci - b, g.gci);
-  };
-  put_groups(L.gP, h.GP);
-  put_groups(L.gA, h.GA);
-  put_groups(L.gT, h.GT);
-  put(img, L.q - b, h.q);
-  put(img, L.l - b, h.l);
-  put(img, L.u - b, h.u);
-  if (!h.x0.empty()) put(st, L.st_x0, h.x0);
-  if (!h.y0.empty()) put(st, L.st_y0, h.y0);
+  memcpy(img, h.img.get(), L.st_total - L.st_static);
+  h.img.reset();
   // descriptor
   BProb* P = (BProb*)st;
   memset(P, 0, sizeof(BProb));
@@ -618,12 +626,12 @@ bool admit(Engine* E, HProb* p) {
   P->set = E->set;
   P->st.phase = kPhaseSetup;
   P->st.rho = E->set.rho0;
-  auto csr = [&](BCsr* M, int rows, const HostGroups& g, const Layout::Groups& o, size_t rp, size_t ci,
+  auto csr = [&](BCsr* M, int rows, const GroupDims& g, const Layout::Groups& o, size_t rp, size_t ci,
                  size_t val) {
     M->rows = rows;
     M->ng = g.ng;
-    M->nlong = (int)g.longs.size();
-    M->nempty = (int)g.empty.size();
+    M->nlong = g.nlong;
+    M->nempty = g.nempty;
     M->rp = (int*)(dev + rp);
     M->ci = (int*)(dev + ci);
     M->val = (double*)(dev + val);
@@ -634,9 +642,9 @@ bool admit(Engine* E, HProb* p) {
     M->gci = (int*)(dev + o.gci);
     M->gval = (double*)(dev + o.gval);
   };
-  csr(&P->P, h.n, h.GP, L.gP, L.Prp, L.Pci, L.Pval);
-  csr(&P->A, h.m, h.GA, L.gA, L.Arp, L.Aci, L.Aval);
-  csr(&P->At, h.n, h.GT, L.gT, L.Trp, L.Tci, L.Tval);
+  csr(&P->P, h.n, h.gP, L.gP, L.Prp, L.Pci, L.Pval);
+  csr(&P->A, h.m, h.gA, L.gA, L.Arp, L.Aci, L.Aval);
+  csr(&P->At, h.n, h.gT, L.gT, L.Trp, L.Tci, L.Tval);
   P->Pdiag = (int*)(dev + L.Pdiag);
   P->ctype = (int*)(dev + L.ctype);
 #define DPTR(f) P->f = (double*)(dev + L.f)
@@ -682,8 +690,8 @@ bool admit(Engine* E, HProb* p) {
   P->stage_bytes = (long long)(L.s_end - L.s_begin);
   P->arena_static = dev + L.s_begin;
   P->upd_q = nullptr;
-  P->warm_x = h.x0.empty() ? nullptr : (const double*)(st + L.st_x0);
-  P->warm_y = h.y0.empty() ? nullptr : (const double*)(st + L.st_y0);
+  P->warm_x = h.warm_x ? (const double*)(st + L.st_x0) : nullptr;
+  P->warm_y = h.warm_y ? (const double*)(st + L.st_y0) : nullptr;
   P->out_x = (double*)(rs + L.rs_x);
   P->out_y = (double*)(rs + L.rs_y);
   P->out = (BResult*)(rs + L.rs_res);
@@ -844,8 +852,7 @@ OSQPInt osqp_batch_add(OSQPBatch* b, const OSQPCscMatrix* P, const OSQPFloat* q,
   HProb* p = new HProb();
   p->id = E->next_id++;
   p->resident = resident ? 1 : 0;
-  build_host(&p->h, P, q, A, l, u, x0, y0);
-  p->L = make_layout(p->h, E->bits / 8);
+  p->L = build_host(&p->h, E->bits / 8, P, q, A, l, u, x0, y0);
   if (p->L.total > E->arena_size || p->L.st_total + p->L.rs_total > E->pool_size) {
     // could never be admitted: larger than the whole arena or staging pool
     fprintf(stderr, "osqp_batch: problem needs %zu bytes of device arena (of %zu) and %zu of staging (of %zu)\n",
@@ -910,14 +917,8 @@ int finish_epoch(Engine* E) {
       HProb* p = E->probs[B.active[i]];
       its += B.tab->e[i].iters_done;
       p->iters_since_add += B.tab->e[i].iters_done;
-      if (p->cmd == 1) {
-        // setup done: the stage and the host copy are no longer read
-        free_problem_memory(E, p, false, true, false);
-        HostData keep;
-        keep.n = p->h.n;
-        keep.m = p->h.m;
-        p->h = std::move(keep);
-      }
+      // setup done: the stage is no longer read
+      if (p->cmd == 1) free_problem_memory(E, p, false, true, false);
       p->cmd = 0;
       const BResult* out = (const BResult*)(E->pool + p->res_off + p->L.rs_res);
       if (out->done) {

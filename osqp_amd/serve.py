@@ -87,8 +87,8 @@ class BatchEngine:
         q, l, u = T.as_f64(q), T.as_f64(l), T.as_f64(u)  # l, u are clamped to +-OSQP_INFTY by the engine
         x0, y0 = T.as_f64(x0), T.as_f64(y0)
         pid = C.c_int(0)
-        rc = self.lib.osqp_batch_add(self._b, Ph.ptr, T.dptr(q), Ah.ptr, T.dptr(l), T.dptr(u), T.dptr(x0),
-                                     T.dptr(y0), 1 if resident else 0, C.byref(pid))
+        rc = self.lib.osqp_batch_add(self._b, Ph.ptr, T.addr(q), Ah.ptr, T.addr(l), T.addr(u), T.addr(x0),
+                                     T.addr(y0), 1 if resident else 0, C.byref(pid))
         if rc:
             raise ValueError(f"osqp_batch_add rejected the problem ({rc})")
         self._dims[pid.value] = (Ph.n, Ah.m)
@@ -96,7 +96,7 @@ class BatchEngine:
 
     def resolve(self, pid, q=None):
         q = T.as_f64(q)
-        rc = self.lib.osqp_batch_resolve(self._b, int(pid), T.dptr(q))
+        rc = self.lib.osqp_batch_resolve(self._b, int(pid), T.addr(q))
         if rc:
             raise RuntimeError(f"osqp_batch_resolve({pid}) failed: not resident, unknown or busy")
 
@@ -118,7 +118,7 @@ class BatchEngine:
         x = np.empty(n)
         y = np.empty(max(m, 1))
         info = BatchInfo()
-        rc = self.lib.osqp_batch_poll(self._b, int(pid), T.dptr(x), T.dptr(y), C.byref(info))
+        rc = self.lib.osqp_batch_poll(self._b, int(pid), T.addr(x), T.addr(y), C.byref(info))
         if rc < 0:
             raise KeyError(pid)
         if rc == 0:
