@@ -70,6 +70,7 @@ struct PcgState {
   double pKp, alpha, beta;
   int iter;            // CG iterations of the current solve
   int active;          // 1 while the CG loop runs
+  int done;            // 1 once the current solve's CG loop has stopped
   int zero_iters;      // consecutive solves with 0 CG iterations
   int solves;          // solves performed
   long long total_iters;
@@ -251,10 +252,12 @@ __global__ void k_comb_kv(const PcgState* st, int n, int np, const TX* partial, 
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// z = A x (FP64): b[n:n+m] (polishing: rho_pol (A x - b2)) and the A x cache
+// z = A x (FP64): b[n:n+m] (polishing: rho_pol (A x - b2)) and the A x cache;
+// skipped while *when == 0 (when given)
 template <int G>
-__global__ void k_comb_z(int m, int n, int np, const double* partial, double* b, double* ax, int polishing,
-                         double rho_pol) {
+__global__ void k_comb_z(const int* when, int m, int n, int np, const double* partial, double* b, double* ax,
+                         int polishing, double rho_pol) {
+  if (when && !*when) return;
   COMB_ROWS(i, j, m, G) {
     const double s = panel_sum<double, G>(partial, np, m, i, j);
     if (j == 0) {
@@ -297,6 +300,7 @@ __device__ void set_tolerance(PcgState* st, double rhs_norm, int admm_iter, doub
   st->scale = 1.0;
   st->iter = 0;
   st->active = 1;
+  st->done = 0;
   st->solves++;
 }
 
@@ -370,9 +374,10 @@ __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int
   }
 }
 
-// x64 += scale * d ; b[0:n] = x64
+// x64 += scale * d ; b[0:n] = x64 (with when_done: only once the loop has stopped)
 template <typename V>
-__global__ void k_dc_apply(const PcgState* st, int n, const V* d, double* x64, double* b) {
+__global__ void k_dc_apply(const PcgState* st, int when_done, int n, const V* d, double* x64, double* b) {
+  if (when_done && !st->done) return;
   const double s = st->scale;
   const int ran = st->iter > 0;
   GRID_STRIDE(i, n) {
@@ -403,7 +408,10 @@ __global__ void k_loop_first(PcgState* st, const double* part_rty, const double*
                              int guard, int* flags) {
   __shared__ double lds[kBlock / kWave];
   if (!st->active) {
-    if (flags && threadIdx.x == 0) flags[0] = 2 * st->solves;
+    if (threadIdx.x == 0) {
+      st->done = 1;
+      if (flags) flags[0] = 2 * st->solves;
+    }
     return;
   }
   const double rty = parts_sum(part_rty, nparts, lds);
@@ -413,6 +421,7 @@ __global__ void k_loop_first(PcgState* st, const double* part_rty, const double*
   st->rnorm = rn;
   const int go = loop_go(st, rty, rn, 0, max_iter, guard);
   st->active = go;
+  st->done = !go;
   if (flags) flags[0] = 2 * st->solves + go;
 }
 
@@ -431,7 +440,10 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pKp = pkp;
     st->alpha = alpha64;
-    if (broke) st->active = 0;
+    if (broke) {
+      st->active = 0;
+      st->done = 1;
+    }
   }
   if (broke) return;
   const V alpha = (V)alpha64;
@@ -476,6 +488,7 @@ __global__ void k_head_pupdate(PcgState* st, const double* part_rty, const doubl
     st->rty[j & 1] = rty;
     st->rnorm = rn;
     st->active = go;
+    st->done = !go;
     if (flags) flags[j] = 2 * st->solves + go;
   }
   if (!go) return;
@@ -527,7 +540,10 @@ __global__ void k_state_init(PcgState* st, double tol_fraction) {
 }
 
 __global__ void k_set_active(PcgState* st, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st->active = v;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->active = v;
+    st->done = !v;
+  }
 }
 
 // ---- the solver object ----------------------------------------------------------
@@ -555,6 +571,7 @@ struct HipPcg {
   int host_solves;    // solves queued (matches st->solves once they ran)
   int gA, gT, gK;     // lanes per row: A rows, A' rows, K rows
   int lookahead;      // CG iterations queued ahead of the last observed flag
+  int last_cg;        // CG iterations of the previous solve (where its loop stopped)
   // combined K-operator CSR [P A'] + its row blocks: FP64 values for the
   // defect-correction residual, FP32 values for the mixed CG
   int *krp, *kci, *k_rows, k_nb;
@@ -738,6 +755,8 @@ void update_precond(HipPcg* s) {
 // Iteration 0 is always queued.  Short iterations (small operators) run two
 // ahead; a skipped iteration costs six launches, less than waiting for the
 // host.  Long ones use lookahead 0 with one launch queued ahead (solve_impl).
+// The answer is definitive: past the deadline the host waits for the stream
+// and reads the flag then (a flag never written counts as a stop).
 bool host_should_stop(HipPcg* s, int it) {
   if (!s->h_flags || it < (s->lookahead > 0 ? s->lookahead : 1)) return false;
   volatile int* f = (volatile int*)s->h_flags;
@@ -752,10 +771,13 @@ bool host_should_stop(HipPcg* s, int it) {
     if ((k & 255) == 0) {
       clock_gettime(CLOCK_MONOTONIC, &t);
       const double el = (t.tv_sec - t0.tv_sec) + 1e-9 * (t.tv_nsec - t0.tv_nsec);
-      if (el > 5.0) return false;
+      if (el > 5.0) break;
       if (el > 1e-3) sched_yield();
     }
   }
+  HIPCHK(hipStreamSynchronize(S()));
+  const int v = f[it - s->lookahead];
+  return v < want || (v & 1) == 0;
 }
 
 // ---- operators: panel SpMV + combine, or the row-block CSR kernel ----------
@@ -841,26 +863,31 @@ int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* p
   return g;
 }
 
-// b[n:n+m] = A x64 (FP64 values), the A x cache; polishing: rho_pol (A x - b2)
-void op_finish_z(HipPcg* s, double* b) {
+// b[n:n+m] = A x64 (FP64 values), the A x cache; polishing: rho_pol (A x - b2).
+// With `when` (panelled A only) the launches do nothing while *when == 0.
+void op_finish_z(HipPcg* s, double* b, const int* when) {
   const int n = s->n, m = s->m;
   if (const PanelMatrix* P = panels_A<double>(s)) {
-    panel_spmv(*P, nullptr, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
+    panel_spmv(*P, when, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
     const int G = comb_group(P->np, m);
     const dim3 g(grid_for((long long)m * G, kBlock, 8192));
     if (G == 8)
-      hipLaunchKernelGGL(k_comb_z<8>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
-                         s->polishing, s->rho_sc);
+      hipLaunchKernelGGL(k_comb_z<8>, g, dim3(kBlock), 0, S(), when, m, n, P->np, (const double*)s->partA, b,
+                         s->ax64, s->polishing, s->rho_sc);
     else if (G == 4)
-      hipLaunchKernelGGL(k_comb_z<4>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
-                         s->polishing, s->rho_sc);
+      hipLaunchKernelGGL(k_comb_z<4>, g, dim3(kBlock), 0, S(), when, m, n, P->np, (const double*)s->partA, b,
+                         s->ax64, s->polishing, s->rho_sc);
     else if (G == 2)
-      hipLaunchKernelGGL(k_comb_z<2>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
-                         s->polishing, s->rho_sc);
+      hipLaunchKernelGGL(k_comb_z<2>, g, dim3(kBlock), 0, S(), when, m, n, P->np, (const double*)s->partA, b,
+                         s->ax64, s->polishing, s->rho_sc);
     else
-      hipLaunchKernelGGL(k_comb_z<1>, g, dim3(kBlock), 0, S(), m, n, P->np, (const double*)s->partA, b, s->ax64,
-                         s->polishing, s->rho_sc);
+      hipLaunchKernelGGL(k_comb_z<1>, g, dim3(kBlock), 0, S(), when, m, n, P->np, (const double*)s->partA, b,
+                         s->ax64, s->polishing, s->rho_sc);
   } else {
+    if (when) {
+      check(hipErrorInvalidValue, "op_finish_z: a guarded tail needs panels", __FILE__, __LINE__);
+      return;
+    }
     const int g = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
     hipLaunchKernelGGL(k_rb_finish_z<double>, dim3(g), dim3(kBlock), 0, S(), (const int*)s->A->rb_rows, s->A->rb_n,
                        (const int*)s->A->rp, (const int*)s->A->ci, (const double*)s->A->val, (const double*)s->x64,
@@ -919,14 +946,33 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
                      gE, s->max_iter, guard, s->d_flags);
   // one CG iteration: the operator (two SpMVs with their combines), the
   // update, and the next loop head fused with the search-direction update.
-  // Without speculation (lookahead 0) the first launch of iteration it + 1
-  // (A's panel SpMV) is queued before the host waits for iteration it + 1's
-  // go, so the GPU has that SpMV to run while the host reacts; a stop skips
-  // just that one launch.
+  // With lookahead 0 the first launch of iteration it + 1 (A's panel SpMV) is
+  // queued before the host waits for iteration it + 1's go, so the GPU has
+  // that SpMV to run while the host reacts; a stop skips just that launch.
+  //
+  // The solve's tail (x64 += scale d, then z = A x64) may be queued before
+  // the host knows the loop has stopped: when iteration it + 1 is where the
+  // previous solve's loop stopped, the tail goes right behind that look-ahead
+  // SpMV, each launch guarded on the device by the loop's `done` flag.  If the
+  // loop stops there, the GPU runs the tail while the host is still reacting;
+  // if it goes on, the three launches exit at once and the tail is queued
+  // unguarded after the loop.  host_should_stop's answers are definitive, so
+  // the host knows which of the two happened.
   const bool ahead = s->lookahead == 0 && m > 0;
+  const bool spec = ahead && s->h_flags && panels_A<double>(s) != nullptr;
+  const int* done = (const int*)((const char*)s->st + offsetof(PcgState, done));
+  auto tail = [&](bool guarded) {
+    hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                       guarded ? 1 : 0, n, (const V*)d, s->x64, b->d);
+    if (m) op_finish_z(s, b->d, guarded ? done : nullptr);
+  };
+  int spec_at = -1, stop_at = s->max_iter;
   if (ahead) av_spmv<V>(s, (const V*)p, w);
   for (int it = 0; it < s->max_iter; ++it) {
-    if (host_should_stop(s, it)) break;
+    if (host_should_stop(s, it)) {
+      stop_at = it;
+      break;
+    }
     if (ahead) {
       av_finish<V>(s, w);
     } else if (m) {
@@ -937,12 +983,17 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
                        d, r, y, (const V*)p, (const V*)Kp, dinv, part0, part1);
     hipLaunchKernelGGL(k_head_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part0,
                        (const double*)part1, gE, it + 1, s->max_iter, guard, s->d_flags, n, p, (const V*)y);
-    if (ahead && it + 1 < s->max_iter) av_spmv<V>(s, (const V*)p, w);
+    if (ahead && it + 1 < s->max_iter) {
+      av_spmv<V>(s, (const V*)p, w);
+      if (spec && it + 1 == s->last_cg) {
+        tail(true);
+        spec_at = it + 1;
+      }
+    }
   }
-  hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st, n,
-                     (const V*)d, s->x64, b->d);
+  s->last_cg = stop_at;
+  if (spec_at < 0 || spec_at != stop_at) tail(false);
   if (m) {
-    op_finish_z(s, b->d);
     s->ax_valid = 1;
     s->ax_Aver = s->A->version;
   }
