@@ -48,35 +48,20 @@ def measured_traffic(kernel):
     return None
 
 
-def serve_mpc(n_req, seed):
-    """MPC stream (nx=12, nu=4, N ~ U{10..50}) through the batched FP32-PCG
-    engine with Poisson arrivals at a rate above saturation (up to 4096 in
-    flight); latency = result seen by the host - arrival.  The plain FP64
-    engine (osqp_setup + osqp_solve per request, one at a time) on a sample of
-    the same requests is the GPU comparison; statuses are compared on it."""
-    import torch
-    from osqp_amd import OSQP
-    from osqp_amd import problems as pr
-    from osqp_amd.serve import BatchEngine
-    st = dict(eps_abs=1e-4, eps_rel=1e-4)
-    rng = np.random.default_rng(seed + 17)
-    Ns = rng.integers(10, 51, n_req)
-    t0 = time.time()
-    reqs = [pr.mpc(int(N), seed=seed * 100000 + k) for k, N in enumerate(Ns)]
-    log(f"[serve] generated {n_req} MPC requests in {time.time() - t0:.1f}s")
-    eng = BatchEngine(precision=32, max_active=4096, **st)
-    eng.solve_batch(reqs[:512])  # warm-up: code paths and the team-size tuner
-    rate = 1e6  # requests/s: far above saturation, so the engine is the bottleneck
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    torch.cuda.synchronize()
+def _mpc_stream(eng, reqs, arrivals, label):
+    """Feeds requests to the engine at their arrival times (seconds from the
+    start) and steps it until every result is seen; returns the elapsed time,
+    per-request latencies (result seen by the host - arrival), statuses,
+    objective values and ADMM iteration counts."""
+    n_req = len(reqs)
     t_start = time.perf_counter()
-    ids, arr_of, lat, status, iters, objs = [], {}, [], {}, [], {}
+    arr_of, lat, status, iters, objs = {}, [], {}, [], {}
     nxt = 0
     pending = set()
     t_log = time.perf_counter()
     while nxt < n_req or pending:
         if time.perf_counter() - t_log > 5.0:
-            log(f"[serve] mpc: {nxt} admitted, {len(lat)} done")
+            log(f"[serve] mpc {label}: {nxt} admitted, {len(lat)} done")
             t_log = time.perf_counter()
         now = time.perf_counter() - t_start
         # take at most 256 arrivals between two steps, so the engine works on
@@ -99,11 +84,42 @@ def serve_mpc(n_req, seed):
             objs[k] = r.info.obj_val
             iters.append(r.info.iter)
             pending.discard(pid)
-    elapsed = time.perf_counter() - t_start
-    cnt = eng.counters()
+    return time.perf_counter() - t_start, lat, status, objs, iters
+
+
+def serve_mpc(n_req, seed):
+    """MPC stream (nx=12, nu=4, N ~ U{10..50}) through the batched FP32-PCG
+    engine.  Throughput: Poisson arrivals at a rate far above saturation (up to
+    4096 in flight), so the engine is the bottleneck.  Latency under load: the
+    same requests again, Poisson arrivals at 70 % of that throughput.  Latency
+    = result seen by the host - arrival.  The plain FP64 engine (osqp_setup +
+    osqp_solve per request, one at a time) on a sample of the same requests is
+    the GPU comparison; statuses are compared on it."""
+    import torch
+    from osqp_amd import OSQP
+    from osqp_amd import problems as pr
+    from osqp_amd.serve import BatchEngine
+    st = dict(eps_abs=1e-4, eps_rel=1e-4)
+    rng = np.random.default_rng(seed + 17)
+    Ns = rng.integers(10, 51, n_req)
+    t0 = time.time()
+    reqs = [pr.mpc(int(N), seed=seed * 100000 + k) for k, N in enumerate(Ns)]
+    log(f"[serve] generated {n_req} MPC requests in {time.time() - t0:.1f}s")
+    eng = BatchEngine(precision=32, max_active=4096, **st)
+    eng.solve_batch(reqs[:512])  # warm-up: code paths and the team-size tuner
+    rate = 1e6  # requests/s: far above saturation
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    torch.cuda.synchronize()
+    c0 = eng.counters()
+    elapsed, lat, status, objs, iters = _mpc_stream(eng, reqs, arrivals, "saturated")
+    c1 = eng.counters()
+    qps = n_req / elapsed
+    rate70 = 0.7 * qps
+    arrivals70 = np.cumsum(rng.exponential(1.0 / rate70, n_req))
+    el70, lat70, status70, _, _ = _mpc_stream(eng, reqs, arrivals70, "70% load")
     tuning = [eng.tuning(b) for b in range(6)]
     eng.close()
-    qps = n_req / elapsed
+    same_runs = sum(int(status70[k] == status[k]) for k in range(n_req))
     # plain FP64 engine, one request at a time (setup + solve), on a sample
     k_plain = 24
     same = 0
@@ -119,19 +135,23 @@ def serve_mpc(n_req, seed):
         s.close()
     plain_qps = k_plain / (time.perf_counter() - ts)
     obj_err = max(abs(objs[k] - plain[k].info.obj_val) / max(1.0, abs(plain[k].info.obj_val)) for k in range(k_plain))
-    log(f"[serve] batched FP32: {qps:.0f} QPs/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
-        f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms; plain FP64: {plain_qps:.1f} QPs/s; "
-        f"statuses identical on {same}/{k_plain}")
+    pct = lambda v, q: float(np.percentile(v, q) * 1e3)
+    log(f"[serve] batched FP32: {qps:.0f} QPs/s, p50 {pct(lat, 50):.1f} ms, p99 {pct(lat, 99):.1f} ms; "
+        f"at {rate70:.0f}/s: p50 {pct(lat70, 50):.1f} ms, p99 {pct(lat70, 99):.1f} ms; "
+        f"plain FP64: {plain_qps:.1f} QPs/s; statuses identical on {same}/{k_plain}")
     return {"workload": f"MPC stream nx=12 nu=4 N~U{{10..50}}, {n_req} requests, Poisson arrivals at "
                         f"{rate:.0e}/s, <=4096 in flight, eps 1e-4",
-            "qps": qps, "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
-                                       "p99": float(np.percentile(lat, 99) * 1e3)},
+            "qps": qps, "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+            "at_70pct_load": {"arrival_rate_per_s": rate70, "qps": n_req / el70,
+                              "latency_ms": {"p50": pct(lat70, 50), "p99": pct(lat70, 99)},
+                              "statuses_same_as_saturated_run": same_runs},
             "mean_admm_iters": float(np.mean(iters)),
             "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
             "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
             "plain_sample": k_plain, "statuses_identical_on_sample": same,
             "max_obj_rel_err_vs_fp64_on_sample": obj_err,
-            "engine": {"epochs": cnt["epochs"], "gpu_s": cnt["gpu_s"], "tuned_team_per_bucket": tuning}}
+            "engine": {"epochs": c1["epochs"] - c0["epochs"], "gpu_s": c1["gpu_s"] - c0["gpu_s"],
+                       "tuned_team_per_bucket": tuning}}
 
 
 def serve_portfolio(n_problems, n_req, seed):
