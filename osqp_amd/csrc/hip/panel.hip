@@ -101,16 +101,24 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
           c[j] = (int)ld_nt(ci + k0 + j * 64 + lane);
           v[j] = ld_nt(val + k0 + j * 64 + lane);
         }
+        // Every index loaded, inside the task or past its end, is a panel-local
+        // column (< W; zero in the padding), so the gathers need no guard and
+        // are issued back to back; entries past the end are then dropped.
+        TX pr[PER];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) prod[j * 64 + lane] = j * 64 + lane < len ? (TX)v[j] * xs[c[j]] : (TX)0;
+        for (int j = 0; j < PER; ++j) pr[j] = (TX)v[j] * xs[c[j]];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) prod[j * 64 + lane] = j * 64 + lane < len ? pr[j] : (TX)0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // L = lanes per row: the largest power of two with L * nr <= 64
+        // (64 for nr <= 1); G = 64 / L row groups; all as shifts
         const int nr = re - rb;
-        int L = 64;
-        while (L > 1 && L * nr > 64) L >>= 1;
-        const int G = 64 / L, g = lane / L, gl = lane % L;
-        const int bound = ((nr + G - 1) / G) * G;
+        const int lgL = nr <= 1 ? 6 : 6 - (32 - __builtin_clz((unsigned)(nr - 1)));
+        const int L = 1 << lgL, lgG = 6 - lgL;
+        const int G = 1 << lgG, g = lane >> lgL, gl = lane & (L - 1);
+        const int bound = ((nr + G - 1) >> lgG) << lgG;
         for (int r = g; r < bound; r += G) {
           const int s = __shfl(rpl, r & 63) - k0;
           const int e = __shfl(rpl, (r + 1) & 63) - k0;
