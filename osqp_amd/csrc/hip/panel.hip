@@ -73,6 +73,9 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
       const int mine = (te - ub + NW - 1) / NW < 64 ? (te - ub + NW - 1) / NW : 64;
       int4 d = make_int4(0, 0, 0, 0);
       if (lane < mine) d = task[ub + lane * NW];
+      // wait for the descriptors here, once: otherwise the wait at the head of
+      // the task loop also waits for the previous task's partial-sum stores
+      asm volatile("" : "+v"(d.x), "+v"(d.y), "+v"(d.z), "+v"(d.w));
       for (int i = 0; i < mine; ++i) {
         const int rb = __builtin_amdgcn_readlane(d.x, i);
         const int re = __builtin_amdgcn_readlane(d.y, i);
