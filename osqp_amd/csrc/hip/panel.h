@@ -9,17 +9,21 @@
 // gathers from a copy of the panel's slice of x in LDS:
 //
 //   * work items ("tasks") are row ranges of one panel holding at most
-//     64 * PER nonzeros and at most 63 rows, listed panel by panel (a panel
-//     row longer than that is a task of its own, reduced in several passes);
+//     64 * PER - (V - 1) nonzeros (V = 4 for FP32, 2 for FP64) and at most 63
+//     rows, listed panel by panel (a panel row longer than that is a task of
+//     its own, reduced in several passes);
 //     the task descriptor carries the nonzero range so a wavefront issues its
 //     stream loads without waiting for a row pointer;
 //   * a workgroup walks a contiguous range of tasks; for every panel in the
 //     range it loads the x slice into LDS once (all loads in flight, then the
 //     LDS stores) and its wavefronts take the panel's tasks independently —
 //     no workgroup barrier inside a panel;
-//   * a wavefront streams the task's (index, value) pairs with branch-free
-//     loads (arrays are padded by 64 * PER entries), multiplies with the LDS
-//     slice into a per-wavefront LDS product buffer, then reduces each row in
+//   * a wavefront streams the window of 64 * PER (index, value) pairs from
+//     the task's first nonzero rounded down to V, each lane V consecutive
+//     pairs per load (a 16-byte value load: the address unit's cost is per
+//     instruction), with branch-free loads (arrays are padded by 64 * PER
+//     entries); it multiplies with the LDS slice into a per-wavefront LDS
+//     product buffer (zero outside the task), then reduces each row in
 //     column order (groups of L lanes per row, L a power of two chosen from the
 //     task's row count; in-order sums within a lane, then a fixed butterfly);
 //   * row totals go to partial[p * R + row]; combine kernels add the np
@@ -38,7 +42,7 @@ constexpr int kPanelRowCap = 63;  // rows per task: one row pointer per lane
 struct PanelMatrix {
   int R, C;
   int W;    // columns per panel
-  int per;  // nonzeros per lane per task (task holds <= 64 * per)
+  int per;  // nonzeros per lane per task window (task holds <= 64 * per - (V - 1))
   int np;
   long long nnz;
   int* rp;         // np * (R + 1) row pointers, absolute into ci/val

@@ -31,6 +31,16 @@ __device__ __forceinline__ T ld_nt(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Entries per lane per load group: a lane loads V consecutive (index, value)
+// pairs with one 16-byte value load and one 8- or 4-byte index load (the
+// address unit's cost is per instruction, so wide loads move more per slot).
+template <typename TV>
+constexpr int panel_vec() { return sizeof(TV) == 4 ? 4 : 2; }
+
 // One task per wavefront at a time; see panel.h for the scheme.
 template <typename TV, typename TX, int W, int NW, int PER>
 __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int ntask, const int4* __restrict__ task,
@@ -39,8 +49,12 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
                                                         int R, int C, const TX* __restrict__ x0, int n0,
                                                         const TX* __restrict__ x1, TX* __restrict__ partial) {
   if (active && !*active) return;
+  constexpr int V = panel_vec<TV>();
+  constexpr int NG = PER / V;          // load groups of 64 V entries
+  constexpr int CAP = 64 * PER - (V - 1);  // entries of a task (the aligned window holds 64 PER)
+  static_assert(PER % V == 0, "PER must be a multiple of the vector width");
   __shared__ TX xs[W];
-  __shared__ TX prod_all[NW][64 * PER];
+  __shared__ __attribute__((aligned(16))) TX prod_all[NW][64 * PER];
   constexpr int XL = W / (NW * 64);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -81,7 +95,7 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
         const int re = __builtin_amdgcn_readlane(d.y, i);
         const int k0 = __builtin_amdgcn_readlane(d.z, i);
         const int len = __builtin_amdgcn_readlane(d.w, i) - k0;
-        if (len > 64 * PER) {
+        if (len > CAP) {
           // one panel row longer than a task: lane-strided passes, then a
           // fixed butterfly
           TX acc = 0;
@@ -97,21 +111,63 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
           continue;
         }
         const int rpl = lane <= re - rb ? rpp[rb + lane] : 0;
-        int c[PER];
-        TV v[PER];
+        // The window of 64 PER entries from base = k0 rounded down to V holds
+        // the task (tasks hold at most CAP entries).  Lane l loads entries
+        // base + g 64 V + l V + [0, V) of group g as vectors; every index
+        // loaded, in the task or not, is a panel-local column (< W; zero in
+        // the padding), so the gathers need no guard; window positions outside
+        // [off, off + len) are then stored as zero products.
+        const int base = k0 & ~(V - 1), off = k0 - base;
+        // all loads first (packed indices: two per 32-bit word), then per
+        // group: gathers, products, one 16-byte LDS store
+        constexpr int CW = V / 2;  // 32-bit index words per lane per group
+        unsigned cw[NG][CW];
+        TV vv[NG][V];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-          c[j] = (int)ld_nt(ci + k0 + j * 64 + lane);
-          v[j] = ld_nt(val + k0 + j * 64 + lane);
+        for (int g = 0; g < NG; ++g) {
+          const long long at = (long long)base + g * 64 * V + lane * V;
+          if constexpr (V == 4) {
+            const u32x2 w2 = ld_nt((const u32x2*)(ci + at));
+            const f32x4 v4 = ld_nt((const f32x4*)(val + at));
+            cw[g][0] = w2.x;
+            cw[g][1] = w2.y;
+            vv[g][0] = v4.x;
+            vv[g][1] = v4.y;
+            vv[g][2] = v4.z;
+            vv[g][3] = v4.w;
+          } else {
+            const unsigned w1 = ld_nt((const unsigned*)(ci + at));
+            const f64x2 v2 = ld_nt((const f64x2*)(val + at));
+            cw[g][0] = w1;
+            vv[g][0] = v2.x;
+            vv[g][1] = v2.y;
+          }
         }
-        // Every index loaded, inside the task or past its end, is a panel-local
-        // column (< W; zero in the padding), so the gathers need no guard and
-        // are issued back to back; entries past the end are then dropped.
-        TX pr[PER];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) pr[j] = (TX)v[j] * xs[c[j]];
+        for (int g = 0; g < NG; ++g) {
+          const int q0 = g * 64 * V + lane * V;
+          TX pr[V];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) prod[j * 64 + lane] = j * 64 + lane < len ? pr[j] : (TX)0;
+          for (int i = 0; i < V; ++i) {
+            const unsigned col = (cw[g][i >> 1] >> (16 * (i & 1))) & 0xffffu;
+            const TX x = xs[col];
+            const int q = q0 + i;
+            pr[i] = (q >= off && q < off + len) ? (TX)vv[g][i] * x : (TX)0;
+          }
+          if constexpr (V == 4) {
+            f32x4 o;
+            o.x = pr[0];
+            o.y = pr[1];
+            o.z = pr[2];
+            o.w = pr[3];
+            *(f32x4*)(prod + q0) = o;
+          } else {
+            f64x2 o;
+            o.x = pr[0];
+            o.y = pr[1];
+            *(f64x2*)(prod + q0) = o;
+          }
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -123,8 +179,8 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
         const int G = 1 << lgG, g = lane >> lgL, gl = lane & (L - 1);
         const int bound = ((nr + G - 1) >> lgG) << lgG;
         for (int r = g; r < bound; r += G) {
-          const int s = __shfl(rpl, r & 63) - k0;
-          const int e = __shfl(rpl, (r + 1) & 63) - k0;
+          const int s = __shfl(rpl, r & 63) - base;
+          const int e = __shfl(rpl, (r + 1) & 63) - base;
           TX acc = 0;
           if (r < nr) {
             int k = s + gl;
@@ -201,9 +257,11 @@ int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int
       lci[(size_t)pos] = (uint16_t)(ci[k] - p * W);
       src[(size_t)pos] = k;
     }
-  // tasks: per panel, greedy row ranges with <= 64 per nonzeros and <= 63
-  // rows; a panel row longer than 64 per nonzeros is a task of its own
-  const int cap = 64 * per;
+  // tasks: per panel, greedy row ranges with <= cap nonzeros and <= 63 rows;
+  // a panel row longer than cap is a task of its own.  cap leaves room for
+  // the kernel's window alignment (it loads 64 per entries from the task's
+  // first nonzero rounded down to the vector width).
+  const int cap = 64 * per - ((bits == 32 ? 4 : 2) - 1);
   std::vector<int4> tasks;
   std::vector<int2> tp;
   for (int p = 0; p < np && R > 0; ++p) {
