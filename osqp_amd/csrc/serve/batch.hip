@@ -339,9 +339,9 @@ Layout make_layout(const HostData& h, int vbytes) {
 }
 
 // Pass 2 of the packing: the long and empty row lists and the slot arrays
-// (a group's entry number e sits in lane e / kGroupPer, element e % kGroupPer;
-// slot = element * kWave + lane), written into the image img whose first
-// byte is block offset base.
+// (a group's entry number e sits in lane e / kGroupPer, element e % kGroupPer,
+// at slot e: a lane's slots are contiguous), written into the image img whose
+// first byte is block offset base.
 void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, const Layout::Groups& o,
                   char* img, size_t base) {
   int* longs = (int*)(img + (o.longs - base));
@@ -372,8 +372,7 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
     int* gr = gr0 + (size_t)grp * kGroupSlots;
     int* gc = gc0 + (size_t)grp * kGroupSlots;
     for (int k = s; k < e; ++k) {
-      const int pos = fill++;
-      const int slot = (pos % kGroupPer) * kWave + pos / kGroupPer;
+      const int slot = fill++;
       gk[slot] = k;
       gr[slot] = r;
       gc[slot] = ci[k];

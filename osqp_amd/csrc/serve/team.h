@@ -69,9 +69,10 @@ constexpr double kCgTolMin = 1e-7;
 
 // CSR matrix with its row groups (rows of 1..kLongRow entries), long-row and
 // empty-row lists.  Slot (g, lane, e) of the groups sits at
-// g * kGroupSlots + e * kWave + lane and holds the group's entry lane * 8 + e
-// (row order, then column order), or padding (gk = grow = -1) after the
-// group's last entry.
+// g * kGroupSlots + lane * kGroupPer + e and holds the group's entry
+// lane * kGroupPer + e (row order, then column order), or padding
+// (gk = grow = -1) after the group's last entry.  A lane's slots are
+// contiguous, so it loads them with 16-byte vector loads.
 struct BCsr {
   int rows;
   int ng;       // row groups
@@ -285,15 +286,31 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
   const double* __restrict__ gval = M.gval;
   const double* __restrict__ val = M.val;
   const int lane = t.lane;
+  static_assert(kGroupPer == 8, "a lane's slots are loaded as two int4 / four double2 vectors");
+  typedef int i32x4 __attribute__((ext_vector_type(4)));
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
   for (int g = t.wv; g < M.ng; g += t.W) {
-    const int base = g * kGroupSlots + lane;
+    const int base = g * kGroupSlots + lane * kGroupPer;
     int rr[kGroupPer], cc[kGroupPer];
     double vv[kGroupPer];
 #pragma unroll
-    for (int e = 0; e < kGroupPer; ++e) {
-      rr[e] = grow[base + e * kWave];
-      cc[e] = gci[base + e * kWave];
-      vv[e] = gval[base + e * kWave];
+    for (int h = 0; h < 2; ++h) {
+      const i32x4 r4 = *(const i32x4*)(grow + base + 4 * h);
+      const i32x4 c4 = *(const i32x4*)(gci + base + 4 * h);
+      rr[4 * h + 0] = r4.x;
+      rr[4 * h + 1] = r4.y;
+      rr[4 * h + 2] = r4.z;
+      rr[4 * h + 3] = r4.w;
+      cc[4 * h + 0] = c4.x;
+      cc[4 * h + 1] = c4.y;
+      cc[4 * h + 2] = c4.z;
+      cc[4 * h + 3] = c4.w;
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const f64x2 v2 = *(const f64x2*)(gval + base + 2 * h);
+      vv[2 * h + 0] = v2.x;
+      vv[2 * h + 1] = v2.y;
     }
     TX v[kGroupPer];
 #pragma unroll
