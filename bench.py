@@ -66,14 +66,15 @@ def _mpc_stream(eng, reqs, arrivals, label):
         now = time.perf_counter() - t_start
         # take at most 256 arrivals between two steps, so the engine works on
         # admitted requests while the host takes in the next ones
-        taken = 0
-        while nxt < n_req and arrivals[nxt] <= now and len(pending) < 4096 and taken < 256:
-            taken += 1
-            p = reqs[nxt]
-            pid = eng.add(p["P"], p["q"], p["A"], p["l"], p["u"])
-            arr_of[pid] = (nxt, arrivals[nxt])
-            pending.add(pid)
-            nxt += 1
+        k1 = nxt
+        while k1 < n_req and arrivals[k1] <= now and len(pending) + (k1 - nxt) < 4096 and k1 - nxt < 256:
+            k1 += 1
+        if k1 > nxt:
+            # the arrivals since the last step, queued in one call
+            for k, pid in zip(range(nxt, k1), eng.add_many(reqs[nxt:k1])):
+                arr_of[pid] = (k, arrivals[k])
+                pending.add(pid)
+            nxt = k1
         done = eng.step()
         t_seen = time.perf_counter() - t_start
         for pid in done:

@@ -70,6 +70,18 @@ OSQPInt osqp_batch_add(OSQPBatch* batch, const OSQPCscMatrix* P, const OSQPFloat
                        const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0,
                        OSQPInt resident, OSQPInt* id);
 
+/* Queue count problems at once: the same as count osqp_batch_add calls in
+ * order, with the host-side preparation (validation, CSR and row-group
+ * images) spread over up to 8 threads.  P, q, A, l, u: arrays of count
+ * pointers; x0 / y0: arrays of count pointers (entries may be null) or null.
+ * ids[k] = the id of problem k, or -1 when it was rejected.  Returns 0 when
+ * all were queued, else the error code of the first rejected problem.  No
+ * counterpart in the reference (one osqp_setup per problem there). */
+OSQPInt osqp_batch_add_many(OSQPBatch* batch, OSQPInt count, const OSQPCscMatrix* const* P, const OSQPFloat* const* q,
+                            const OSQPCscMatrix* const* A, const OSQPFloat* const* l, const OSQPFloat* const* u,
+                            const OSQPFloat* const* x0, const OSQPFloat* const* y0, OSQPInt resident,
+                            OSQPInt* ids);
+
 /* Queue a warm-started re-solve of a resident problem whose last solve has
  * ended and been polled; q_new (may be null) replaces q first.  Returns 0, or
  * -1 when the problem is unknown, not resident or busy. */

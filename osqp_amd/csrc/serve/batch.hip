@@ -19,9 +19,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <map>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "osqp_amd_serve.h"
@@ -742,6 +744,55 @@ bool ensure_graph(Engine* E, Bucket& B, int T, int tpc) {
   return true;
 }
 
+constexpr int kAddThreads = 8;  // worker threads of osqp_batch_add_many
+
+// Validates a problem (src/auxil.c validate_data) and builds its host image.
+// Reads the engine's fixed configuration only, so problems may be prepared
+// on several threads at once.  Returns 0 with *out set, or an error code.
+int prepare(const Engine* E, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A, const OSQPFloat* l,
+            const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, OSQPInt resident, HProb** out) {
+  *out = nullptr;
+  if (!P || !A || !q) return OSQP_DATA_VALIDATION_ERROR;
+  const int n = P->n, m = A->m;
+  if (n <= 0 || m < 0 || P->m != n || A->n != n) return OSQP_DATA_VALIDATION_ERROR;
+  if (m > 0 && (!l || !u)) return OSQP_DATA_VALIDATION_ERROR;
+  if (n > kMaxRedBlocks * kRedBlock || m > kMaxRedBlocks * kRedBlock) {
+    fprintf(stderr, "osqp_batch: problem too large for the batched engine (n, m <= %d)\n", kMaxRedBlocks * kRedBlock);
+    return OSQP_DATA_VALIDATION_ERROR;
+  }
+  for (int j = 0; j < n; ++j)
+    for (int k = P->p[j]; k < P->p[j + 1]; ++k)
+      if (P->i[k] > j || P->i[k] < 0) return OSQP_DATA_VALIDATION_ERROR;
+  for (int j = 0; j < n; ++j)
+    for (int k = A->p[j]; k < A->p[j + 1]; ++k)
+      if (A->i[k] < 0 || A->i[k] >= m) return OSQP_DATA_VALIDATION_ERROR;
+  for (int i = 0; i < m; ++i)
+    if (l[i] > u[i]) return OSQP_DATA_VALIDATION_ERROR;
+  HProb* p = new HProb();
+  p->resident = resident ? 1 : 0;
+  p->L = build_host(&p->h, E->bits / 8, P, q, A, l, u, x0, y0);
+  if (p->L.total > E->arena_size || p->L.st_total + p->L.rs_total > E->pool_size) {
+    // could never be admitted: larger than the whole arena or staging pool
+    fprintf(stderr, "osqp_batch: problem needs %zu bytes of device arena (of %zu) and %zu of staging (of %zu)\n",
+            p->L.total, E->arena_size, p->L.st_total + p->L.rs_total, E->pool_size);
+    delete p;
+    return OSQP_DATA_VALIDATION_ERROR;
+  }
+  p->bucket = bucket_of(p->h);
+  *out = p;
+  return 0;
+}
+
+// Queues a prepared problem (in arrival order); returns its id.
+int enqueue(Engine* E, HProb* p) {
+  p->id = E->next_id++;
+  p->state = kQueued;
+  p->t_req = now_s();
+  E->probs[p->id] = p;
+  E->queue.push_back(p->id);
+  return p->id;
+}
+
 }  // namespace
 }  // namespace osqp_serve
 
@@ -830,43 +881,46 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
 OSQPInt osqp_batch_add(OSQPBatch* b, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
                        const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0,
                        OSQPInt resident, OSQPInt* id) {
-  if (!b || !P || !A || !q || !id) return OSQP_DATA_VALIDATION_ERROR;
-  const int n = P->n, m = A->m;
-  // src/auxil.c validate_data
-  if (n <= 0 || m < 0 || P->m != n || A->n != n) return OSQP_DATA_VALIDATION_ERROR;
-  if (m > 0 && (!l || !u)) return OSQP_DATA_VALIDATION_ERROR;
-  if (n > kMaxRedBlocks * kRedBlock || m > kMaxRedBlocks * kRedBlock) {
-    fprintf(stderr, "osqp_batch: problem too large for the batched engine (n, m <= %d)\n", kMaxRedBlocks * kRedBlock);
-    return OSQP_DATA_VALIDATION_ERROR;
-  }
-  for (int j = 0; j < n; ++j)
-    for (int k = P->p[j]; k < P->p[j + 1]; ++k)
-      if (P->i[k] > j || P->i[k] < 0) return OSQP_DATA_VALIDATION_ERROR;
-  for (int j = 0; j < n; ++j)
-    for (int k = A->p[j]; k < A->p[j + 1]; ++k)
-      if (A->i[k] < 0 || A->i[k] >= m) return OSQP_DATA_VALIDATION_ERROR;
-  for (int i = 0; i < m; ++i)
-    if (l[i] > u[i]) return OSQP_DATA_VALIDATION_ERROR;
-  Engine* E = &b->e;
-  HProb* p = new HProb();
-  p->id = E->next_id++;
-  p->resident = resident ? 1 : 0;
-  p->L = build_host(&p->h, E->bits / 8, P, q, A, l, u, x0, y0);
-  if (p->L.total > E->arena_size || p->L.st_total + p->L.rs_total > E->pool_size) {
-    // could never be admitted: larger than the whole arena or staging pool
-    fprintf(stderr, "osqp_batch: problem needs %zu bytes of device arena (of %zu) and %zu of staging (of %zu)\n",
-            p->L.total, E->arena_size, p->L.st_total + p->L.rs_total, E->pool_size);
-    E->next_id--;
-    delete p;
-    return OSQP_DATA_VALIDATION_ERROR;
-  }
-  p->bucket = bucket_of(p->h);
-  p->state = kQueued;
-  p->t_req = now_s();
-  E->probs[p->id] = p;
-  E->queue.push_back(p->id);
-  *id = p->id;
+  if (!b || !id) return OSQP_DATA_VALIDATION_ERROR;
+  HProb* p = nullptr;
+  const int rc = osqp_serve::prepare(&b->e, P, q, A, l, u, x0, y0, resident, &p);
+  if (rc) return rc;
+  *id = osqp_serve::enqueue(&b->e, p);
   return 0;
+}
+
+OSQPInt osqp_batch_add_many(OSQPBatch* b, OSQPInt count, const OSQPCscMatrix* const* P, const OSQPFloat* const* q,
+                            const OSQPCscMatrix* const* A, const OSQPFloat* const* l, const OSQPFloat* const* u,
+                            const OSQPFloat* const* x0, const OSQPFloat* const* y0, OSQPInt resident,
+                            OSQPInt* ids) {
+  if (!b || count < 0 || (count > 0 && (!P || !q || !A || !l || !u || !ids))) return OSQP_DATA_VALIDATION_ERROR;
+  std::vector<HProb*> ready((size_t)count, nullptr);
+  std::vector<int> codes((size_t)count, 0);
+  // the host-side preparation (validation, CSR / row-group image) of each
+  // problem touches only its own data: spread it over worker threads, then
+  // queue in order on this thread
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int k = next.fetch_add(1); k < count; k = next.fetch_add(1))
+      codes[(size_t)k] = osqp_serve::prepare(&b->e, P[k], q[k], A[k], l[k], u[k], x0 ? x0[k] : nullptr,
+                                             y0 ? y0[k] : nullptr, resident, &ready[(size_t)k]);
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nthreads = (int)std::min<long>(std::min<long>(kAddThreads, hw ? (long)hw : 1L), count / 8 + 1);
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  OSQPInt first_error = 0;
+  for (int k = 0; k < count; ++k) {
+    if (codes[(size_t)k]) {
+      ids[k] = -1;
+      if (!first_error) first_error = codes[(size_t)k];
+    } else {
+      ids[k] = osqp_serve::enqueue(&b->e, ready[(size_t)k]);
+    }
+  }
+  return first_error;
 }
 
 OSQPInt osqp_batch_resolve(OSQPBatch* b, OSQPInt id, const OSQPFloat* q_new) {

@@ -40,6 +40,8 @@ def _bind(lib):
     lib.osqp_batch_create.restype = C.c_int
     lib.osqp_batch_add.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.POINTER(C.c_int)]
     lib.osqp_batch_add.restype = C.c_int
+    lib.osqp_batch_add_many.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
+    lib.osqp_batch_add_many.restype = C.c_int
     lib.osqp_batch_resolve.argtypes = [vp, C.c_int, vp]
     lib.osqp_batch_resolve.restype = C.c_int
     lib.osqp_batch_step.argtypes = [vp, C.POINTER(C.c_int)]
@@ -93,6 +95,34 @@ class BatchEngine:
             raise ValueError(f"osqp_batch_add rejected the problem ({rc})")
         self._dims[pid.value] = (Ph.n, Ah.m)
         return pid.value
+
+    def add_many(self, problems, resident=False):
+        """Queues a list of problems (dicts with P, q, A, l, u and optional x0,
+        y0) in one call; their host-side preparation runs on several threads.
+        Returns their ids in order (None for a rejected problem)."""
+        k = len(problems)
+        if k == 0:
+            return []
+        arr = lambda: (C.c_void_p * k)()
+        Pp, qp, Ap, lp, up, xp, yp = arr(), arr(), arr(), arr(), arr(), arr(), arr()
+        keep = []
+        for i, p in enumerate(problems):
+            Ph, Ah = T.CscHolder(p["P"]), T.CscHolder(p["A"])
+            q, l, u = T.as_f64(p["q"]), T.as_f64(p["l"]), T.as_f64(p["u"])
+            x0, y0 = T.as_f64(p.get("x0")), T.as_f64(p.get("y0"))
+            keep.append((Ph, Ah, q, l, u, x0, y0))
+            Pp[i], Ap[i] = C.addressof(Ph.s), C.addressof(Ah.s)
+            qp[i], lp[i], up[i], xp[i], yp[i] = T.addr(q), T.addr(l), T.addr(u), T.addr(x0), T.addr(y0)
+        ids = (C.c_int * k)()
+        self.lib.osqp_batch_add_many(self._b, k, Pp, qp, Ap, lp, up, xp, yp, 1 if resident else 0, ids)
+        out = []
+        for i, (Ph, Ah, *_rest) in enumerate(keep):
+            if ids[i] >= 0:
+                self._dims[ids[i]] = (Ph.n, Ah.m)
+                out.append(ids[i])
+            else:
+                out.append(None)
+        return out
 
     def resolve(self, pid, q=None):
         q = T.as_f64(q)
@@ -154,7 +184,9 @@ class BatchEngine:
     def solve_batch(self, problems, max_epochs=100000):
         """Adds every problem (dicts with P, q, A, l, u), runs epochs until all
         have ended and returns their results in order."""
-        ids = [self.add(p["P"], p["q"], p["A"], p["l"], p["u"], p.get("x0"), p.get("y0")) for p in problems]
+        ids = self.add_many(problems)
+        if any(i is None for i in ids):
+            raise ValueError("osqp_batch_add_many rejected a problem")
         out = {}
         for _ in range(max_epochs):
             if not self.pending():

@@ -122,6 +122,32 @@ def test_alone_vs_mix_bit_identical():
         same(a, b)
 
 
+def test_add_many_matches_add():
+    """osqp_batch_add_many (preparation on worker threads) queues the same
+    problems as one osqp_batch_add call each, in order: bit-identical results;
+    a rejected problem (l > u) gets no id and the rest are queued."""
+    rng = np.random.default_rng(3)
+    probs = [pr.mpc(int(N), seed=300 + k) for k, N in enumerate(rng.integers(10, 51, 40))]
+    probs += [pr.random_qp(40, 150, 0.15, 0.15, seed=70 + k) for k in range(4)]
+    eng = BatchEngine(precision=32, **ST)
+    one = run_all(eng, probs)
+    eng.close()
+    bad = dict(probs[3])
+    bad["l"] = bad["u"] + 1.0
+    eng = BatchEngine(precision=32, **ST)
+    ids = eng.add_many(probs[:3] + [bad] + probs[3:])
+    assert ids[3] is None and all(i is not None for i in ids[:3] + ids[4:])
+    assert ids[:3] + ids[4:] == sorted(ids[:3] + ids[4:])
+    got = {}
+    while eng.pending():
+        for i in eng.step():
+            got[i] = eng.poll(i)
+    eng.close()
+    many = [got[i] for i in ids[:3] + ids[4:]]
+    for a, b in zip(one, many):
+        same(a, b)
+
+
 def test_team_size_does_not_change_results():
     probs = [pr.mpc(30, seed=2), pr.random_qp(80, 400, 0.08, 0.08, seed=4), pr.portfolio(500, seed=8)]
     outs = []
