@@ -252,6 +252,70 @@ __global__ void k_comb_kv(const PcgState* st, int n, int np, const TX* partial, 
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// The same for n divisible by RB = 16 / sizeof(TX): a group of G lanes serves
+// RB adjacent rows, and a lane loads their partials of a panel as one 16-byte
+// vector (two or four times the data per load instruction of the form above;
+// the address unit's cost is per instruction).  Each row's partials are added
+// in the same order as by panel_sum, so Kv is bit-identical to k_comb_kv's.
+template <typename TX, int G>
+__global__ void k_comb_kv_rows(const PcgState* st, int n, int np, const TX* partial, double sigma, const TX* v,
+                               const double* sub, TX* Kv, double* partials) {
+  if (!st->active) return;
+  constexpr int RB = 16 / (int)sizeof(TX);
+  typedef TX VT __attribute__((ext_vector_type(RB)));
+  __shared__ double lds[kBlock / kWave];
+  const TX sg = (TX)sigma;
+  Acc<TX> dot;
+  double mx = 0.0;
+  const int nq = n / RB;
+  COMB_ROWS(q, j, nq, G) {
+    const int p0 = (np * j) / G, p1 = (np * (j + 1)) / G;
+    const VT* __restrict__ col = (const VT*)(partial + (long long)q * RB);
+    TX s[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) s[r] = 0;
+    int p = p0;
+    for (; p + 4 <= p1; p += 4) {
+      const VT a0 = col[(long long)p * nq], a1 = col[(long long)(p + 1) * nq];
+      const VT a2 = col[(long long)(p + 2) * nq], a3 = col[(long long)(p + 3) * nq];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        s[r] += a0[r];
+        s[r] += a1[r];
+        s[r] += a2[r];
+        s[r] += a3[r];
+      }
+    }
+    for (; p < p1; ++p) {
+      const VT a = col[(long long)p * nq];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) s[r] += a[r];
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) s[r] += __shfl_xor(s[r], off, G);
+    if (j == 0) {
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int i = q * RB + r;
+        const TX vi = v[i];
+        TX kv = sg * vi + s[r];
+        if (sub) {
+          kv -= (TX)sub[i];
+          const double a = fabs((double)kv);
+          mx = a > mx ? a : mx;
+        } else {
+          dot.add(vi * kv);
+        }
+        Kv[i] = kv;
+      }
+    }
+  }
+  const double t = sub ? block_max(mx, lds) : block_sum(dot.value(), lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 // z = A x (FP64): b[n:n+m] (polishing: rho_pol (A x - b2)) and the A x cache;
 // skipped while *when == 0 (when given)
 template <int G>
@@ -840,6 +904,25 @@ int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* p
   const int n = s->n;
   if (const PanelMatrix* P = panels_K<V>(s)) {
     panel_spmv(*P, active_flag(s), v, n, w, (V*)s->partK);
+    constexpr int RB = 16 / (int)sizeof(V);
+    if (n % RB == 0) {
+      const int nq = n / RB;
+      const int G = comb_group(P->np, nq);
+      const int g = grid_for((long long)nq * G, kBlock, kMaxReduceBlocks);
+      if (G == 8)
+        hipLaunchKernelGGL((k_comb_kv_rows<V, 8>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                           (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+      else if (G == 4)
+        hipLaunchKernelGGL((k_comb_kv_rows<V, 4>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                           (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+      else if (G == 2)
+        hipLaunchKernelGGL((k_comb_kv_rows<V, 2>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                           (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+      else
+        hipLaunchKernelGGL((k_comb_kv_rows<V, 1>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, n, P->np,
+                           (const V*)s->partK, s->sigma, v, sub, Kv, parts);
+      return g;
+    }
     const int G = comb_group(P->np, n);
     const int g = grid_for((long long)n * G, kBlock, kMaxReduceBlocks);
     if (G == 8)
