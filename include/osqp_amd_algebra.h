@@ -176,6 +176,17 @@ void       osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* de
                                 const OSQPVectorf* x_prev, OSQPVectorf* z, OSQPVectorf* y, OSQPVectorf* delta_y,
                                 const OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
                                 const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u);
+/* Queued reductions (no counterpart in the reference, whose reductions each
+ * return to the host): each call queues the same reduction as
+ * OSQPVectorf_norm_inf / scaled_norm_inf / dot_prod / dot_prod_signed (same
+ * kernel, same bits) and returns at once; osqp_amd_red_wait() waits for all
+ * queued ones with one host round trip and stores each result in its *dst.
+ * Up to 31 may be queued (a further call waits first). */
+void       osqp_amd_red_norm_inf(const OSQPVectorf* v, OSQPFloat* dst);
+void       osqp_amd_red_scaled_norm_inf(const OSQPVectorf* S, const OSQPVectorf* v, OSQPFloat* dst);
+void       osqp_amd_red_dot_prod(const OSQPVectorf* a, const OSQPVectorf* b, OSQPFloat* dst);
+void       osqp_amd_red_dot_prod_signed(const OSQPVectorf* a, const OSQPVectorf* b, OSQPInt sign, OSQPFloat* dst);
+void       osqp_amd_red_wait(void);
 /* Last HIP error seen by the backend (0 if none); cleared on read. */
 OSQPInt    osqp_amd_last_error(void);
 

@@ -166,14 +166,28 @@ __device__ void finish_reduce(bool mx, double t, double* partials, unsigned* tic
   }
 }
 
+// A thread takes elements i0, i0 + S, i0 + 2S, ... (S = the grid's thread
+// count) in order, with the loads of eight of them in flight at a time.
+constexpr int kRedUnroll = 8;
 __global__ void k_reduce(int op, RedArgs r, int n, double* partials, unsigned* ticket, double* out, double seq) {
   __shared__ double lds[kBlock / kWave];
   const bool mx = is_max_op(op);
+  const long long S = (long long)gridDim.x * blockDim.x;
   double acc = 0.0;
-  GRID_STRIDE(i, n) {
-    double v = red_elem(op, r, i);
-    if (mx) acc = v > acc ? v : acc;
-    else acc += v;
+  for (long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += kRedUnroll * S) {
+    double v[kRedUnroll];
+#pragma unroll
+    for (int u = 0; u < kRedUnroll; ++u) {
+      const long long i = i0 + u * S;
+      v[u] = i < n ? red_elem(op, r, i) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kRedUnroll; ++u) {
+      if (i0 + u * S < n) {
+        if (mx) acc = v[u] > acc ? v[u] : acc;
+        else acc += v[u];
+      }
+    }
   }
   const double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
   finish_reduce(mx, t, partials, ticket, out, seq, lds);
@@ -199,11 +213,65 @@ double wait_result(Runtime& R, double seq) {
   return h[0];
 }
 
+// Queued reductions: kernel k writes [value, seq] to slot k of the
+// host-coherent scalars.  Kernels on the stream complete in order and each
+// fences its value before its sequence number, so once the last queued
+// sequence number is visible every queued value is.
+constexpr int kRedSlots = 31;  // h_scalars holds 32 [value, seq] slots; slot 0 is the synchronous one
+
+inline int red_grid(int n) {
+  // about one element per lane per unrolled round: few blocks, so few
+  // ticket atomics on the one counter
+  const long long gg = (n + (long long)kBlock * kRedUnroll - 1) / ((long long)kBlock * kRedUnroll);
+  return (int)(gg < 1 ? 1 : (gg > 512 ? 512 : gg));
+}
+
+void red_drain(Runtime& R) {
+  const int k = R.red_queued;
+  if (k == 0) return;
+  const double last = R.red_slot_seq[k - 1];
+  volatile double* h = R.h_scalars;
+  struct timespec t0, t;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  bool seen = false;
+  for (long it = 1; !seen; ++it) {
+    if (h[2 * k + 1] == last) seen = true;
+    else if ((it & 1023) == 0) {
+      clock_gettime(CLOCK_MONOTONIC, &t);
+      if ((t.tv_sec - t0.tv_sec) + 1e-9 * (t.tv_nsec - t0.tv_nsec) > 0.5) break;
+    }
+  }
+  if (!seen) HIPCHK(hipStreamSynchronize(R.stream));
+  for (int j = 0; j < k; ++j) *R.red_dst[j] = h[2 * (j + 1)];
+  R.red_queued = 0;
+}
+
+void red_queue(int op, const RedArgs& r, int n, double* dst) {
+  Runtime& R = rt();
+  if (n <= 0) {
+    *dst = 0.0;
+    return;
+  }
+  if (R.red_queued == kRedSlots) red_drain(R);
+  const int k = R.red_queued;
+  const double seq = ++R.red_seq;
+  hipLaunchKernelGGL(k_reduce, dim3(red_grid(n)), dim3(kBlock), 0, R.stream, op, r, n, R.d_partials, R.d_ticket,
+                     R.dh_scalars + 2 * (k + 1), seq);
+  if (LAUNCHCHK()) {
+    *dst = 0.0;
+    return;
+  }
+  R.red_dst[k] = dst;
+  R.red_slot_seq[k] = seq;
+  R.red_queued = k + 1;
+}
+
 // one kernel; the host reads the result from host-coherent memory
 double reduce(int op, const RedArgs& r, int n) {
   if (n <= 0) return 0.0;
   Runtime& R = rt();
-  int g = reduce_grid(n);
+  red_drain(R);
+  const int g = red_grid(n);
   const double seq = ++R.red_seq;
   hipLaunchKernelGGL(k_reduce, dim3(g), dim3(kBlock), 0, R.stream, op, r, n, R.d_partials, R.d_ticket,
                      R.dh_scalars, seq);
@@ -479,6 +547,28 @@ OSQPFloat OSQPVectorf_dot_prod_signed(const OSQPVectorf* a, const OSQPVectorf* b
   if (sign == -1) return reduce(R_DOT_NEG, r, a->length);
   return reduce(R_DOT, r, a->length);
 }
+
+void osqp_amd_red_norm_inf(const OSQPVectorf* v, OSQPFloat* dst) {
+  RedArgs r{v->d, nullptr, nullptr, nullptr, 0, 0};
+  red_queue(R_NORM_INF, r, v->length, dst);
+}
+
+void osqp_amd_red_scaled_norm_inf(const OSQPVectorf* Sv, const OSQPVectorf* v, OSQPFloat* dst) {
+  RedArgs r{Sv->d, v->d, nullptr, nullptr, 0, 0};
+  red_queue(R_SCALED_NORM_INF, r, v->length, dst);
+}
+
+void osqp_amd_red_dot_prod(const OSQPVectorf* a, const OSQPVectorf* b, OSQPFloat* dst) {
+  RedArgs r{a->d, b->d, nullptr, nullptr, 0, 0};
+  red_queue(R_DOT, r, a->length, dst);
+}
+
+void osqp_amd_red_dot_prod_signed(const OSQPVectorf* a, const OSQPVectorf* b, OSQPInt sign, OSQPFloat* dst) {
+  RedArgs r{a->d, b->d, nullptr, nullptr, 0, 0};
+  red_queue(sign == 1 ? R_DOT_POS : (sign == -1 ? R_DOT_NEG : R_DOT), r, a->length, dst);
+}
+
+void osqp_amd_red_wait(void) { red_drain(rt()); }
 
 OSQPInt OSQPVectorf_all_leq(const OSQPVectorf* l, const OSQPVectorf* u) {
   RedArgs r{l->d, u->d, nullptr, nullptr, 0, 0};

@@ -58,15 +58,22 @@ OSQPFloat osqp_amd_toc(OSQPWorkspace* w) {
 OSQPFloat compute_rho_estimate(const OSQPSolver* solver) {
   OSQPWorkspace* work = solver->work;
   OSQPFloat prim_res = work->scaled_prim_res, dual_res = work->scaled_dual_res;
-  OSQPFloat prim_res_norm, dual_res_norm, temp, rho_estimate;
-  prim_res_norm = OSQPVectorf_norm_inf(work->z);
-  temp = OSQPVectorf_norm_inf(work->Ax);
+  OSQPFloat prim_res_norm, dual_res_norm, temp, rho_estimate, nz, nAx, nq, nAty, nPx;
+  /* the five norms queued, one host round trip */
+  osqp_amd_red_norm_inf(work->z, &nz);
+  osqp_amd_red_norm_inf(work->Ax, &nAx);
+  osqp_amd_red_norm_inf(work->data->q, &nq);
+  osqp_amd_red_norm_inf(work->Aty, &nAty);
+  osqp_amd_red_norm_inf(work->Px, &nPx);
+  osqp_amd_red_wait();
+  prim_res_norm = nz;
+  temp = nAx;
   prim_res_norm = c_max(prim_res_norm, temp);
   prim_res /= (prim_res_norm + OSQP_DIVISION_TOL);
-  dual_res_norm = OSQPVectorf_norm_inf(work->data->q);
-  temp = OSQPVectorf_norm_inf(work->Aty);
+  dual_res_norm = nq;
+  temp = nAty;
   dual_res_norm = c_max(dual_res_norm, temp);
-  temp = OSQPVectorf_norm_inf(work->Px);
+  temp = nPx;
   dual_res_norm = c_max(dual_res_norm, temp);
   dual_res /= (dual_res_norm + OSQP_DIVISION_TOL);
   rho_estimate = solver->settings->rho * sqrt(prim_res / dual_res);
@@ -177,18 +184,29 @@ void update_y(OSQPSolver* solver) {
   OSQPVectorf_plus(work->y, work->y, work->delta_y);
 }
 
-/* auxil.c:231-285 */
-void compute_obj_val_dual_gap(const OSQPSolver* solver, const OSQPVectorf* x, const OSQPVectorf* y,
-                              OSQPFloat* prim_obj_val, OSQPFloat* dual_obj_val, OSQPFloat* duality_gap) {
+/* auxil.c:231-285, in two parts: the dots are queued (obj_terms_queue), then
+ * combined once they have arrived (obj_terms_finish). */
+typedef struct {
+  OSQPFloat quad, lin, sup_u, sup_l;
+} ObjTerms;
+
+static void obj_terms_queue(const OSQPSolver* solver, const OSQPVectorf* x, const OSQPVectorf* y, ObjTerms* t) {
   OSQPWorkspace* work = solver->work;
-  OSQPFloat quad = OSQPVectorf_dot_prod(work->Px, x);
-  OSQPFloat lin = OSQPVectorf_dot_prod(work->data->q, x);
-  OSQPFloat sup;
+  osqp_amd_red_dot_prod(work->Px, x, &t->quad);
+  osqp_amd_red_dot_prod(work->data->q, x, &t->lin);
   OSQPVectorf_copy(work->z_prev, y);
   OSQPVectorf_project_polar_reccone(work->z_prev, work->data->l, work->data->u, OSQP_INFTY * OSQP_MIN_SCALING);
   OSQPVectorf_round_to_zero(work->z_prev, OSQP_ZERO_DEADZONE);
-  sup = OSQPVectorf_dot_prod_signed(work->data->u, work->z_prev, +1);
-  sup += OSQPVectorf_dot_prod_signed(work->data->l, work->z_prev, -1);
+  osqp_amd_red_dot_prod_signed(work->data->u, work->z_prev, +1, &t->sup_u);
+  osqp_amd_red_dot_prod_signed(work->data->l, work->z_prev, -1, &t->sup_l);
+}
+
+static void obj_terms_finish(const OSQPSolver* solver, const ObjTerms* t, OSQPFloat* prim_obj_val,
+                             OSQPFloat* dual_obj_val, OSQPFloat* duality_gap) {
+  OSQPWorkspace* work = solver->work;
+  const OSQPFloat quad = t->quad, lin = t->lin;
+  OSQPFloat sup = t->sup_u;
+  sup += t->sup_l;
   *prim_obj_val = 0.5 * quad + lin;
   *dual_obj_val = -0.5 * quad - sup;
   work->scaled_dual_gap = quad + lin + sup;
@@ -202,6 +220,14 @@ void compute_obj_val_dual_gap(const OSQPSolver* solver, const OSQPVectorf* x, co
   work->xtPx = quad;
   work->qtx = lin;
   work->SC = sup;
+}
+
+void compute_obj_val_dual_gap(const OSQPSolver* solver, const OSQPVectorf* x, const OSQPVectorf* y,
+                              OSQPFloat* prim_obj_val, OSQPFloat* dual_obj_val, OSQPFloat* duality_gap) {
+  ObjTerms t;
+  obj_terms_queue(solver, x, y, &t);
+  osqp_amd_red_wait();
+  obj_terms_finish(solver, &t, prim_obj_val, dual_obj_val, duality_gap);
 }
 
 static OSQPInt unscaled_termination(const OSQPSolver* s) {
@@ -232,6 +258,7 @@ static OSQPFloat compute_prim_res(OSQPSolver* solver, const OSQPVectorf* x, cons
 static OSQPFloat compute_prim_tol(const OSQPSolver* solver, OSQPFloat eps_abs, OSQPFloat eps_rel) {
   OSQPWorkspace* work = solver->work;
   OSQPFloat a, b;
+  if (work->tol_valid) return eps_abs + eps_rel * work->tol_prim_norm;
   if (unscaled_termination(solver)) {
     a = OSQPVectorf_scaled_norm_inf(work->scaling->Einv, work->z);
     b = OSQPVectorf_scaled_norm_inf(work->scaling->Einv, work->Ax);
@@ -262,6 +289,7 @@ static OSQPFloat compute_dual_res(OSQPSolver* solver, const OSQPVectorf* x, cons
 static OSQPFloat compute_dual_tol(const OSQPSolver* solver, OSQPFloat eps_abs, OSQPFloat eps_rel) {
   OSQPWorkspace* work = solver->work;
   OSQPFloat r, t;
+  if (work->tol_valid) return eps_abs + eps_rel * work->tol_dual_norm;
   if (unscaled_termination(solver)) {
     r = OSQPVectorf_scaled_norm_inf(work->scaling->Dinv, work->data->q);
     t = OSQPVectorf_scaled_norm_inf(work->scaling->Dinv, work->Aty);
@@ -378,14 +406,73 @@ void store_solution(OSQPSolver* solver, OSQPSolution* solution) {
 }
 
 /* auxil.c:676-762 (non-polishing branch) */
+/* The same vector operations as compute_prim_res, compute_dual_res,
+ * compute_obj_val_dual_gap, compute_prim_tol and compute_dual_tol, in the
+ * same order, with their thirteen reductions queued and waited for once
+ * (one host round trip instead of thirteen; the same kernels, so the same
+ * bits).  The tolerance norms are kept for check_termination. */
 void update_info(OSQPSolver* solver, OSQPInt iter) {
   OSQPInfo* info = solver->info;
   OSQPWorkspace* work = solver->work;
+  const OSQPInt m = work->data->m, uns = unscaled_termination(solver);
+  const OSQPVectorf *x = work->x, *y = work->y, *z = work->z;
+  OSQPFloat pr = 0., pr_s = 0., du = 0., du_s = 0., ta = 0., tb = 0., tq = 0., tAty = 0., tPx = 0.;
+  ObjTerms ot;
   info->iter = iter;
-  if (work->data->m == 0) info->prim_res = 0.;
-  else info->prim_res = compute_prim_res(solver, work->x, work->z);
-  info->dual_res = compute_dual_res(solver, work->x, work->y);
-  compute_obj_val_dual_gap(solver, work->x, work->y, &info->obj_val, &info->dual_obj_val, &info->duality_gap);
+  /* compute_prim_res */
+  if (m) {
+    OSQPMatrix_Axpy(work->data->A, x, work->Ax, 1.0, 0.0);
+    OSQPVectorf_minus(work->z_prev, work->Ax, z);
+    osqp_amd_red_norm_inf(work->z_prev, &pr);
+    if (uns) osqp_amd_red_scaled_norm_inf(work->scaling->Einv, work->z_prev, &pr_s);
+  }
+  /* compute_dual_res */
+  OSQPVectorf_copy(work->x_prev, work->data->q);
+  OSQPMatrix_Axpy(work->data->P, x, work->Px, 1.0, 0.0);
+  OSQPVectorf_plus(work->x_prev, work->x_prev, work->Px);
+  if (m) {
+    OSQPMatrix_Atxpy(work->data->A, y, work->Aty, 1.0, 0.0);
+    OSQPVectorf_plus(work->x_prev, work->x_prev, work->Aty);
+  }
+  osqp_amd_red_norm_inf(work->x_prev, &du);
+  if (uns) osqp_amd_red_scaled_norm_inf(work->scaling->Dinv, work->x_prev, &du_s);
+  /* compute_obj_val_dual_gap */
+  obj_terms_queue(solver, x, y, &ot);
+  /* compute_prim_tol / compute_dual_tol */
+  if (uns) {
+    if (m) {
+      osqp_amd_red_scaled_norm_inf(work->scaling->Einv, z, &ta);
+      osqp_amd_red_scaled_norm_inf(work->scaling->Einv, work->Ax, &tb);
+    }
+    osqp_amd_red_scaled_norm_inf(work->scaling->Dinv, work->data->q, &tq);
+    osqp_amd_red_scaled_norm_inf(work->scaling->Dinv, work->Aty, &tAty);
+    osqp_amd_red_scaled_norm_inf(work->scaling->Dinv, work->Px, &tPx);
+  } else {
+    if (m) {
+      osqp_amd_red_norm_inf(z, &ta);
+      osqp_amd_red_norm_inf(work->Ax, &tb);
+    }
+    osqp_amd_red_norm_inf(work->data->q, &tq);
+    osqp_amd_red_norm_inf(work->Aty, &tAty);
+    osqp_amd_red_norm_inf(work->Px, &tPx);
+  }
+  osqp_amd_red_wait();
+  if (m) {
+    work->scaled_prim_res = pr;
+    info->prim_res = uns ? pr_s : pr;
+  } else {
+    info->prim_res = 0.;
+  }
+  work->scaled_dual_res = du;
+  info->dual_res = uns ? work->scaling->cinv * du_s : du;
+  obj_terms_finish(solver, &ot, &info->obj_val, &info->dual_obj_val, &info->duality_gap);
+  work->tol_prim_norm = c_max(ta, tb);
+  {
+    OSQPFloat r = c_max(tq, tAty);
+    r = c_max(r, tPx);
+    work->tol_dual_norm = uns ? r * work->scaling->cinv : r;
+  }
+  work->tol_valid = 1;
   info->primdual_int += c_absval(info->duality_gap);
   info->solve_time = osqp_amd_toc(work);
   info->rel_kkt_error = c_max(c_max(info->dual_res, info->prim_res), info->duality_gap);

@@ -51,6 +51,10 @@ struct OSQPWorkspace_ {
   OSQPVectorf *D_temp, *D_temp_A, *E_temp;
   OSQPScaling* scaling;
   OSQPFloat    scaled_prim_res, scaled_dual_res;
+  /* the tolerance norms of the last update_info (max of the two / three
+   * norms in compute_prim_tol / compute_dual_tol), valid while tol_valid */
+  OSQPFloat    tol_prim_norm, tol_dual_norm;
+  OSQPInt      tol_valid;
   OSQPFloat    rho_inv;
   struct timespec tic;
   OSQPInt first_run, clear_update_time, rho_update_from_solve;

@@ -318,6 +318,7 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
   for (iter = 1; iter <= settings->max_iter; iter++) {
     swap_vectors(&(work->x), &(work->x_prev));
     swap_vectors(&(work->z), &(work->z_prev));
+    work->tol_valid = 0;  /* the iterates change: update_info recomputes the tolerance norms */
     update_xz_tilde(solver, iter);
     if (settings->rho_is_vec) {
       update_xzy(solver);
