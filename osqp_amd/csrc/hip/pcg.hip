@@ -467,39 +467,41 @@ __device__ inline int loop_go(const PcgState* st, double rty, double rn, int j, 
   return go;
 }
 
-// head of CG iteration 0, after the defect-correction init; one block
-__global__ void k_loop_first(PcgState* st, const double* part_rty, const double* part_rn, int nparts, int max_iter,
-                             int guard, int* flags) {
-  __shared__ double lds[kBlock / kWave];
-  if (!st->active) {
-    if (threadIdx.x == 0) {
-      st->done = 1;
-      if (flags) flags[0] = 2 * st->solves;
-    }
-    return;
-  }
-  const double rty = parts_sum(part_rty, nparts, lds);
-  const double rn = parts_max(part_rn, nparts, lds);
-  if (threadIdx.x != 0) return;
-  st->rty[0] = rty;
-  st->rnorm = rn;
-  const int go = loop_go(st, rty, rn, 0, max_iter, guard);
-  st->active = go;
-  st->done = !go;
-  if (flags) flags[0] = 2 * st->solves + go;
-}
-
 // CG iteration j: alpha = r'y / p'Kp from the operator's p'Kp partials; then
 // d += alpha p; r += alpha Kp; y = M^-1 r; partials r'y, max|r|.  On an FP32
 // breakdown (p'Kp not positive and finite) every block skips the update and
-// block 0 stops the loop.
+// block 0 stops the loop.  Iteration 0 also runs the loop head of the
+// defect-correction init (r'y and max|r| from its partials init_rty /
+// init_rn, the stop test, flag 0), so no one-block kernel sits between the
+// init and the first operator; when that head stops the loop, the operator
+// of iteration 0 has run for nothing and the update is skipped.
 template <typename V>
 __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, int guard, int n, V* x, V* r, V* y,
-                         const V* p, const V* Kp, const V* dinv, double* part_rty, double* part_rn) {
+                         const V* p, const V* Kp, const V* dinv, double* part_rty, double* part_rn,
+                         const double* init_rty, const double* init_rn, int ninit, int max_iter, int* flags) {
   if (!st->active) return;
   __shared__ double lds[kBlock / kWave];
+  double rty_j;
+  if (j == 0) {
+    const double rty0 = parts_sum(init_rty, ninit, lds);
+    const double rn0 = parts_max(init_rn, ninit, lds);
+    const int go = loop_go(st, rty0, rn0, 0, max_iter, guard);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rty[0] = rty0;
+      st->rnorm = rn0;
+      if (!go) {
+        st->active = 0;
+        st->done = 1;
+      }
+      if (flags) flags[0] = 2 * st->solves + go;
+    }
+    if (!go) return;
+    rty_j = rty0;
+  } else {
+    rty_j = st->rty[j & 1];
+  }
   const double pkp = parts_sum(part_pkp, npkp, lds);
-  const double alpha64 = st->rty[j & 1] / pkp;
+  const double alpha64 = rty_j / pkp;
   const bool broke = guard && !(pkp > 0.0 && isfinite(pkp));
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pKp = pkp;
@@ -627,7 +629,7 @@ struct HipPcg {
   void *d, *p, *Kp, *r, *y, *w, *dinv;
   double* dtmp;       // FP64 n-vector (preconditioner diagonal)
   PcgState* st;
-  double* parts;      // 3 * kMaxReduceBlocks partial slots
+  double* parts;      // 5 * kMaxReduceBlocks partial slots
   PcgState* h_st;     // pinned mirror for counters
   int* h_flags;       // host-coherent per-iteration loop flags
   int* d_flags;       // device address of h_flags
@@ -985,6 +987,8 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   double* part0 = s->parts;
   double* part1 = s->parts + kMaxReduceBlocks;
   double* part2 = s->parts + 2 * kMaxReduceBlocks;
+  double* part3 = s->parts + 3 * kMaxReduceBlocks;  // the init's r'y / max|r| partials
+  double* part4 = s->parts + 4 * kMaxReduceBlocks;
   const int gK = s->k_nb < kMaxReduceBlocks ? s->k_nb : kMaxReduceBlocks;
   const int gE = grid_for(n, kBlock, kMaxReduceBlocks);
   const int gM = grid_for(m, kBlock, 4096);
@@ -1024,9 +1028,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
     nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)b->d, s->r64, part2);
   }
   hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nres, n,
-                     (const double*)s->r64, dinv, r, y, p, d, part0, part1);
-  hipLaunchKernelGGL(k_loop_first, dim3(1), dim3(kBlock), 0, S(), s->st, (const double*)part0, (const double*)part1,
-                     gE, s->max_iter, guard, s->d_flags);
+                     (const double*)s->r64, dinv, r, y, p, d, part3, part4);
   // one CG iteration: the operator (two SpMVs with their combines), the
   // update, and the next loop head fused with the search-direction update.
   // With lookahead 0 the first launch of iteration it + 1 (A's panel SpMV) is
@@ -1063,7 +1065,8 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
     }
     const int nk = op_kv<V>(s, (const V*)p, (const V*)w, (const double*)nullptr, Kp, part2);
     hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nk, it, guard, n,
-                       d, r, y, (const V*)p, (const V*)Kp, dinv, part0, part1);
+                       d, r, y, (const V*)p, (const V*)Kp, dinv, part0, part1, (const double*)part3,
+                       (const double*)part4, gE, s->max_iter, s->d_flags);
     hipLaunchKernelGGL(k_head_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part0,
                        (const double*)part1, gE, it + 1, s->max_iter, guard, s->d_flags, n, p, (const V*)y);
     if (ahead && it + 1 < s->max_iter) {
@@ -1241,7 +1244,7 @@ OSQPInt osqp_algebra_init_linsys_solver(LinSysSolver** sp, const OSQPMatrix* P, 
   s->dinv = dev_alloc(vb * n);
   s->dtmp = (double*)dev_alloc(sizeof(double) * n);
   s->st = (PcgState*)dev_alloc(sizeof(PcgState));
-  s->parts = (double*)dev_alloc(sizeof(double) * 3 * kMaxReduceBlocks);
+  s->parts = (double*)dev_alloc(sizeof(double) * 5 * kMaxReduceBlocks);
   if (!s->x64 || !s->rhs64 || !s->r64 || !s->w64 || !s->ax64 || !s->d || !s->p || !s->Kp || !s->r || !s->y ||
       !s->w || !s->dinv || !s->dtmp || !s->st || !s->parts) {
     pcg_free(&s->base);
