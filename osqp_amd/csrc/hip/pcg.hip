@@ -1037,12 +1037,13 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   //
   // The solve's tail (x64 += scale d, then z = A x64) may be queued before
   // the host knows the loop has stopped: when iteration it + 1 is where the
-  // previous solve's loop stopped, the tail goes right behind that look-ahead
-  // SpMV, each launch guarded on the device by the loop's `done` flag.  If the
-  // loop stops there, the GPU runs the tail while the host is still reacting;
-  // if it goes on, the three launches exit at once and the tail is queued
-  // unguarded after the loop.  host_should_stop's answers are definitive, so
-  // the host knows which of the two happened.
+  // previous solve's loop stopped, the tail is queued in place of the
+  // look-ahead SpMV, each launch guarded on the device by the loop's `done`
+  // flag.  If the loop stops there, the GPU runs the tail while the host is
+  // still reacting; if it goes on, the three launches exit at once, the host
+  // queues the SpMV it left out, and the tail is queued unguarded after the
+  // loop.  host_should_stop's answers are definitive, so the host knows which
+  // of the two happened.
   const bool ahead = s->lookahead == 0 && m > 0;
   const bool spec = ahead && s->h_flags && panels_A<double>(s) != nullptr;
   const int* done = (const int*)((const char*)s->st + offsetof(PcgState, done));
@@ -1059,6 +1060,8 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
       break;
     }
     if (ahead) {
+      // the look-ahead SpMV was left out where the loop was predicted to stop
+      if (it == spec_at) av_spmv<V>(s, (const V*)p, w);
       av_finish<V>(s, w);
     } else if (m) {
       op_av_rho<V>(s, (const V*)p, w);
@@ -1070,10 +1073,13 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
     hipLaunchKernelGGL(k_head_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part0,
                        (const double*)part1, gE, it + 1, s->max_iter, guard, s->d_flags, n, p, (const V*)y);
     if (ahead && it + 1 < s->max_iter) {
-      av_spmv<V>(s, (const V*)p, w);
       if (spec && it + 1 == s->last_cg) {
+        // predicted stop: the tail instead of the look-ahead SpMV (a skipped
+        // SpMV launch would sit between the loop head and the tail)
         tail(true);
         spec_at = it + 1;
+      } else {
+        av_spmv<V>(s, (const V*)p, w);
       }
     }
   }
