@@ -81,6 +81,9 @@ __device__ inline T wave_max(T v) {
   return v;
 }
 
+// Block totals: every thread combines the wavefront totals itself, in
+// wavefront order, so all threads get the same bits without a broadcast
+// round; the final barrier lets the caller reuse lds at once.
 template <typename T>
 __device__ inline T block_sum(T v, T* lds) {
   v = wave_sum(v);
@@ -88,12 +91,7 @@ __device__ inline T block_sum(T v, T* lds) {
   if (lane == 0) lds[w] = v;
   __syncthreads();
   T r = 0;
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) r += lds[i];
-    lds[0] = r;
-  }
-  __syncthreads();
-  r = lds[0];
+  for (int i = 0; i < (int)(blockDim.x / kWave); ++i) r += lds[i];
   __syncthreads();
   return r;
 }
@@ -104,14 +102,8 @@ __device__ inline T block_max(T v, T* lds) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   if (lane == 0) lds[w] = v;
   __syncthreads();
-  T r = 0;
-  if (threadIdx.x == 0) {
-    r = lds[0];
-    for (int i = 1; i < (int)(blockDim.x / kWave); ++i) r = lds[i] > r ? lds[i] : r;
-    lds[0] = r;
-  }
-  __syncthreads();
-  r = lds[0];
+  T r = lds[0];
+  for (int i = 1; i < (int)(blockDim.x / kWave); ++i) r = lds[i] > r ? lds[i] : r;
   __syncthreads();
   return r;
 }
