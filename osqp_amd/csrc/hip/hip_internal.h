@@ -108,6 +108,25 @@ __device__ inline T block_max(T v, T* lds) {
   return r;
 }
 
+// block_sum(s) and block_max(m) in one round (lds holds 2 * blockDim.x / 64
+// values); the same bits as the two calls.
+__device__ inline void block_sum_max(double& s, double& m, double* lds) {
+  s = wave_sum(s);
+  m = wave_max(m);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = (int)(blockDim.x / kWave);
+  if (lane == 0) {
+    lds[w] = s;
+    lds[nw + w] = m;
+  }
+  __syncthreads();
+  double rs = 0, rm = lds[nw];
+  for (int i = 0; i < nw; ++i) rs += lds[i];
+  for (int i = 1; i < nw; ++i) rm = lds[nw + i] > rm ? lds[nw + i] : rm;
+  __syncthreads();
+  s = rs;
+  m = rm;
+}
+
 // Sum of a group of G consecutive lanes (G a power of two <= 64); every lane
 // of the group receives the total.
 template <int G, typename T>

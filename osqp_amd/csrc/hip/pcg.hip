@@ -378,6 +378,19 @@ __device__ inline double parts_max(const double* part, int nparts, double* lds) 
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) mx = part[i] > mx ? part[i] : mx;
   return block_max(mx, lds);
 }
+// parts_sum(a) and parts_max(b) in one pass and one block round (lds: 2 *
+// blockDim.x / 64 values); the same bits as the two calls
+__device__ inline void parts_sum_max(const double* a, const double* b, int nparts, double* lds, double* sum,
+                                     double* mx) {
+  double s = 0.0, m = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    s += a[i];
+    m = b[i] > m ? b[i] : m;
+  }
+  block_sum_max(s, m, lds);
+  *sum = s;
+  *mx = m;
+}
 __device__ inline double parts_sum(const double* part, int nparts, double* lds) {
   double s = 0.0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
@@ -409,7 +422,8 @@ __global__ void k_tol_w(PcgState* st, int admm_iter, double spr, double sdr, dou
 template <typename V>
 __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int n, const double* r64, const V* dinv,
                              V* r, V* y, V* p, V* d, double* part_rty, double* part_rn) {
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds2[2 * kBlock / kWave];
+  double* lds = lds2;
   const double rn0 = parts_max(part_res, nres, lds);
   const double scale = (rn0 > 0.0 && isfinite(rn0)) ? rn0 : 1.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -430,8 +444,8 @@ __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int
     double a = fabs((double)ri);
     mx = a > mx ? a : mx;
   }
-  double t1 = block_sum(dot.value(), lds);
-  double t2 = block_max(mx, lds);
+  double t1 = dot.value(), t2 = mx;
+  block_sum_max(t1, t2, lds2);
   if (threadIdx.x == 0) {
     part_rty[blockIdx.x] = t1;
     part_rn[blockIdx.x] = t2;
@@ -480,11 +494,12 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
                          const V* p, const V* Kp, const V* dinv, double* part_rty, double* part_rn,
                          const double* init_rty, const double* init_rn, int ninit, int max_iter, int* flags) {
   if (!st->active) return;
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds2[2 * kBlock / kWave];
+  double* lds = lds2;
   double rty_j;
   if (j == 0) {
-    const double rty0 = parts_sum(init_rty, ninit, lds);
-    const double rn0 = parts_max(init_rn, ninit, lds);
+    double rty0, rn0;
+    parts_sum_max(init_rty, init_rn, ninit, lds2, &rty0, &rn0);
     const int go = loop_go(st, rty0, rn0, 0, max_iter, guard);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->rty[0] = rty0;
@@ -525,8 +540,8 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
     double a = fabs((double)ri);
     mx = a > mx ? a : mx;
   }
-  double t1 = block_sum(dot.value(), lds);
-  double t2 = block_max(mx, lds);
+  double t1 = dot.value(), t2 = mx;
+  block_sum_max(t1, t2, lds2);
   if (threadIdx.x == 0) {
     part_rty[blockIdx.x] = t1;
     part_rn[blockIdx.x] = t2;
@@ -538,13 +553,14 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
 template <typename V>
 __global__ void k_head_pupdate(PcgState* st, const double* part_rty, const double* part_rn, int nparts, int j,
                                int max_iter, int guard, int* flags, int n, V* p, const V* y) {
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds2[2 * kBlock / kWave];
+  double* lds = lds2;
   if (!st->active) {
     if (flags && blockIdx.x == 0 && threadIdx.x == 0) flags[j] = 2 * st->solves;
     return;
   }
-  const double rty = parts_sum(part_rty, nparts, lds);
-  const double rn = parts_max(part_rn, nparts, lds);
+  double rty, rn;
+  parts_sum_max(part_rty, part_rn, nparts, lds2, &rty, &rn);
   const double beta64 = rty / st->rty[(j - 1) & 1];
   const int go = loop_go(st, rty, rn, j, max_iter, guard);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
