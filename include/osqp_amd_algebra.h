@@ -179,9 +179,11 @@ void       osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* de
 /* Queued reductions (no counterpart in the reference, whose reductions each
  * return to the host): each call queues the same reduction as
  * OSQPVectorf_norm_inf / scaled_norm_inf / dot_prod / dot_prod_signed (same
- * kernel, same bits) and returns at once; osqp_amd_red_wait() waits for all
- * queued ones with one host round trip and stores each result in its *dst.
- * Up to 31 may be queued (a further call waits first). */
+ * arithmetic, same bits) and returns at once; osqp_amd_red_wait() runs all
+ * queued ones in one launch with one host round trip and stores each result
+ * in its *dst.  A queued reduction reads its vectors when the queue runs, so
+ * queue it only once its inputs are final.  Up to 16 may be queued (a
+ * further call runs the queue first). */
 void       osqp_amd_red_norm_inf(const OSQPVectorf* v, OSQPFloat* dst);
 void       osqp_amd_red_scaled_norm_inf(const OSQPVectorf* S, const OSQPVectorf* v, OSQPFloat* dst);
 void       osqp_amd_red_dot_prod(const OSQPVectorf* a, const OSQPVectorf* b, OSQPFloat* dst);

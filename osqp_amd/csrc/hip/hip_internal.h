@@ -27,10 +27,6 @@ struct Runtime {
   double* d_partials = nullptr;    // per-block partials (4 * kMaxReduceBlocks doubles)
   unsigned* d_ticket = nullptr;    // finished-block counter of the running reduction
   double red_seq = 0.0;            // sequence number of the last reduction
-  // queued reductions: slot k (1..kRedSlots) of h_scalars holds [value, seq]
-  int red_queued = 0;
-  double* red_dst[32] = {};
-  double red_slot_seq[32] = {};
   int last_error = 0;
   int num_cu = 0;
   char device_name[256] = {0};

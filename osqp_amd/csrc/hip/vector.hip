@@ -140,19 +140,19 @@ __device__ inline double red_elem(int op, const RedArgs& r, long long i) {
 // timing, so results are deterministic -- writes the result to host-visible
 // memory (out[0], then the sequence number `seq` in out[1]) and re-arms the
 // ticket for the next reduction on the stream.
-__device__ void finish_reduce(bool mx, double t, double* partials, unsigned* ticket, double* out, double seq,
-                              double* lds) {
+__device__ void finish_reduce_n(bool mx, double t, double* partials, unsigned* ticket, double* out, double seq,
+                                double* lds, int nblocks) {
   __shared__ int last;
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = t;
     __threadfence();  // the partial is visible device-wide before the ticket
-    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    last = atomicAdd(ticket, 1u) == (unsigned)nblocks - 1;
     if (last) __threadfence();  // every other block's partial is visible here
   }
   __syncthreads();
   if (!last) return;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
     const double v = partials[i];
     if (mx) acc = v > acc ? v : acc;
     else acc += v;
@@ -164,6 +164,11 @@ __device__ void finish_reduce(bool mx, double t, double* partials, unsigned* tic
     __threadfence_system();  // the result reaches the host before its sequence number
     ((volatile double*)out)[1] = seq;
   }
+}
+
+__device__ void finish_reduce(bool mx, double t, double* partials, unsigned* ticket, double* out, double seq,
+                              double* lds) {
+  finish_reduce_n(mx, t, partials, ticket, out, seq, lds, (int)gridDim.x);
 }
 
 // A thread takes elements i0, i0 + S, i0 + 2S, ... (S = the grid's thread
@@ -193,6 +198,47 @@ __global__ void k_reduce(int op, RedArgs r, int n, double* partials, unsigned* t
   finish_reduce(mx, t, partials, ticket, out, seq, lds);
 }
 
+// Several reductions in one launch: row k of the grid (blockIdx.y) is
+// reduction k, run exactly as k_reduce would run it alone on a grid of
+// t[k].g blocks (the same elements per lane, the same order, so the same
+// bits), with its own partials, ticket and result slot k + 1.
+constexpr int kMultiMax = 16;
+struct RedTask {
+  int op, n, g, pad;
+  RedArgs r;
+};
+struct RedBatch {
+  RedTask t[kMultiMax];
+};
+
+__global__ void k_reduce_multi(RedBatch B, double* partials, unsigned* tickets, double* out, double seq0) {
+  const int k = blockIdx.y;
+  const int op = B.t[k].op, n = B.t[k].n, g = B.t[k].g;
+  const RedArgs r = B.t[k].r;
+  if ((int)blockIdx.x >= g) return;
+  __shared__ double lds[kBlock / kWave];
+  const bool mx = is_max_op(op);
+  const long long S = (long long)g * blockDim.x;
+  double acc = 0.0;
+  for (long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += kRedUnroll * S) {
+    double v[kRedUnroll];
+#pragma unroll
+    for (int u = 0; u < kRedUnroll; ++u) {
+      const long long i = i0 + u * S;
+      v[u] = i < n ? red_elem(op, r, i) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kRedUnroll; ++u) {
+      if (i0 + u * S < n) {
+        if (mx) acc = v[u] > acc ? v[u] : acc;
+        else acc += v[u];
+      }
+    }
+  }
+  const double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
+  finish_reduce_n(mx, t, partials + (long long)k * 512, tickets + 1 + k, out + 2 * (k + 1), seq0 + k, lds, g);
+}
+
 // The host waits for reduction `seq` by watching its sequence slot (a stream
 // synchronisation sleeps and wakes tens of microseconds late, and the
 // termination checks run a dozen reductions back to back).  After half a
@@ -213,12 +259,11 @@ double wait_result(Runtime& R, double seq) {
   return h[0];
 }
 
-// Queued reductions: kernel k writes [value, seq] to slot k of the
-// host-coherent scalars.  Kernels on the stream complete in order and each
-// fences its value before its sequence number, so once the last queued
-// sequence number is visible every queued value is.
-constexpr int kRedSlots = 31;  // h_scalars holds 32 [value, seq] slots; slot 0 is the synchronous one
-
+// Queued reductions are recorded on the host and run together, in one
+// k_reduce_multi launch, when they are waited for: each reads its vectors as
+// they are at that moment, so a caller queues a reduction only once its
+// inputs are final.  Reduction k writes [value, seq] to slot k + 1 of the
+// host-coherent scalars (slot 0 is the synchronous path's).
 inline int red_grid(int n) {
   // about one element per lane per unrolled round: few blocks, so few
   // ticket atomics on the one counter
@@ -226,24 +271,41 @@ inline int red_grid(int n) {
   return (int)(gg < 1 ? 1 : (gg > 512 ? 512 : gg));
 }
 
+struct RedQueue {
+  RedBatch b;
+  double* dst[kMultiMax];
+  int count = 0;
+};
+RedQueue g_redq;
+
 void red_drain(Runtime& R) {
-  const int k = R.red_queued;
+  RedQueue& Q = g_redq;
+  const int k = Q.count;
   if (k == 0) return;
-  const double last = R.red_slot_seq[k - 1];
+  Q.count = 0;
+  int gmax = 1;
+  for (int j = 0; j < k; ++j) gmax = Q.b.t[j].g > gmax ? Q.b.t[j].g : gmax;
+  const double seq0 = R.red_seq + 1;
+  R.red_seq += k;
+  hipLaunchKernelGGL(k_reduce_multi, dim3(gmax, k), dim3(kBlock), 0, R.stream, Q.b, R.d_partials, R.d_ticket,
+                     R.dh_scalars, seq0);
+  if (LAUNCHCHK()) {
+    for (int j = 0; j < k; ++j) *Q.dst[j] = 0.0;
+    return;
+  }
   volatile double* h = R.h_scalars;
   struct timespec t0, t;
   clock_gettime(CLOCK_MONOTONIC, &t0);
-  bool seen = false;
-  for (long it = 1; !seen; ++it) {
-    if (h[2 * k + 1] == last) seen = true;
-    else if ((it & 1023) == 0) {
+  int seen = 0;
+  for (long it = 1; seen < k; ++it) {
+    while (seen < k && h[2 * (seen + 1) + 1] == seq0 + seen) ++seen;
+    if (seen < k && (it & 1023) == 0) {
       clock_gettime(CLOCK_MONOTONIC, &t);
       if ((t.tv_sec - t0.tv_sec) + 1e-9 * (t.tv_nsec - t0.tv_nsec) > 0.5) break;
     }
   }
-  if (!seen) HIPCHK(hipStreamSynchronize(R.stream));
-  for (int j = 0; j < k; ++j) *R.red_dst[j] = h[2 * (j + 1)];
-  R.red_queued = 0;
+  if (seen < k) HIPCHK(hipStreamSynchronize(R.stream));
+  for (int j = 0; j < k; ++j) *Q.dst[j] = h[2 * (j + 1)];
 }
 
 void red_queue(int op, const RedArgs& r, int n, double* dst) {
@@ -252,18 +314,15 @@ void red_queue(int op, const RedArgs& r, int n, double* dst) {
     *dst = 0.0;
     return;
   }
-  if (R.red_queued == kRedSlots) red_drain(R);
-  const int k = R.red_queued;
-  const double seq = ++R.red_seq;
-  hipLaunchKernelGGL(k_reduce, dim3(red_grid(n)), dim3(kBlock), 0, R.stream, op, r, n, R.d_partials, R.d_ticket,
-                     R.dh_scalars + 2 * (k + 1), seq);
-  if (LAUNCHCHK()) {
-    *dst = 0.0;
-    return;
-  }
-  R.red_dst[k] = dst;
-  R.red_slot_seq[k] = seq;
-  R.red_queued = k + 1;
+  RedQueue& Q = g_redq;
+  if (Q.count == kMultiMax) red_drain(R);
+  RedTask& T = Q.b.t[Q.count];
+  T.op = op;
+  T.n = n;
+  T.g = red_grid(n);
+  T.pad = 0;
+  T.r = r;
+  Q.dst[Q.count++] = dst;
 }
 
 // one kernel; the host reads the result from host-coherent memory

@@ -190,15 +190,17 @@ typedef struct {
   OSQPFloat quad, lin, sup_u, sup_l;
 } ObjTerms;
 
-static void obj_terms_queue(const OSQPSolver* solver, const OSQPVectorf* x, const OSQPVectorf* y, ObjTerms* t) {
+/* scratch: an m-vector for the projection of y (the reference uses z_prev) */
+static void obj_terms_queue(const OSQPSolver* solver, const OSQPVectorf* x, const OSQPVectorf* y,
+                            OSQPVectorf* scratch, ObjTerms* t) {
   OSQPWorkspace* work = solver->work;
   osqp_amd_red_dot_prod(work->Px, x, &t->quad);
   osqp_amd_red_dot_prod(work->data->q, x, &t->lin);
-  OSQPVectorf_copy(work->z_prev, y);
-  OSQPVectorf_project_polar_reccone(work->z_prev, work->data->l, work->data->u, OSQP_INFTY * OSQP_MIN_SCALING);
-  OSQPVectorf_round_to_zero(work->z_prev, OSQP_ZERO_DEADZONE);
-  osqp_amd_red_dot_prod_signed(work->data->u, work->z_prev, +1, &t->sup_u);
-  osqp_amd_red_dot_prod_signed(work->data->l, work->z_prev, -1, &t->sup_l);
+  OSQPVectorf_copy(scratch, y);
+  OSQPVectorf_project_polar_reccone(scratch, work->data->l, work->data->u, OSQP_INFTY * OSQP_MIN_SCALING);
+  OSQPVectorf_round_to_zero(scratch, OSQP_ZERO_DEADZONE);
+  osqp_amd_red_dot_prod_signed(work->data->u, scratch, +1, &t->sup_u);
+  osqp_amd_red_dot_prod_signed(work->data->l, scratch, -1, &t->sup_l);
 }
 
 static void obj_terms_finish(const OSQPSolver* solver, const ObjTerms* t, OSQPFloat* prim_obj_val,
@@ -225,7 +227,7 @@ static void obj_terms_finish(const OSQPSolver* solver, const ObjTerms* t, OSQPFl
 void compute_obj_val_dual_gap(const OSQPSolver* solver, const OSQPVectorf* x, const OSQPVectorf* y,
                               OSQPFloat* prim_obj_val, OSQPFloat* dual_obj_val, OSQPFloat* duality_gap) {
   ObjTerms t;
-  obj_terms_queue(solver, x, y, &t);
+  obj_terms_queue(solver, x, y, solver->work->z_prev, &t);
   osqp_amd_red_wait();
   obj_terms_finish(solver, &t, prim_obj_val, dual_obj_val, duality_gap);
 }
@@ -408,9 +410,13 @@ void store_solution(OSQPSolver* solver, OSQPSolution* solution) {
 /* auxil.c:676-762 (non-polishing branch) */
 /* The same vector operations as compute_prim_res, compute_dual_res,
  * compute_obj_val_dual_gap, compute_prim_tol and compute_dual_tol, in the
- * same order, with their thirteen reductions queued and waited for once
- * (one host round trip instead of thirteen; the same kernels, so the same
- * bits).  The tolerance norms are kept for check_termination. */
+ * same order, with their thirteen reductions queued and run together (one
+ * launch and one host round trip instead of thirteen; the same arithmetic,
+ * so the same bits).  Queued reductions read their vectors when they run,
+ * so the projection of y for the objective terms goes to Adelta_x (free
+ * here; the infeasibility checks overwrite it before reading) instead of
+ * z_prev, which still holds A x - z.  The tolerance norms are kept for
+ * check_termination. */
 void update_info(OSQPSolver* solver, OSQPInt iter) {
   OSQPInfo* info = solver->info;
   OSQPWorkspace* work = solver->work;
@@ -437,7 +443,7 @@ void update_info(OSQPSolver* solver, OSQPInt iter) {
   osqp_amd_red_norm_inf(work->x_prev, &du);
   if (uns) osqp_amd_red_scaled_norm_inf(work->scaling->Dinv, work->x_prev, &du_s);
   /* compute_obj_val_dual_gap */
-  obj_terms_queue(solver, x, y, &ot);
+  obj_terms_queue(solver, x, y, work->Adelta_x, &ot);
   /* compute_prim_tol / compute_dual_tol */
   if (uns) {
     if (m) {
