@@ -168,14 +168,17 @@ OSQPInt    osqp_amd_linsys_bytes(LinSysSolver* s, double* per_cg_iter, double* p
 /* Fused ADMM steps for the rho-vector case (no counterpart in the reference,
  * which issues the same arithmetic as separate vector calls; results are
  * bit-identical): compute_rhs (src/auxil.c:136-158) and update_x / update_z /
- * update_y (src/auxil.c:172-229) in one pass each. */
+ * update_y (src/auxil.c:172-229) in one pass each.  With next = 1 the update
+ * also writes the next iteration's compute_rhs result (from the new x, z, y,
+ * with sigma and q) over xt / zt, valid while rho, sigma and q stay. */
 void       osqp_amd_admm_rhs(OSQPVectorf* xt, OSQPVectorf* zt, OSQPFloat sigma, const OSQPVectorf* x_prev,
                              const OSQPVectorf* q, const OSQPVectorf* rho_inv, const OSQPVectorf* y,
                              const OSQPVectorf* z_prev);
-void       osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x, const OSQPVectorf* xt,
+void       osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x, OSQPVectorf* xt,
                                 const OSQPVectorf* x_prev, OSQPVectorf* z, OSQPVectorf* y, OSQPVectorf* delta_y,
-                                const OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
-                                const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u);
+                                OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
+                                const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u,
+                                OSQPInt next, OSQPFloat sigma, const OSQPVectorf* q);
 /* Queued reductions (no counterpart in the reference, whose reductions each
  * return to the host): each call queues the same reduction as
  * OSQPVectorf_norm_inf / scaled_norm_inf / dot_prod / dot_prod_signed (same

@@ -379,15 +379,19 @@ __global__ void k_admm_rhs(int n, int m, double sigma, double* xt, const double*
     }
   }
 }
-//   update_x (auxil.c:172-184), update_z (:186-208), update_y (:210-229)
-__global__ void k_admm_update(int n, int m, double a, double* x, double* dx, const double* xt, const double* xp,
-                              double* z, double* y, double* dy, const double* zt, const double* zp,
-                              const double* rho, const double* rhoinv, const double* l, const double* u) {
+//   update_x (auxil.c:172-184), update_z (:186-208), update_y (:210-229);
+//   with `next`, also the next iteration's compute_rhs from the new x, z, y
+//   (written over xt / zt once read; the arithmetic of k_admm_rhs)
+__global__ void k_admm_update(int n, int m, double a, double* x, double* dx, double* xt, const double* xp,
+                              double* z, double* y, double* dy, double* zt, const double* zp,
+                              const double* rho, const double* rhoinv, const double* l, const double* u, int next,
+                              double sigma, const double* q) {
   GRID_STRIDE(k, (long long)n + m) {
     if (k < n) {
       const double v = a * xt[k] + (1.0 - a) * xp[k];
       x[k] = v;
       dx[k] = v - xp[k];
+      if (next) xt[k] = sigma * v + (-1.0) * q[k];
     } else {
       const long long i = k - n;
       double v = rhoinv[i] * y[i];
@@ -397,7 +401,12 @@ __global__ void k_admm_update(int n, int m, double a, double* x, double* dx, con
       double d = a * zt[i] + (1.0 - a) * zp[i] + (-1.0) * v;
       d = d * rho[i];
       dy[i] = d;
-      y[i] = y[i] + d;
+      const double yn = y[i] + d;
+      y[i] = yn;
+      if (next) {
+        const double w = rhoinv[i] * yn;
+        zt[i] = (-1.0) * w + 1.0 * v;
+      }
     }
   }
 }
@@ -666,13 +675,15 @@ void osqp_amd_admm_rhs(OSQPVectorf* xt, OSQPVectorf* zt, OSQPFloat sigma, const 
   EW_LAUNCH(k_admm_rhs, (long long)n + m, n, m, sigma, xt->d, x_prev->d, q->d, zt->d, rho_inv->d, y->d, z_prev->d);
 }
 
-void osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x, const OSQPVectorf* xt,
+void osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x, OSQPVectorf* xt,
                           const OSQPVectorf* x_prev, OSQPVectorf* z, OSQPVectorf* y, OSQPVectorf* delta_y,
-                          const OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
-                          const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u) {
+                          OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
+                          const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u, OSQPInt next,
+                          OSQPFloat sigma, const OSQPVectorf* q) {
   const int n = x->length, m = z->length;
   EW_LAUNCH(k_admm_update, (long long)n + m, n, m, alpha, x->d, delta_x->d, xt->d, x_prev->d, z->d, y->d,
-            delta_y->d, zt->d, z_prev->d, rho->d, rho_inv->d, l->d, u->d);
+            delta_y->d, zt->d, z_prev->d, rho->d, rho_inv->d, l->d, u->d, next ? 1 : 0, sigma,
+            next ? q->d : nullptr);
 }
 
 }  // extern "C"

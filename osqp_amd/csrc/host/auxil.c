@@ -84,6 +84,7 @@ OSQPFloat compute_rho_estimate(const OSQPSolver* solver) {
 OSQPInt adapt_rho(OSQPSolver* solver) {
   OSQPInt exitflag = 0;
   OSQPFloat rho_new = compute_rho_estimate(solver);
+  solver->work->rhs_ready = 0;  /* compute_rhs uses rho_inv */
   solver->info->rho_estimate = rho_new;
   if ((rho_new > solver->settings->rho * solver->settings->adaptive_rho_tolerance) ||
       (rho_new < solver->settings->rho / solver->settings->adaptive_rho_tolerance)) {
@@ -127,6 +128,11 @@ void swap_vectors(OSQPVectorf** a, OSQPVectorf** b) {
 static void compute_rhs(OSQPSolver* solver) {
   OSQPWorkspace* work = solver->work;
   OSQPSettings* settings = solver->settings;
+  if (work->rhs_ready) {
+    /* written by the previous iteration's fused update (same arithmetic) */
+    work->rhs_ready = 0;
+    return;
+  }
   if (settings->rho_is_vec) {
     /* add_scaled(xt, sigma, x_prev, -1, q); ew_prod(zt, rho_inv, y);
      * add_scaled(zt, -1, zt, 1, z_prev) — one fused pass */
@@ -145,12 +151,16 @@ void update_xz_tilde(OSQPSolver* solver, OSQPInt admm_iter) {
 }
 
 /* auxil.c:172-229 for the rho-vector case: update_x, update_z, update_y in
- * one fused pass (same element arithmetic) */
+ * one fused pass (same element arithmetic), which also computes the next
+ * iteration's compute_rhs (auxil.c:136-158) from the new x, z, y: the next
+ * compute_rhs is then skipped unless rho, sigma or q change first (adapt_rho
+ * and every solve's start clear rhs_ready). */
 void update_xzy(OSQPSolver* solver) {
   OSQPWorkspace* work = solver->work;
   osqp_amd_admm_update(solver->settings->alpha, work->x, work->delta_x, work->xtilde_view, work->x_prev, work->z,
                        work->y, work->delta_y, work->ztilde_view, work->z_prev, work->rho_vec, work->rho_inv_vec,
-                       work->data->l, work->data->u);
+                       work->data->l, work->data->u, 1, solver->settings->sigma, work->data->q);
+  work->rhs_ready = 1;
 }
 
 /* auxil.c:172-184 */

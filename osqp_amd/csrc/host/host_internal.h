@@ -55,6 +55,8 @@ struct OSQPWorkspace_ {
    * norms in compute_prim_tol / compute_dual_tol), valid while tol_valid */
   OSQPFloat    tol_prim_norm, tol_dual_norm;
   OSQPInt      tol_valid;
+  /* xz_tilde already holds the next iteration's compute_rhs result */
+  OSQPInt      rhs_ready;
   OSQPFloat    rho_inv;
   struct timespec tic;
   OSQPInt first_run, clear_update_time, rho_update_from_solve;

@@ -314,6 +314,7 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
   osqp_amd_tic(work);
   if (settings->verbose) printf("iter   objective    prim res   dual res   rho        time\n");
   if (!settings->warm_starting) osqp_cold_start(solver);
+  work->rhs_ready = 0;
 
   for (iter = 1; iter <= settings->max_iter; iter++) {
     swap_vectors(&(work->x), &(work->x_prev));
