@@ -380,3 +380,34 @@ def test_spmv_skewed_rows(algebra, seed):
     assert np.all(np.abs(vo.get() - A.T @ yv) <= _bound(abs(A.T), np.abs(yv), np.diff(A.indptr)))
     for v in (MA, vx, vy, vyv, vo):
         v.free()
+
+
+def test_panel_spmv_skewed_rows(algebra):
+    """The FP64 panel path (over 2^20 nonzeros): rows longer than a task (the
+    multi-pass long-row path), a run of empty rows and rows of every length
+    from 0 to 120, so task windows start at every alignment; A x and A' y
+    against scipy within the summation-order bound."""
+    rng = np.random.default_rng(4)
+    m, n = 20000, 4000
+    lens = rng.integers(0, 121, m)
+    lens[[3, 777, 15000]] = [n, 2500, 900]   # longer than a 383-entry task
+    lens[5000:5100] = 0                      # empty rows
+    rows = np.repeat(np.arange(m), lens)
+    cols = np.concatenate([rng.choice(n, k, replace=False) for k in lens])
+    A = sp.csc_matrix((rng.standard_normal(len(rows)), (rows, cols)), shape=(m, n))
+    assert A.nnz >= 1 << 20
+    L = algebra.lib()
+    MA = Mat(A)
+    for rep in range(2):  # the second product runs on panels built by the first
+        x = rng.standard_normal(n)
+        vx, vy = Vec(x), Vec(n=m)
+        L.OSQPMatrix_Axpy(MA.h, vx.h, vy.h, 1.0, 0.0)
+        bnd = _bound(abs(A), np.abs(x), np.diff(A.tocsr().indptr))
+        assert np.all(np.abs(vy.get() - A @ x) <= bnd)
+        yv = rng.standard_normal(m)
+        vyv, vo = Vec(yv), Vec(n=n)
+        L.OSQPMatrix_Atxpy(MA.h, vyv.h, vo.h, 1.0, 0.0)
+        assert np.all(np.abs(vo.get() - A.T @ yv) <= _bound(abs(A.T), np.abs(yv), np.diff(A.indptr)))
+        for v in (vx, vy, vyv, vo):
+            v.free()
+    MA.free()
