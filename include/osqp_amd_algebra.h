@@ -170,7 +170,8 @@ OSQPInt    osqp_amd_linsys_bytes(LinSysSolver* s, double* per_cg_iter, double* p
  * bit-identical): compute_rhs (src/auxil.c:136-158) and update_x / update_z /
  * update_y (src/auxil.c:172-229) in one pass each.  With next = 1 the update
  * also writes the next iteration's compute_rhs result (from the new x, z, y,
- * with sigma and q) over xt / zt, valid while rho, sigma and q stay. */
+ * with sigma and q) over xt / zt, valid while rho, sigma and q stay; w_next
+ * (device pointer, may be null) receives rho o (zt_read - zt_next). */
 void       osqp_amd_admm_rhs(OSQPVectorf* xt, OSQPVectorf* zt, OSQPFloat sigma, const OSQPVectorf* x_prev,
                              const OSQPVectorf* q, const OSQPVectorf* rho_inv, const OSQPVectorf* y,
                              const OSQPVectorf* z_prev);
@@ -178,7 +179,14 @@ void       osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* de
                                 const OSQPVectorf* x_prev, OSQPVectorf* z, OSQPVectorf* y, OSQPVectorf* delta_y,
                                 OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
                                 const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u,
-                                OSQPInt next, OSQPFloat sigma, const OSQPVectorf* q);
+                                OSQPInt next, OSQPFloat sigma, const OSQPVectorf* q, OSQPFloat* w_next);
+/* The PCG's hand-off for that fused right-hand side (no counterpart in the
+ * reference): w_target returns the device m-vector in which the update may
+ * write w = rho o (A x - b2) for the next solve (A x being the z~ the last
+ * solve returned), or null when the next solve would not use it;
+ * set_w_given tells the next solve whether it was written. */
+OSQPFloat* osqp_amd_linsys_w_target(LinSysSolver* s);
+void       osqp_amd_linsys_set_w_given(LinSysSolver* s, OSQPInt given);
 /* Queued reductions (no counterpart in the reference, whose reductions each
  * return to the host): each call queues the same reduction as
  * OSQPVectorf_norm_inf / scaled_norm_inf / dot_prod / dot_prod_signed (same

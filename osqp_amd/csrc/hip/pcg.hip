@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_Kv(const PcgState* st, const int*
                                                   const TX* __restrict__ v, const TX* __restrict__ w,
                                                   const double* __restrict__ sub, TX* __restrict__ Kv,
                                                   double* partials) {
-  if (!st->active) return;
+  if (!sub && !st->active) return;  // the residual (sub given) runs outside the CG loop
   __shared__ TX prod[kChunk];
   __shared__ double lds[kBlock / kWave];
   const TX sg = (TX)sigma;
@@ -228,7 +228,7 @@ __global__ void k_comb_av(const PcgState* st, int m, int np, const TX* partial, 
 template <typename TX, int G>
 __global__ void k_comb_kv(const PcgState* st, int n, int np, const TX* partial, double sigma, const TX* v,
                           const double* sub, TX* Kv, double* partials) {
-  if (!st->active) return;
+  if (!sub && !st->active) return;  // the residual (sub given) runs outside the CG loop
   __shared__ double lds[kBlock / kWave];
   const TX sg = (TX)sigma;
   Acc<TX> dot;
@@ -260,7 +260,7 @@ __global__ void k_comb_kv(const PcgState* st, int n, int np, const TX* partial, 
 template <typename TX, int G>
 __global__ void k_comb_kv_rows(const PcgState* st, int n, int np, const TX* partial, double sigma, const TX* v,
                                const double* sub, TX* Kv, double* partials) {
-  if (!st->active) return;
+  if (!sub && !st->active) return;  // the residual (sub given) runs outside the CG loop
   constexpr int RB = 16 / (int)sizeof(TX);
   typedef TX VT __attribute__((ext_vector_type(RB)));
   __shared__ double lds[kBlock / kWave];
@@ -421,12 +421,15 @@ __global__ void k_tol_w(PcgState* st, int admm_iter, double spr, double sdr, dou
 // d = 0; partials r'y, max|r|.  `part_res` must not alias the outputs.
 template <typename V>
 __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int n, const double* r64, const V* dinv,
-                             V* r, V* y, V* p, V* d, double* part_rty, double* part_rn) {
+                             V* r, V* y, V* p, V* d, double* part_rty, double* part_rn, int settol, int admm_iter,
+                             double spr, double sdr, double tol_fraction, int reduction_threshold) {
   __shared__ double lds2[2 * kBlock / kWave];
   double* lds = lds2;
   const double rn0 = parts_max(part_res, nres, lds);
   const double scale = (rn0 > 0.0 && isfinite(rn0)) ? rn0 : 1.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // cached path with w64 given: the tolerance update of k_tol_w, here
+    if (settol) set_tolerance(st, 0.0, admm_iter, spr, sdr, 0, tol_fraction, reduction_threshold);
     st->scale = scale;
     st->eps_cmp = st->eps / scale;
   }
@@ -654,6 +657,7 @@ struct HipPcg {
   int gA, gT, gK;     // lanes per row: A rows, A' rows, K rows
   int lookahead;      // CG iterations queued ahead of the last observed flag
   int last_cg;        // CG iterations of the previous solve (where its loop stopped)
+  int w_given;        // w64 = rho o (A x64 - b2) of the next solve already written (host core's fused update)
   // combined K-operator CSR [P A'] + its row blocks: FP64 values for the
   // defect-correction residual, FP32 values for the mixed CG
   int *krp, *kci, *k_rows, k_nb;
@@ -921,7 +925,7 @@ template <typename V>
 int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* parts) {
   const int n = s->n;
   if (const PanelMatrix* P = panels_K<V>(s)) {
-    panel_spmv(*P, active_flag(s), v, n, w, (V*)s->partK);
+    panel_spmv(*P, sub ? nullptr : active_flag(s), v, n, w, (V*)s->partK);
     constexpr int RB = 16 / (int)sizeof(V);
     if (n % RB == 0) {
       const int nq = n / RB;
@@ -1023,6 +1027,8 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   const double spr = s->scaled_prim_res ? *s->scaled_prim_res : 0.0;
   const double sdr = s->scaled_dual_res ? *s->scaled_dual_res : 0.0;
   const bool cached = m > 0 && s->ax_valid && s->ax_Aver == s->A->version && admm_iter > 1 && !s->polishing;
+  const bool w_given = cached && s->w_given;
+  s->w_given = 0;
   int nres;
   if (!cached) {
     // explicit rhs: needed for ||rhs|| at the first ADMM iteration and in polishing
@@ -1037,14 +1043,18 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
     nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)s->rhs64, s->r64, part2);
   } else {
     // tolerance; w64 = rho o (A x64 - b2) from the cached A x64;
-    // r64 = sigma x64 + P x64 + A' w64 - b1
-    hipLaunchKernelGGL(k_tol_w, dim3(gM), dim3(kBlock), 0, S(), s->st, admm_iter, spr, sdr, s->tol_fraction,
-                       s->reduction_threshold, m, (const double*)s->ax64, (const double*)(b->d + n), rho, s->rho_sc,
-                       s->w64);
+    // r64 = sigma x64 + P x64 + A' w64 - b1.  When the host core's fused
+    // ADMM update has already written w64 (osqp_amd_linsys_set_w_given), the
+    // tolerance update moves into the CG init.
+    if (!w_given)
+      hipLaunchKernelGGL(k_tol_w, dim3(gM), dim3(kBlock), 0, S(), s->st, admm_iter, spr, sdr, s->tol_fraction,
+                         s->reduction_threshold, m, (const double*)s->ax64, (const double*)(b->d + n), rho,
+                         s->rho_sc, s->w64);
     nres = op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, (const double*)b->d, s->r64, part2);
   }
   hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nres, n,
-                     (const double*)s->r64, dinv, r, y, p, d, part3, part4);
+                     (const double*)s->r64, dinv, r, y, p, d, part3, part4, w_given ? 1 : 0, admm_iter, spr, sdr,
+                     s->tol_fraction, s->reduction_threshold);
   // one CG iteration: the operator (two SpMVs with their combines), the
   // update, and the next loop head fused with the search-direction update.
   // With lookahead 0 the first launch of iteration it + 1 (A's panel SpMV) is
@@ -1186,6 +1196,7 @@ OSQPInt pcg_update_matrices(LinSysSolver* self, const OSQPMatrix*, const OSQPInt
                             const OSQPInt*, OSQPInt) {
   HipPcg* s = (HipPcg*)self;
   s->ax_valid = 0;
+  s->w_given = 0;
   update_precond(s);
   return 0;
 }
@@ -1193,6 +1204,7 @@ OSQPInt pcg_update_matrices(LinSysSolver* self, const OSQPMatrix*, const OSQPInt
 OSQPInt pcg_update_rho(LinSysSolver* self, const OSQPVectorf*, OSQPFloat rho_sc) {
   HipPcg* s = (HipPcg*)self;
   s->rho_sc = rho_sc;
+  s->w_given = 0;
   update_precond(s);
   return 0;
 }
@@ -1452,6 +1464,17 @@ OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per
   if (per_cg_iter) *per_cg_iter = cg;
   if (per_solve) *per_solve = outer;
   return 0;
+}
+
+OSQPFloat* osqp_amd_linsys_w_target(LinSysSolver* ls) {
+  HipPcg* s = (HipPcg*)ls;
+  if (!s || s->m <= 0 || s->polishing || !s->rho_vec || !s->ax_valid || s->ax_Aver != s->A->version) return nullptr;
+  return s->w64;
+}
+
+void osqp_amd_linsys_set_w_given(LinSysSolver* ls, OSQPInt given) {
+  HipPcg* s = (HipPcg*)ls;
+  if (s) s->w_given = given ? 1 : 0;
 }
 
 void osqp_amd_linsys_counters(LinSysSolver* ls, OSQPInt* cg_iters, OSQPInt* solves) {

@@ -381,11 +381,14 @@ __global__ void k_admm_rhs(int n, int m, double sigma, double* xt, const double*
 }
 //   update_x (auxil.c:172-184), update_z (:186-208), update_y (:210-229);
 //   with `next`, also the next iteration's compute_rhs from the new x, z, y
-//   (written over xt / zt once read; the arithmetic of k_admm_rhs)
+//   (written over xt / zt once read; the arithmetic of k_admm_rhs), and with
+//   wn the PCG's w = rho o (A x - b2) for that right-hand side: A x of the
+//   last solve is the zt read here (the solve returns z~ = A x), so this is
+//   the arithmetic of the PCG's k_tol_w on the same values
 __global__ void k_admm_update(int n, int m, double a, double* x, double* dx, double* xt, const double* xp,
                               double* z, double* y, double* dy, double* zt, const double* zp,
                               const double* rho, const double* rhoinv, const double* l, const double* u, int next,
-                              double sigma, const double* q) {
+                              double sigma, const double* q, double* wn) {
   GRID_STRIDE(k, (long long)n + m) {
     if (k < n) {
       const double v = a * xt[k] + (1.0 - a) * xp[k];
@@ -394,18 +397,21 @@ __global__ void k_admm_update(int n, int m, double a, double* x, double* dx, dou
       if (next) xt[k] = sigma * v + (-1.0) * q[k];
     } else {
       const long long i = k - n;
+      const double ztk = zt[i];
       double v = rhoinv[i] * y[i];
-      v += a * zt[i] + (1.0 - a) * zp[i];  // add_scaled3's increment form (x == a, sca == 1)
+      v += a * ztk + (1.0 - a) * zp[i];  // add_scaled3's increment form (x == a, sca == 1)
       v = cmin(cmax(v, l[i]), u[i]);
       z[i] = v;
-      double d = a * zt[i] + (1.0 - a) * zp[i] + (-1.0) * v;
+      double d = a * ztk + (1.0 - a) * zp[i] + (-1.0) * v;
       d = d * rho[i];
       dy[i] = d;
       const double yn = y[i] + d;
       y[i] = yn;
       if (next) {
         const double w = rhoinv[i] * yn;
-        zt[i] = (-1.0) * w + 1.0 * v;
+        const double ztn = (-1.0) * w + 1.0 * v;
+        zt[i] = ztn;
+        if (wn) wn[i] = rho[i] * (ztk - ztn);
       }
     }
   }
@@ -679,11 +685,11 @@ void osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x,
                           const OSQPVectorf* x_prev, OSQPVectorf* z, OSQPVectorf* y, OSQPVectorf* delta_y,
                           OSQPVectorf* zt, const OSQPVectorf* z_prev, const OSQPVectorf* rho,
                           const OSQPVectorf* rho_inv, const OSQPVectorf* l, const OSQPVectorf* u, OSQPInt next,
-                          OSQPFloat sigma, const OSQPVectorf* q) {
+                          OSQPFloat sigma, const OSQPVectorf* q, OSQPFloat* w_next) {
   const int n = x->length, m = z->length;
   EW_LAUNCH(k_admm_update, (long long)n + m, n, m, alpha, x->d, delta_x->d, xt->d, x_prev->d, z->d, y->d,
             delta_y->d, zt->d, z_prev->d, rho->d, rho_inv->d, l->d, u->d, next ? 1 : 0, sigma,
-            next ? q->d : nullptr);
+            next ? q->d : nullptr, next ? w_next : nullptr);
 }
 
 }  // extern "C"

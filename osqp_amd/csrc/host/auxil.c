@@ -129,10 +129,14 @@ static void compute_rhs(OSQPSolver* solver) {
   OSQPWorkspace* work = solver->work;
   OSQPSettings* settings = solver->settings;
   if (work->rhs_ready) {
-    /* written by the previous iteration's fused update (same arithmetic) */
+    /* written by the previous iteration's fused update (same arithmetic),
+     * with the PCG's w for it when w_ready */
     work->rhs_ready = 0;
+    osqp_amd_linsys_set_w_given(work->linsys_solver, work->w_ready);
+    work->w_ready = 0;
     return;
   }
+  osqp_amd_linsys_set_w_given(work->linsys_solver, 0);
   if (settings->rho_is_vec) {
     /* add_scaled(xt, sigma, x_prev, -1, q); ew_prod(zt, rho_inv, y);
      * add_scaled(zt, -1, zt, 1, z_prev) — one fused pass */
@@ -157,10 +161,12 @@ void update_xz_tilde(OSQPSolver* solver, OSQPInt admm_iter) {
  * and every solve's start clear rhs_ready). */
 void update_xzy(OSQPSolver* solver) {
   OSQPWorkspace* work = solver->work;
+  OSQPFloat* w = osqp_amd_linsys_w_target(work->linsys_solver);
   osqp_amd_admm_update(solver->settings->alpha, work->x, work->delta_x, work->xtilde_view, work->x_prev, work->z,
                        work->y, work->delta_y, work->ztilde_view, work->z_prev, work->rho_vec, work->rho_inv_vec,
-                       work->data->l, work->data->u, 1, solver->settings->sigma, work->data->q);
+                       work->data->l, work->data->u, 1, solver->settings->sigma, work->data->q, w);
   work->rhs_ready = 1;
+  work->w_ready = w != NULL;
 }
 
 /* auxil.c:172-184 */

@@ -57,6 +57,7 @@ struct OSQPWorkspace_ {
   OSQPInt      tol_valid;
   /* xz_tilde already holds the next iteration's compute_rhs result */
   OSQPInt      rhs_ready;
+  OSQPInt      w_ready;   /* ... and the PCG's w for it */
   OSQPFloat    rho_inv;
   struct timespec tic;
   OSQPInt first_run, clear_update_time, rho_update_from_solve;
