@@ -53,7 +53,7 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
   constexpr int NG = PER / V;          // load groups of 64 V entries
   constexpr int CAP = 64 * PER - (V - 1);  // entries of a task (the aligned window holds 64 PER)
   static_assert(PER % V == 0, "PER must be a multiple of the vector width");
-  __shared__ TX xs[W];
+  __shared__ __attribute__((aligned(16))) TX xs[W];
   __shared__ __attribute__((aligned(16))) TX prod_all[NW][64 * PER];
   constexpr int XL = W / (NW * 64);
   const int lane = threadIdx.x & 63;
@@ -65,20 +65,43 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
     const int2 pe = tpanel[t];
     const int p = pe.x;
     const int te = pe.y < t_end ? pe.y : t_end;
-    // the panel's slice of x into LDS: every load in flight, then the stores
+    // the panel's slice of x into LDS: every load in flight, then the stores.
+    // A full panel lying wholly in x0 or wholly in x1, 16-byte aligned there,
+    // goes as 16-byte vectors (thread i takes vectors i, i + 1024, ...);
+    // otherwise element by element.  The choice is the same for the whole
+    // workgroup.
     const int c0 = p * W;
     const int w = (C - c0) < W ? (C - c0) : W;
-    TX xr[XL];
-#pragma unroll
-    for (int j = 0; j < XL; ++j) {
-      const int i = j * NW * 64 + threadIdx.x;
-      const int c = c0 + i;
-      xr[j] = i < w ? (c < n0 ? x0[c] : x1[c - n0]) : (TX)0;
+    constexpr int XV = 16 / (int)sizeof(TX);  // elements per vector
+    constexpr int XG = XL / XV;               // vectors per thread
+    static_assert(XL % XV == 0, "the slice must split into whole vectors per thread");
+    const TX* src = nullptr;
+    if (w == W) {
+      if (c0 + W <= n0) src = x0 + c0;
+      else if (c0 >= n0 && x1) src = x1 + (c0 - n0);
     }
-    __syncthreads();  // the previous panel's readers are done
+    if (src && ((uintptr_t)src & 15) == 0) {
+      typedef TX VT __attribute__((ext_vector_type(XV)));
+      VT xv[XG];
 #pragma unroll
-    for (int j = 0; j < XL; ++j) xs[j * NW * 64 + threadIdx.x] = xr[j];
-    __syncthreads();
+      for (int g = 0; g < XG; ++g) xv[g] = *(const VT*)(src + (g * NW * 64 + threadIdx.x) * XV);
+      __syncthreads();  // the previous panel's readers are done
+#pragma unroll
+      for (int g = 0; g < XG; ++g) *(VT*)(xs + (g * NW * 64 + threadIdx.x) * XV) = xv[g];
+      __syncthreads();
+    } else {
+      TX xr[XL];
+#pragma unroll
+      for (int j = 0; j < XL; ++j) {
+        const int i = j * NW * 64 + threadIdx.x;
+        const int c = c0 + i;
+        xr[j] = i < w ? (c < n0 ? x0[c] : x1[c - n0]) : (TX)0;
+      }
+      __syncthreads();  // the previous panel's readers are done
+#pragma unroll
+      for (int j = 0; j < XL; ++j) xs[j * NW * 64 + threadIdx.x] = xr[j];
+      __syncthreads();
+    }
     const int* rpp = rp + (long long)p * (R + 1);
     TX* out = partial + (long long)p * R;
     // this wavefront's tasks of the panel: t + wv + i NW; lane i holds the
