@@ -57,6 +57,9 @@ def lib():
             "OSQPMatrix_submatrix_byrows": ([vp, vp], vp), "OSQPMatrix_is_eq": ([vp, vp, d], i),
             "osqp_algebra_init_libs": ([i], i), "osqp_algebra_free_libs": ([], None),
             "osqp_amd_algebra_sync": ([], None),
+            "osqp_amd_red_norm_inf": ([vp, vp], None), "osqp_amd_red_scaled_norm_inf": ([vp, vp, vp], None),
+            "osqp_amd_red_dot_prod": ([vp, vp, vp], None), "osqp_amd_red_dot_prod_signed": ([vp, vp, i, vp], None),
+            "osqp_amd_red_wait": ([], None),
         }
         for name, (a, r) in sig.items():
             f = getattr(L, name)

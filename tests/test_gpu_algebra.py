@@ -98,6 +98,43 @@ def test_reductions(algebra, n):
         v.free()
 
 
+@pytest.mark.parametrize("n", [0, 1, 1000, 300_001])
+def test_queued_reductions_match(algebra, n):
+    """Queued reductions (run together in one launch when waited for) give
+    the bits of the synchronous calls, including past the 16-entry queue
+    (which runs the queue first) and for empty vectors."""
+    import ctypes as C
+    (a, b, s), _ = rng_vecs(n + 7, n)
+    L = algebra.lib()
+    va, vb, vs = Vec(a), Vec(b), Vec(s)
+    calls = [("norm", (va,)), ("scaled", (vs, va)), ("dot", (va, vb)), ("signed", (va, vb, 1)),
+             ("signed", (va, vb, -1)), ("norm", (vb,)), ("scaled", (va, vb)), ("dot", (vb, vs))] * 3
+    outs = (C.c_double * len(calls))()
+    for k, (kind, args) in enumerate(calls):
+        dst = C.cast(C.addressof(outs) + 8 * k, C.c_void_p)
+        if kind == "norm":
+            L.osqp_amd_red_norm_inf(args[0].h, dst)
+        elif kind == "scaled":
+            L.osqp_amd_red_scaled_norm_inf(args[0].h, args[1].h, dst)
+        elif kind == "dot":
+            L.osqp_amd_red_dot_prod(args[0].h, args[1].h, dst)
+        else:
+            L.osqp_amd_red_dot_prod_signed(args[0].h, args[1].h, args[2], dst)
+    L.osqp_amd_red_wait()
+    for k, (kind, args) in enumerate(calls):
+        if kind == "norm":
+            ref = L.OSQPVectorf_norm_inf(args[0].h)
+        elif kind == "scaled":
+            ref = L.OSQPVectorf_scaled_norm_inf(args[0].h, args[1].h)
+        elif kind == "dot":
+            ref = L.OSQPVectorf_dot_prod(args[0].h, args[1].h)
+        else:
+            ref = L.OSQPVectorf_dot_prod_signed(args[0].h, args[1].h, args[2])
+        assert outs[k] == ref, (k, kind, outs[k], ref)
+    for v in (va, vb, vs):
+        v.free()
+
+
 def test_reccone_and_bounds_type(algebra):
     """vector.c:676-727 and 885-921 with infinite, equal and loose bounds."""
     rng = np.random.default_rng(5)
