@@ -915,7 +915,11 @@ OSQPInt osqp_batch_add_many(OSQPBatch* b, OSQPInt count, const OSQPCscMatrix* co
   const unsigned hw = std::thread::hardware_concurrency();
   const int nthreads = (int)std::min<long>(std::min<long>(kAddThreads, hw ? (long)hw : 1L), count / 8 + 1);
   std::vector<std::thread> pool;
-  for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+  try {
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+  } catch (...) {
+    // no more threads: the ones started and this one take all the problems
+  }
   work();
   for (auto& th : pool) th.join();
   OSQPInt first_error = 0;
