@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_spmv(const int* __restrict__ rows
                                                    const int* __restrict__ ci, const double* __restrict__ val,
                                                    const double* __restrict__ x, double* __restrict__ y, double alpha,
                                                    double beta) {
-  __shared__ double prod[kChunk];
+  __shared__ __attribute__((aligned(16))) double prod[kChunk];
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     rb_spmv_block<double, double>(rows[b], rows[b + 1], rp, ci, val, prod, [&](int c) { return x[c]; },
                                   [&](int row, double sum) {
@@ -278,11 +278,11 @@ int build_row_blocks(const int* rp, int rows, int* out) {
   out[0] = 0;
   for (int r = 0; r < rows; ++r) {
     const int len = rp[r + 1] - rp[r];
-    if (len > kChunk) {
+    if (len > kCap) {
       if (start < r) out[++nb] = r;      // close the open block
       out[++nb] = r + 1;                 // the long row alone
       start = r + 1;
-    } else if (rp[r + 1] - rp[start] > kChunk || r - start >= kChunk) {
+    } else if (rp[r + 1] - rp[start] > kCap || r - start >= kChunk) {
       out[++nb] = r;                     // block [start, r)
       start = r;
     }

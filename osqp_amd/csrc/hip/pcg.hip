@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_Av_rho(const PcgState* st, const 
                                                       const double* __restrict__ rho, double rho_sc,
                                                       TX* __restrict__ w) {
   if (!st->active) return;
-  __shared__ TX prod[kChunk];
+  __shared__ __attribute__((aligned(16))) TX prod[kChunk];
   for (int b = blockIdx.x; b < nb; b += gridDim.x)
     rb_spmv_block<TV, TX>(rows[b], rows[b + 1], rp, ci, val, prod, [&](int c) { return v[c]; },
                           [&](int row, TX sum) { w[row] = (TX)rho_at(rho, rho_sc, row) * sum; });
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_Kv(const PcgState* st, const int*
                                                   const double* __restrict__ sub, TX* __restrict__ Kv,
                                                   double* partials) {
   if (!sub && !st->active) return;  // the residual (sub given) runs outside the CG loop
-  __shared__ TX prod[kChunk];
+  __shared__ __attribute__((aligned(16))) TX prod[kChunk];
   __shared__ double lds[kBlock / kWave];
   const TX sg = (TX)sigma;
   Acc<TX> dot;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_rhs(const int* __restrict__ rows,
                                                    const int* __restrict__ kci, const TV* __restrict__ kval, int n,
                                                    const double* __restrict__ b, const double* __restrict__ rho,
                                                    double rho_sc, double* __restrict__ rhs, double* partials) {
-  __shared__ double prod[kChunk];
+  __shared__ __attribute__((aligned(16))) double prod[kChunk];
   __shared__ double lds[kBlock / kWave];
   const double* b2 = b + n;
   double mx = 0.0;
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_finish_z(const int* __restrict__ 
                                                         const TV* __restrict__ val, const double* __restrict__ x,
                                                         double* __restrict__ b, double* __restrict__ ax, int n,
                                                         int polishing, double rho_pol) {
-  __shared__ double prod[kChunk];
+  __shared__ __attribute__((aligned(16))) double prod[kChunk];
   for (int blk = blockIdx.x; blk < nb; blk += gridDim.x)
     rb_spmv_block<TV, double>(rows[blk], rows[blk + 1], rp, ci, val, prod, [&](int c) { return x[c]; },
                               [&](int row, double sum) {

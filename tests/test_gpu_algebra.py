@@ -419,6 +419,32 @@ def test_spmv_skewed_rows(algebra, seed):
         v.free()
 
 
+def test_spmv_row_block_window_edges(algebra):
+    """Row-block partition at its limits: rows of 2043..2049 nonzeros (either
+    side of the 2045-nonzero block capacity), blocks starting at every
+    alignment, and rows of 17..300 nonzeros reduced by lane groups."""
+    rng = np.random.default_rng(7)
+    n = 4000
+    lens = np.concatenate([[2043, 2044, 2045, 2046, 2047, 2048, 2049, 1, 2, 3, 17, 20, 33, 64, 65, 100, 300, 0],
+                           rng.integers(0, 120, 700), rng.integers(15, 40, 500), [2045, 1, 2046]])
+    m = len(lens)
+    rows = np.repeat(np.arange(m), lens)
+    cols = np.concatenate([rng.choice(n, k, replace=False) for k in lens])
+    A = sp.csc_matrix((rng.standard_normal(len(rows)), (rows, cols)), shape=(m, n))
+    L = algebra.lib()
+    MA = Mat(A)
+    x = rng.standard_normal(n)
+    vx, vy = Vec(x), Vec(n=m)
+    L.OSQPMatrix_Axpy(MA.h, vx.h, vy.h, 1.0, 0.0)
+    assert np.all(np.abs(vy.get() - A @ x) <= _bound(abs(A), np.abs(x), np.diff(A.tocsr().indptr)))
+    yv = rng.standard_normal(m)
+    vyv, vo = Vec(yv), Vec(n=n)
+    L.OSQPMatrix_Atxpy(MA.h, vyv.h, vo.h, 1.0, 0.0)
+    assert np.all(np.abs(vo.get() - A.T @ yv) <= _bound(abs(A.T), np.abs(yv), np.diff(A.indptr)))
+    for v in (MA, vx, vy, vyv, vo):
+        v.free()
+
+
 def test_panel_spmv_skewed_rows(algebra):
     """The FP64 panel path (over 2^20 nonzeros): rows longer than a task (the
     multi-pass long-row path), a run of empty rows and rows of every length
