@@ -499,6 +499,18 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
   if (!st->active) return;
   __shared__ double lds2[2 * kBlock / kWave];
   double* lds = lds2;
+  // the first grid-stride element's operands do not depend on alpha: they
+  // are loaded before the partial sums, so the two memory latencies overlap
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  V x0 = 0, p0 = 0, r0 = 0, kp0 = 0, d0 = 0;
+  if (i0 < n) {
+    x0 = x[i0];
+    p0 = p[i0];
+    r0 = r[i0];
+    kp0 = Kp[i0];
+    d0 = dinv[i0];
+  }
   double rty_j;
   if (j == 0) {
     double rty0, rn0;
@@ -533,7 +545,17 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
   const V alpha = (V)alpha64;
   Acc<V> dot;
   double mx = 0.0;
-  GRID_STRIDE(i, n) {
+  if (i0 < n) {
+    x[i0] = x0 + alpha * p0;
+    V ri = r0 + alpha * kp0;
+    V yi = d0 * ri;
+    r[i0] = ri;
+    y[i0] = yi;
+    dot.add(yi * ri);
+    double a = fabs((double)ri);
+    mx = a > mx ? a : mx;
+  }
+  for (long long i = i0 + stride; i < n; i += stride) {
     x[i] += alpha * p[i];
     V ri = r[i] + alpha * Kp[i];
     V yi = dinv[i] * ri;
@@ -562,6 +584,13 @@ __global__ void k_head_pupdate(PcgState* st, const double* part_rty, const doubl
     if (flags && blockIdx.x == 0 && threadIdx.x == 0) flags[j] = 2 * st->solves;
     return;
   }
+  // the first grid-stride element is loaded before the partial sums
+  const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  V p0 = 0, y0 = 0;
+  if (i0 < n) {
+    p0 = p[i0];
+    y0 = y[i0];
+  }
   double rty, rn;
   parts_sum_max(part_rty, part_rn, nparts, lds2, &rty, &rn);
   const double beta64 = rty / st->rty[(j - 1) & 1];
@@ -578,7 +607,9 @@ __global__ void k_head_pupdate(PcgState* st, const double* part_rty, const doubl
   }
   if (!go) return;
   const V beta = (V)beta64;
-  GRID_STRIDE(i, n) p[i] = beta * p[i] - y[i];
+  if (i0 < n) p[i0] = beta * p0 - y0;
+  for (long long i = i0 + (long long)gridDim.x * blockDim.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = beta * p[i] - y[i];
 }
 
 // diag preconditioner: d = sigma + diag(P) + sum_j rho_j A_ji^2
