@@ -298,6 +298,11 @@ OSQPInt osqp_amd_time_fp64_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_
  * path (see osqp_amd_linsys_bytes in osqp_amd_algebra.h). */
 OSQPInt osqp_amd_pcg_bytes(OSQPSolver* solver, double* per_cg_iter, double* per_solve);
 
+/* Number of roctx ranges pushed and popped and of events marked by the
+ * profiler annotations (settings->profiler_level 1 or 2; the sections of
+ * src/profilers.c:4-28).  For tests of the annotation backend. */
+OSQPInt osqp_amd_profiler_counts(long long* pushes, long long* pops, long long* marks);
+
 #ifdef __cplusplus
 }
 #endif

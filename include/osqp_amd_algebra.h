@@ -18,6 +18,8 @@
 #ifndef OSQP_AMD_ALGEBRA_H
 #define OSQP_AMD_ALGEBRA_H
 
+#include <stddef.h>
+
 #include "osqp_amd.h"
 
 #ifdef __cplusplus
@@ -202,6 +204,33 @@ void       osqp_amd_red_dot_prod_signed(const OSQPVectorf* a, const OSQPVectorf*
 void       osqp_amd_red_wait(void);
 /* Last HIP error seen by the backend (0 if none); cleared on read. */
 OSQPInt    osqp_amd_last_error(void);
+/* 1 when p points to device memory (hipMalloc), 0 for host memory, and a
+ * copy of bytes from either to host memory (0 on success).  osqp_setup and
+ * the serving engine accept device-resident arrays through them (the
+ * reference's setup reads host arrays only, src/auxil.c:950-1017). */
+OSQPInt    osqp_amd_ptr_is_device(const void* p);
+OSQPInt    osqp_amd_copy_to_host(void* dst, const void* src, size_t bytes);
+/* Copies the backend's structures of M to host arrays, for checks of the
+ * integer work against OSQP's (csc_utils.c:247-286 triplet_to_csr,
+ * cuda_csr.cu:489-577 triu->full and transpose).  Any pointer may be null.
+ *   rp, ci, val : the CSR copy (A: m+1 / nnz entries; P: the full symmetric
+ *                 matrix, n+1 / nnz_csr entries, rows sorted by column);
+ *   trp, tci, tval: CSR of A' (n+1 / nnz entries; general matrices only);
+ *   csc2csr     : CSR slot of each entry of the input CSC (nnz_in);
+ *   csc2mirror  : P only, the transposed slot of each input entry, -1 on
+ *                 the diagonal (nnz_in);
+ *   diag        : P only, the slot of (j, j) in row j, -1 if absent (n).
+ * *nnz_csr / *nnz_in (may be null) receive the two counts.  Returns 0. */
+OSQPInt    osqp_amd_matrix_export(const OSQPMatrix* M, OSQPInt* nnz_csr, OSQPInt* nnz_in, OSQPInt* rp, OSQPInt* ci,
+                                  OSQPFloat* val, OSQPInt* trp, OSQPInt* tci, OSQPFloat* tval, OSQPInt* csc2csr,
+                                  OSQPInt* csc2mirror, OSQPInt* diag);
+/* The solver's problem matrices (scaled, as the ADMM iterations see them). */
+OSQPInt    osqp_amd_solver_matrices(OSQPSolver* solver, const OSQPMatrix** P, const OSQPMatrix** A);
+/* The solver's Ruiz scaling (src/scaling.c:48-164) and scaled vectors:
+ * D (n), E (m), c, and the scaled q (n), l, u (m).  Any pointer may be null;
+ * returns 1 when the solver was set up without scaling. */
+OSQPInt    osqp_amd_solver_scaling(OSQPSolver* solver, OSQPFloat* D, OSQPFloat* E, OSQPFloat* c, OSQPFloat* q,
+                                   OSQPFloat* l, OSQPFloat* u);
 
 #ifdef __cplusplus
 }

@@ -24,8 +24,16 @@
  * A problem's results do not depend on the other problems in flight or on the
  * engine's tuning (team sizes): its arithmetic is fixed by its own data.
  *
- * Settings: those of OSQPSettings, with rho_is_vec = 1, polishing = 0 and
- * adaptive_rho in {disabled, iterations}; time_limit is not applied.
+ * Settings: those of OSQPSettings, with rho_is_vec = 1 and polishing = 0.
+ * All four adaptive_rho modes apply (src/osqp_api.c:776-835); the time-based
+ * one measures the setup and loop times on the device clock, so, as in the
+ * reference, its results depend on timing.  time_limit (src/osqp_api.c:
+ * 743-759) counts from the start of a problem's setup on the GPU (first
+ * solve) or of its re-solve, on the device clock; limits of 1e6 s or more
+ * are not applied (the default is 1e10 s).
+ * Results follow OSQP's contract (src/auxil.c:598-674): x and y are OSQP_NAN
+ * without a solution, and the infeasibility certificates are returned
+ * normalised (OSQP_NAN where they do not apply).
  * Calls on one engine must not overlap (one host thread drives an engine).
  */
 #ifndef OSQP_AMD_SERVE_H
@@ -63,6 +71,11 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* settings, OSQPI
                           long long arena_bytes);
 
 /* Queue a problem (setup + first solve).  x0 / y0: optional warm start.
+ * Lengths: q and x0 hold n = P->n entries, l, u and y0 hold m = A->m; the
+ * engine reads exactly that many (a shorter array is a caller error).  The
+ * arrays (those of P and A included; the two OSQPCscMatrix structs are host
+ * memory) may be host or device memory: device arrays are copied to the
+ * host for validation and packing.
  * resident = 1 keeps it on the GPU after the solve for osqp_batch_resolve;
  * resident = 0 frees it once its result is polled.
  * Returns 0 with *id set, or OSQP_DATA_VALIDATION_ERROR. */
@@ -83,9 +96,28 @@ OSQPInt osqp_batch_add_many(OSQPBatch* batch, OSQPInt count, const OSQPCscMatrix
                             OSQPInt* ids);
 
 /* Queue a warm-started re-solve of a resident problem whose last solve has
- * ended and been polled; q_new (may be null) replaces q first.  Returns 0, or
- * -1 when the problem is unknown, not resident or busy. */
+ * ended and been polled; q_new (may be null, host or device memory)
+ * replaces q first, with osqp_update_data_vec's semantics (src/osqp_api.c:
+ * 1184-1257).  Matrix values staged by osqp_batch_update_data_mat since the
+ * last solve are applied first.  Returns 0, -1 when the problem is unknown,
+ * not resident or busy, or -2 when no staging memory is free for the staged
+ * values (retry after polling other results). */
 OSQPInt osqp_batch_resolve(OSQPBatch* batch, OSQPInt id, const OSQPFloat* q_new);
+
+/* osqp_update_data_mat (src/osqp_api.c:1309-1409) for a resident problem
+ * whose last solve has ended and been polled: new values of P (its upper
+ * triangle, in the CSC order of osqp_batch_add) and / or A (CSC order), all
+ * of them (index array null, count 0 or nnz) or those listed by the index
+ * array.  Values and indices may be host or device memory.  The values are
+ * staged; the next osqp_batch_resolve applies them on the GPU as the
+ * reference does (unscale the data, write the values, scale again, rebuild
+ * the preconditioner) before it solves, warm-started from the iterates.
+ * Several calls before one resolve apply in call order.  Returns 0, 1 (P
+ * count or indices invalid), 2 (A count or indices invalid), or -1 (unknown,
+ * not resident or busy). */
+OSQPInt osqp_batch_update_data_mat(OSQPBatch* batch, OSQPInt id, const OSQPFloat* Px_new,
+                                   const OSQPInt* Px_new_idx, OSQPInt P_new_n, const OSQPFloat* Ax_new,
+                                   const OSQPInt* Ax_new_idx, OSQPInt A_new_n);
 
 /* Admits queued problems, finishes the epoch launched by the previous step
  * and launches the next one without waiting for it.  *completed (may be
@@ -98,10 +130,12 @@ OSQPInt osqp_batch_step(OSQPBatch* batch, OSQPInt* completed);
  * them written to ids); returns their number. */
 OSQPInt osqp_batch_last_completed(const OSQPBatch* batch, OSQPInt* ids, OSQPInt cap);
 
-/* 1 and the result (x: n, y: m, either may be null) when the problem's solve
- * has ended; 0 while it runs; -1 for an unknown id.  A non-resident problem is
- * released by a successful poll. */
-OSQPInt osqp_batch_poll(OSQPBatch* batch, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OSQPBatchInfo* info);
+/* 1 and the result when the problem's solve has ended; 0 while it runs; -1
+ * for an unknown id.  x: n, y: m, prim_inf_cert: m, dual_inf_cert: n (any may
+ * be null), as OSQPSolution holds them after osqp_solve.  A non-resident
+ * problem is released by a successful poll. */
+OSQPInt osqp_batch_poll(OSQPBatch* batch, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OSQPFloat* prim_inf_cert,
+                        OSQPFloat* dual_inf_cert, OSQPBatchInfo* info);
 
 /* Drop a problem that is not running.  Returns 0 or -1. */
 OSQPInt osqp_batch_release(OSQPBatch* batch, OSQPInt id);

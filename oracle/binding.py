@@ -141,15 +141,16 @@ class OracleSolver:
             self._ln.ctypes.data if l is not None else None,
             self._un.ctypes.data if u is not None else None)
 
-    def update_data_mat(self, Px=None, Px_idx=None, Ax=None, Ax_idx=None):
+    def update_data_mat(self, Px=None, Px_idx=None, Ax=None, Ax_idx=None, Px_n0=False, Ax_n0=False):
+        """osqp_update_data_mat; Px_n0 / Ax_n0 pass a count of 0 (all values)."""
         self._keep = [as_f64(Px), None if Px_idx is None else np.ascontiguousarray(Px_idx, np.int32),
                       as_f64(Ax), None if Ax_idx is None else np.ascontiguousarray(Ax_idx, np.int32)]
         Pxn, Pin, Axn, Ain = self._keep
         return self.lib.oracle_update_data_mat(
             self.h, Pxn.ctypes.data if Pxn is not None else None,
-            Pin.ctypes.data if Pin is not None else None, 0 if Pxn is None else len(Pxn),
+            Pin.ctypes.data if Pin is not None else None, 0 if Pxn is None or Px_n0 else len(Pxn),
             Axn.ctypes.data if Axn is not None else None,
-            Ain.ctypes.data if Ain is not None else None, 0 if Axn is None else len(Axn))
+            Ain.ctypes.data if Ain is not None else None, 0 if Axn is None or Ax_n0 else len(Axn))
 
     def warm_start(self, x=None, y=None):
         self._wx, self._wy = as_f64(x), as_f64(y)

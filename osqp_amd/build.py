@@ -19,14 +19,14 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libosqp_amd.so")
 
 HIP_SOURCES = ["hip/runtime.hip", "hip/vector.hip", "hip/matrix.hip", "hip/panel.hip", "hip/pcg.hip", "serve/batch.hip"]
-C_SOURCES = ["host/auxil.c", "host/osqp_api.c", "host/polish.c"]
+C_SOURCES = ["host/auxil.c", "host/osqp_api.c", "host/polish.c", "host/profilers.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu99", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
-           "-I", os.path.join(ROOT, "include")]
+           "-I", os.path.join(ROOT, "include"), "-I/opt/rocm/include"]
 
 
 def _run(cmd):
@@ -55,7 +55,9 @@ def build(verbose=False):
                 print(out)
     objs = [_obj(s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))] + [_obj(s) for s in C_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    # roctx (rocprofiler-sdk) carries the profiler sections
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocprofiler-sdk-roctx"])
     shutil.move(tmp, LIB)
     return LIB
 

@@ -361,6 +361,29 @@ void build_full_from_triu(int n, const int* p, const int* i, std::vector<int>& r
 
 extern "C" {
 
+OSQPInt osqp_amd_matrix_export(const OSQPMatrix* M, OSQPInt* nnz_csr, OSQPInt* nnz_in, OSQPInt* rp, OSQPInt* ci,
+                               OSQPFloat* val, OSQPInt* trp, OSQPInt* tci, OSQPFloat* tval, OSQPInt* csc2csr,
+                               OSQPInt* csc2mirror, OSQPInt* diag) {
+  if (!M) return 1;
+  const int rows = M->is_triu ? M->n : M->m;
+  if (nnz_csr) *nnz_csr = M->nnz;
+  if (nnz_in) *nnz_in = M->nnz_in;
+  if (rp) download(rp, M->rp, sizeof(int) * ((size_t)rows + 1));
+  if (ci && M->nnz) download(ci, M->ci, sizeof(int) * (size_t)M->nnz);
+  if (val && M->nnz) download(val, M->val, sizeof(double) * (size_t)M->nnz);
+  if (!M->is_triu) {
+    if (trp) download(trp, M->trp, sizeof(int) * ((size_t)M->n + 1));
+    if (tci && M->nnz_in) download(tci, M->tci, sizeof(int) * (size_t)M->nnz_in);
+    if (tval && M->nnz_in) download(tval, M->tval, sizeof(double) * (size_t)M->nnz_in);
+  }
+  if (csc2csr && M->nnz_in) download(csc2csr, M->csc2csr, sizeof(int) * (size_t)M->nnz_in);
+  if (M->is_triu) {
+    if (csc2mirror && M->nnz_in) download(csc2mirror, M->csc2mirror, sizeof(int) * (size_t)M->nnz_in);
+    if (diag && M->n) download(diag, M->diag_pos, sizeof(int) * (size_t)M->n);
+  }
+  return 0;
+}
+
 OSQPMatrix* OSQPMatrix_new_from_csc(const OSQPCscMatrix* A, OSQPInt is_triu) {
   OSQPMatrix* M = (OSQPMatrix*)calloc(1, sizeof(OSQPMatrix));
   if (!M) return nullptr;

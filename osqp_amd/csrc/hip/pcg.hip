@@ -48,6 +48,7 @@
 #include "hip_internal.h"
 #include "spmv_rb.h"
 #include "panel.h"
+#include "../profilers.h"
 
 using namespace osqp_hip;
 
@@ -1116,6 +1117,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
       stop_at = it;
       break;
     }
+    osqp_amd_sec_push(OSQP_AMD_SEC_LINSYS_MVM);
     if (ahead) {
       // the look-ahead SpMV was left out where the loop was predicted to stop
       if (it == spec_at) av_spmv<V>(s, (const V*)p, w);
@@ -1124,6 +1126,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
       op_av_rho<V>(s, (const V*)p, w);
     }
     const int nk = op_kv<V>(s, (const V*)p, (const V*)w, (const double*)nullptr, Kp, part2);
+    osqp_amd_sec_pop(OSQP_AMD_SEC_LINSYS_MVM);
     hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nk, it, guard, n,
                        d, r, y, (const V*)p, (const V*)Kp, dinv, part0, part1, (const double*)part3,
                        (const double*)part4, gE, s->max_iter, s->d_flags);
@@ -1180,11 +1183,13 @@ OSQPInt pcg_solve(LinSysSolver* self, OSQPVectorf* b, OSQPInt admm_iter) {
   }
   if (s->kver_P != s->P->version || s->kver_A != s->A->version || (s->bits == 32 && s->A->fp32_dirty))
     update_precond(s);
+  osqp_amd_sec_push(OSQP_AMD_SEC_LINSYS_SOLVE);
   if (s->bits == 32) {
     solve_impl<float>(s, b, admm_iter);
   } else {
     solve_impl<double>(s, b, admm_iter);
   }
+  osqp_amd_sec_pop(OSQP_AMD_SEC_LINSYS_SOLVE);
   return 0;
 }
 

@@ -114,6 +114,25 @@ using namespace osqp_hip;
 
 extern "C" {
 
+OSQPInt osqp_amd_ptr_is_device(const void* p) {
+  if (!p) return 0;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // plain host memory: not an error for the caller
+    return 0;
+  }
+  return a.type == hipMemoryTypeDevice ? 1 : 0;
+}
+
+OSQPInt osqp_amd_copy_to_host(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return 0;
+  if (!osqp_amd_ptr_is_device(src)) {
+    memcpy(dst, src, bytes);
+    return 0;
+  }
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+}
+
 OSQPInt osqp_algebra_linsys_supported(void) { return OSQP_CAPABILITY_INDIRECT_SOLVER; }
 
 enum osqp_linsys_solver_type osqp_algebra_default_linsys(void) { return OSQP_INDIRECT_SOLVER; }

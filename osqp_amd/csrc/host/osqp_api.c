@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include "host_internal.h"
+#include "../profilers.h"
 
 OSQPInt osqp_capabilities(void) {
   return osqp_algebra_linsys_supported() | OSQP_CAPABILITY_UPDATE_MATRICES;
@@ -116,14 +117,104 @@ static OSQPSettings* copy_settings(const OSQPSettings* s) {
 /* ---- setup: src/osqp_api.c:332-631 ------------------------------------------ */
 #define ALLOC_OR_FAIL(ptr) do { if (!(ptr)) return osqp_error(OSQP_MEM_ALLOC_ERROR); } while (0)
 
+static OSQPInt setup_workspace(OSQPSolver** solverp, const OSQPCscMatrix* P, const OSQPFloat* q,
+                               const OSQPCscMatrix* A, const OSQPFloat* l, const OSQPFloat* u, OSQPInt m, OSQPInt n,
+                               const OSQPSettings* settings);
+
+/* Host copies of a problem's arrays for a caller that passes device memory
+ * (an extension: the reference's setup reads host arrays, src/auxil.c:
+ * 950-1017).  Host arrays are used in place. */
+typedef struct {
+  OSQPCscMatrix P, A;
+  const OSQPFloat *q, *l, *u;
+  void* owned[9];
+  int n_owned;
+} host_inputs;
+
+static void* host_copy(host_inputs* H, const void* src, size_t bytes, int* fail) {
+  void* dst = malloc(bytes ? bytes : 1);
+  if (!dst) {
+    *fail = 1;
+    return NULL;
+  }
+  H->owned[H->n_owned++] = dst;
+  if (bytes && osqp_amd_copy_to_host(dst, src, bytes)) *fail = 1;
+  return dst;
+}
+
+static int make_host_inputs(host_inputs* H, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
+                            const OSQPFloat* l, const OSQPFloat* u, OSQPInt m, OSQPInt n) {
+  const void* ptrs[9];
+  int k, dev = 0, fail = 0;
+  memset(H, 0, sizeof(*H));
+  if (!P || !A) return 0;
+  H->P = *P;
+  H->A = *A;
+  H->q = q;
+  H->l = l;
+  H->u = u;
+  ptrs[0] = P->p, ptrs[1] = P->i, ptrs[2] = P->x, ptrs[3] = A->p, ptrs[4] = A->i, ptrs[5] = A->x;
+  ptrs[6] = q, ptrs[7] = l, ptrs[8] = u;
+  for (k = 0; k < 9; k++) dev |= osqp_amd_ptr_is_device(ptrs[k]) ? 1 : 0;
+  if (!dev || n < 0 || m < 0 || !P->p || !A->p) return 0;
+  H->P.p = (OSQPInt*)host_copy(H, P->p, sizeof(OSQPInt) * ((size_t)n + 1), &fail);
+  H->A.p = (OSQPInt*)host_copy(H, A->p, sizeof(OSQPInt) * ((size_t)n + 1), &fail);
+  if (fail) return 1;
+  H->P.i = (OSQPInt*)host_copy(H, P->i, sizeof(OSQPInt) * (size_t)H->P.p[n], &fail);
+  H->P.x = (OSQPFloat*)host_copy(H, P->x, sizeof(OSQPFloat) * (size_t)H->P.p[n], &fail);
+  H->A.i = (OSQPInt*)host_copy(H, A->i, sizeof(OSQPInt) * (size_t)H->A.p[n], &fail);
+  H->A.x = (OSQPFloat*)host_copy(H, A->x, sizeof(OSQPFloat) * (size_t)H->A.p[n], &fail);
+  if (q) H->q = (const OSQPFloat*)host_copy(H, q, sizeof(OSQPFloat) * (size_t)n, &fail);
+  if (l) H->l = (const OSQPFloat*)host_copy(H, l, sizeof(OSQPFloat) * (size_t)m, &fail);
+  if (u) H->u = (const OSQPFloat*)host_copy(H, u, sizeof(OSQPFloat) * (size_t)m, &fail);
+  return fail;
+}
+
+static void free_host_inputs(host_inputs* H) {
+  int k;
+  for (k = 0; k < H->n_owned; k++) free(H->owned[k]);
+  H->n_owned = 0;
+}
+
 OSQPInt osqp_setup(OSQPSolver** solverp, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
                    const OSQPFloat* l, const OSQPFloat* u, OSQPInt m, OSQPInt n, const OSQPSettings* settings) {
   OSQPInt exitflag;
+  host_inputs H;
+  if (make_host_inputs(&H, P, q, A, l, u, m, n)) {
+    free_host_inputs(&H);
+    return osqp_error(OSQP_DATA_VALIDATION_ERROR);
+  }
+  if (H.n_owned) {
+    P = &H.P;
+    A = &H.A;
+    q = H.q;
+    l = H.l;
+    u = H.u;
+  }
+  exitflag = validate_data(P, q, A, l, u, m, n);
+  if (exitflag) {
+    free_host_inputs(&H);
+    return osqp_error(OSQP_DATA_VALIDATION_ERROR);
+  }
+  if (validate_settings(settings, 1)) {
+    free_host_inputs(&H);
+    return osqp_error(OSQP_SETTINGS_VALIDATION_ERROR);
+  }
+  /* src/osqp_api.c:353-354,621 */
+  osqp_amd_profiler_level(settings->profiler_level);
+  osqp_amd_sec_push(OSQP_AMD_SEC_SETUP);
+  exitflag = setup_workspace(solverp, P, q, A, l, u, m, n, settings);
+  osqp_amd_sec_pop(OSQP_AMD_SEC_SETUP);
+  free_host_inputs(&H);
+  return exitflag;
+}
+
+static OSQPInt setup_workspace(OSQPSolver** solverp, const OSQPCscMatrix* P, const OSQPFloat* q,
+                               const OSQPCscMatrix* A, const OSQPFloat* l, const OSQPFloat* u, OSQPInt m, OSQPInt n,
+                               const OSQPSettings* settings) {
+  OSQPInt exitflag;
   OSQPSolver* solver;
   OSQPWorkspace* work;
-
-  if (validate_data(P, q, A, l, u, m, n)) return osqp_error(OSQP_DATA_VALIDATION_ERROR);
-  if (validate_settings(settings, 1)) return osqp_error(OSQP_SETTINGS_VALIDATION_ERROR);
 
   solver = (OSQPSolver*)calloc(1, sizeof(OSQPSolver));
   ALLOC_OR_FAIL(solver);
@@ -210,7 +301,9 @@ OSQPInt osqp_setup(OSQPSolver** solverp, const OSQPCscMatrix* P, const OSQPFloat
     work->D_temp_A = OSQPVectorf_calloc(n);
     work->E_temp = OSQPVectorf_calloc(m);
     ALLOC_OR_FAIL(work->D_temp && work->D_temp_A && work->E_temp);
+    osqp_amd_sec_push(OSQP_AMD_SEC_SCALE);
     scale_data(solver);
+    osqp_amd_sec_pop(OSQP_AMD_SEC_SCALE);
   } else {
     work->scaling = NULL;
     work->D_temp = NULL;
@@ -225,8 +318,10 @@ OSQPInt osqp_setup(OSQPSolver** solverp, const OSQPCscMatrix* P, const OSQPFloat
     work->rho_inv = 1. / settings->rho;
   }
 
+  osqp_amd_sec_push(OSQP_AMD_SEC_LINSYS_INIT);
   exitflag = osqp_algebra_init_linsys_solver(&(work->linsys_solver), work->data->P, work->data->A, work->rho_vec,
                                              solver->settings, &work->scaled_prim_res, &work->scaled_dual_res, 0);
+  osqp_amd_sec_pop(OSQP_AMD_SEC_LINSYS_INIT);
   if (exitflag == OSQP_NONCVX_ERROR) {
     update_status(solver->info, OSQP_NON_CVX);
     return osqp_error(exitflag);
@@ -312,21 +407,42 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
   if (work->clear_update_time == 1) solver->info->update_time = 0.0;
   work->rho_update_from_solve = 1;
   osqp_amd_tic(work);
+  osqp_amd_sec_push(OSQP_AMD_SEC_OPT_SOLVE);
   if (settings->verbose) printf("iter   objective    prim res   dual res   rho        time\n");
+  osqp_amd_interrupt_listen();  /* src/osqp_api.c:690-693 */
   if (!settings->warm_starting) osqp_cold_start(solver);
   work->rhs_ready = 0;
 
   for (iter = 1; iter <= settings->max_iter; iter++) {
+    osqp_amd_sec_push(OSQP_AMD_SEC_ADMM_ITER);
     swap_vectors(&(work->x), &(work->x_prev));
     swap_vectors(&(work->z), &(work->z_prev));
     work->tol_valid = 0;  /* the iterates change: update_info recomputes the tolerance norms */
+    osqp_amd_sec_push(OSQP_AMD_SEC_ADMM_KKT_SOLVE);
     update_xz_tilde(solver, iter);
+    osqp_amd_sec_pop(OSQP_AMD_SEC_ADMM_KKT_SOLVE);
+    osqp_amd_sec_push(OSQP_AMD_SEC_ADMM_UPDATE);
     if (settings->rho_is_vec) {
+      /* one fused kernel: the projection section covers the whole update */
+      osqp_amd_sec_push(OSQP_AMD_SEC_ADMM_PROJ);
       update_xzy(solver);
+      osqp_amd_sec_pop(OSQP_AMD_SEC_ADMM_PROJ);
     } else {
       update_x(solver);
+      osqp_amd_sec_push(OSQP_AMD_SEC_ADMM_PROJ);
       update_z(solver);
+      osqp_amd_sec_pop(OSQP_AMD_SEC_ADMM_PROJ);
       update_y(solver);
+    }
+    osqp_amd_sec_pop(OSQP_AMD_SEC_ADMM_UPDATE);
+    osqp_amd_sec_pop(OSQP_AMD_SEC_ADMM_ITER);
+
+    /* src/osqp_api.c:728-737 */
+    if (osqp_amd_interrupted()) {
+      update_status(solver->info, OSQP_SIGINT);
+      if (settings->verbose) printf("Solver interrupted\n");
+      exitflag = 1;
+      goto exit;
     }
 
     if (work->first_run) temp_run_time = solver->info->setup_time + osqp_amd_toc(work);
@@ -372,6 +488,7 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
     if (can_adapt && (settings->adaptive_rho == OSQP_ADAPTIVE_RHO_UPDATE_KKT_ERROR))
       can_adapt = (solver->info->rel_kkt_error <= settings->adaptive_rho_fraction * work->last_rel_kkt) ? 1 : 0;
     if (can_adapt) {
+      osqp_amd_event_mark(OSQP_AMD_EVENT_RHO_UPDATE);
       if (adapt_rho(solver)) {
         fprintf(stderr, "ERROR in osqp_solve: Failed rho update\n");
         exitflag = 1;
@@ -408,7 +525,9 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
   /* src/osqp_api.c:981-990 */
   solver->info->polish_time = 0.0;
   if (settings->polishing && (solver->info->status_val == OSQP_SOLVED)) {
+    osqp_amd_sec_push(OSQP_AMD_SEC_POLISH);
     exitflag = polish(solver);
+    osqp_amd_sec_pop(OSQP_AMD_SEC_POLISH);
     if (exitflag > 0) {
       fprintf(stderr, "ERROR in osqp_solve: Failed polishing\n");
       goto exit;
@@ -427,6 +546,8 @@ OSQPInt osqp_solve(OSQPSolver* solver) {
   store_solution(solver, solver->solution);
 
 exit:
+  osqp_amd_interrupt_restore();
+  osqp_amd_sec_pop(OSQP_AMD_SEC_OPT_SOLVE);
   return exitflag;
 }
 
@@ -662,6 +783,7 @@ OSQPInt osqp_update_settings(OSQPSolver* solver, const OSQPSettings* new_setting
   settings = solver->settings;
   if (validate_settings(new_settings, 0)) return osqp_error(OSQP_SETTINGS_VALIDATION_ERROR);
   settings->profiler_level = new_settings->profiler_level;
+  osqp_amd_profiler_level(settings->profiler_level);  /* src/osqp_api.c:1490-1491 */
   settings->verbose = new_settings->verbose;
   settings->warm_starting = new_settings->warm_starting;
   settings->polishing = new_settings->polishing;
@@ -707,4 +829,26 @@ OSQPInt osqp_amd_time_fp64_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_
 OSQPInt osqp_amd_pcg_bytes(OSQPSolver* solver, double* per_cg_iter, double* per_solve) {
   if (!solver || !solver->work || !solver->work->linsys_solver) return 1;
   return osqp_amd_linsys_bytes(solver->work->linsys_solver, per_cg_iter, per_solve);
+}
+
+OSQPInt osqp_amd_solver_matrices(OSQPSolver* solver, const OSQPMatrix** P, const OSQPMatrix** A) {
+  if (!solver || !solver->work || !solver->work->data) return 1;
+  if (P) *P = solver->work->data->P;
+  if (A) *A = solver->work->data->A;
+  return 0;
+}
+
+OSQPInt osqp_amd_solver_scaling(OSQPSolver* solver, OSQPFloat* D, OSQPFloat* E, OSQPFloat* c, OSQPFloat* q,
+                                OSQPFloat* l, OSQPFloat* u) {
+  OSQPWorkspace* work;
+  if (!solver || !solver->work || !solver->work->data) return 1;
+  work = solver->work;
+  if (q) OSQPVectorf_to_raw(q, work->data->q);
+  if (l && work->data->m) OSQPVectorf_to_raw(l, work->data->l);
+  if (u && work->data->m) OSQPVectorf_to_raw(u, work->data->u);
+  if (!work->scaling) return 1;
+  if (D) OSQPVectorf_to_raw(D, work->scaling->D);
+  if (E && work->data->m) OSQPVectorf_to_raw(E, work->scaling->E);
+  if (c) *c = work->scaling->c;
+  return 0;
 }

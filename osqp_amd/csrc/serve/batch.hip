@@ -38,8 +38,9 @@ namespace osqp_serve {
 struct BEntry {
   BProb* dev;           // the problem's descriptor in HBM
   const BProb* init;    // host-mapped descriptor of a new problem (cmd 1)
-  const double* upd_q;  // cmd 2: new q (raw) or null
-  int cmd;              // 0 continue, 1 setup + solve, 2 new solve
+  const double* upd_q;  // cmd 2 / 3: new q (raw) or null
+  const BMatUpd* upd_mat;  // cmd 3: new matrix values (host-mapped)
+  int cmd;              // 0 continue, 1 setup + solve, 2 new solve, 3 new matrix values + new solve
   int iters_done;       // written by the kernel
 };
 
@@ -56,8 +57,14 @@ constexpr int kCounterRing = 1024;
 template <typename V>
 __device__ void run_entry(const Team& t, BProb& P, const BEntry& E, int iters, int* done_iters) {
   Solver<V> sv(t, P);
-  if (E.cmd == 1) sv.setup();
-  else if (E.cmd == 2) sv.new_solve(E.upd_q);
+  if (E.cmd == 1) {
+    sv.setup();
+  } else if (E.cmd == 2) {
+    sv.new_solve(E.upd_q);
+  } else if (E.cmd == 3) {
+    sv.update_matrices(*E.upd_mat);
+    sv.new_solve(E.upd_q);
+  }
   int k = 0;
   if (sv.S.phase == kPhaseRun) {
     for (; k < iters;) {
@@ -85,7 +92,7 @@ __global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters) {
   __shared__ double red2[kMaxRedBlocks];
   extern __shared__ double dyn_lds[];
   __shared__ BProb P;
-  __shared__ int slot, done_iters;
+  __shared__ int slot, done_iters, decision;
   Team t;
   t.tid = threadIdx.x;
   t.T = blockDim.x;
@@ -94,6 +101,7 @@ __global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters) {
   t.W = blockDim.x / kWave;
   t.red = red;
   t.red2 = red2;
+  t.flag = &decision;
   t.lds = (char*)dyn_lds;
   t.lds_bytes = kTeamLdsBytes;
   const int ep = tab->epoch;
@@ -133,6 +141,55 @@ double now_s() {
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// copies bytes from host or device memory (callers may pass either)
+void copy_in(void* dst, const void* src, size_t bytes) {
+  if (osqp_amd_copy_to_host(dst, src, bytes)) fprintf(stderr, "osqp_batch: copy from device memory failed\n");
+}
+
+// A problem's arrays on the host: the caller's own where they are host
+// memory; copies where any of them is device memory.
+struct HostInputs {
+  OSQPCscMatrix P{}, A{};
+  const OSQPFloat *q = nullptr, *l = nullptr, *u = nullptr, *x0 = nullptr, *y0 = nullptr;
+  std::vector<OSQPInt> pp, pi, ap, ai;
+  std::vector<OSQPFloat> px, ax, vq, vl, vu, vx0, vy0;
+};
+
+bool host_inputs(const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A, const OSQPFloat* l,
+                 const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, HostInputs* H) {
+  H->P = *P;
+  H->A = *A;
+  H->q = q;
+  H->l = l;
+  H->u = u;
+  H->x0 = x0;
+  H->y0 = y0;
+  const void* ptrs[] = {P->p, P->i, P->x, A->p, A->i, A->x, q, l, u, x0, y0};
+  bool dev = false;
+  for (const void* v : ptrs) dev = dev || osqp_amd_ptr_is_device(v);
+  if (!dev) return true;
+  const size_t n = (size_t)P->n, m = (size_t)A->m;
+  bool good = true;
+  auto grab = [&](auto& vec, const auto* src, size_t count) {
+    vec.resize(count ? count : 1);
+    if (count) good = good && osqp_amd_copy_to_host(vec.data(), src, sizeof(vec[0]) * count) == 0;
+    return vec.data();
+  };
+  H->P.p = grab(H->pp, P->p, n + 1);
+  H->A.p = grab(H->ap, A->p, n + 1);
+  if (!good) return false;
+  H->P.i = grab(H->pi, P->i, (size_t)H->pp[n]);
+  H->P.x = grab(H->px, P->x, (size_t)H->pp[n]);
+  H->A.i = grab(H->ai, A->i, (size_t)H->ap[n]);
+  H->A.x = grab(H->ax, A->x, (size_t)H->ap[n]);
+  H->q = grab(H->vq, q, n);
+  H->l = l ? grab(H->vl, l, m) : nullptr;
+  H->u = u ? grab(H->vu, u, m) : nullptr;
+  H->x0 = x0 ? grab(H->vx0, x0, n) : nullptr;
+  H->y0 = y0 ? grab(H->vy0, y0, m) : nullptr;
+  return good;
+}
 
 // first-fit allocator over [0, size) with coalescing
 class RangeAlloc {
@@ -217,6 +274,13 @@ struct HostData {
   GroupDims gP, gA, gT;
   bool warm_x = false, warm_y = false;
   std::unique_ptr<char[]> img;  // stage bytes [st_static, st_total): static image, x0, y0
+  // resident problems: where each entry of the input CSC of P (triangle) and
+  // of A lives in the HBM copies, for value updates (osqp_update_data_mat):
+  // P entry k at row-of-column slot pslot_lo[k] and, off the diagonal, at
+  // the mirrored slot pslot_up[k] (-1 on the diagonal); A entry k at CSR slot
+  // aslot[k] (and at slot k of A', which is the CSC itself)
+  std::vector<int> pslot_lo, pslot_up, aslot;
+  int nnzP_in = 0;
 };
 
 // byte layout of a problem's HBM block and of its pinned images
@@ -234,7 +298,7 @@ struct Layout {
   size_t total;
   // pinned: init descriptor + static image + warm start; results (+ q buffer)
   size_t st_total, st_static, st_x0, st_y0;
-  size_t rs_total, rs_x, rs_y, rs_res, rs_q;
+  size_t rs_total, rs_x, rs_y, rs_pc, rs_dc, rs_res, rs_q;
 };
 
 Layout make_layout(const HostData& h, int vbytes) {
@@ -326,12 +390,16 @@ Layout make_layout(const HostData& h, int vbytes) {
   L.st_y0 = s;
   s += h.warm_y ? 8 * m : 0;
   L.st_total = align_up(s, 256);
-  // pinned results: [x][y][BResult][q buffer]
+  // pinned results: [x][y][prim_inf_cert][dual_inf_cert][BResult][q buffer]
   size_t r = 0;
   L.rs_x = r;
   r = align_up(r + 8 * (n ? n : 1), 64);
   L.rs_y = r;
   r = align_up(r + 8 * (m ? m : 1), 64);
+  L.rs_pc = r;
+  r = align_up(r + 8 * (m ? m : 1), 64);
+  L.rs_dc = r;
+  r = align_up(r + 8 * (n ? n : 1), 64);
   L.rs_res = r;
   r = align_up(r + sizeof(BResult), 64);
   L.rs_q = r;
@@ -387,11 +455,18 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
 // symmetric CSR with sorted columns and diagonal positions; bounds clamped to
 // +-OSQP_INFTY as osqp_setup does; the row groups of all three; the warm start.
 Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
-                  const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0) {
+                  const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0,
+                  bool keep_maps) {
   const int n = P->n, m = A->m;
   h->n = n;
   h->m = m;
   h->nnzA = A->p[n];
+  h->nnzP_in = P->p[n];
+  if (keep_maps) {
+    h->pslot_lo.assign((size_t)h->nnzP_in, -1);
+    h->pslot_up.assign((size_t)h->nnzP_in, -1);
+    h->aslot.assign((size_t)h->nnzA, -1);
+  }
   // row pointers first: their sizes fix the layout (per-thread scratch keeps
   // a stream of adds free of allocations)
   thread_local std::vector<int> arp, prp, next;
@@ -433,6 +508,7 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
       const int pos = next[A->i[k]]++;
       Aci[pos] = j;
       Aval[pos] = A->x[k];
+      if (keep_maps) h->aslot[(size_t)k] = pos;
     }
   // P full: row r = column r of the triangle (rows i <= r) then the entries
   // (r, c) of the triangle's columns c > r, ascending
@@ -449,6 +525,7 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
       Pci[pos] = P->i[k];
       Pval[pos] = P->x[k];
       if (P->i[k] == c) Pdiag[c] = pos;
+      if (keep_maps) h->pslot_lo[(size_t)k] = pos;
     }
   for (int c = 0; c < n; ++c)  // upper part of row r: (r, c) for c > r
     for (int k = P->p[c]; k < P->p[c + 1]; ++k) {
@@ -457,6 +534,7 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
       const int pos = next[r]++;
       Pci[pos] = c;
       Pval[pos] = P->x[k];
+      if (keep_maps) h->pslot_up[(size_t)k] = pos;
     }
   // a triangle column's rows are not required to be sorted: sort each row
   for (int r = 0; r < n; ++r) {
@@ -464,16 +542,26 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
     bool sorted = true;
     for (int k = s + 1; k < e && sorted; ++k) sorted = Pci[k - 1] < Pci[k];
     if (!sorted) {
-      std::vector<std::pair<int, double>> row;
-      for (int k = s; k < e; ++k) row.emplace_back(Pci[k], Pval[k]);
-      std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+      std::vector<std::pair<int, int>> row;  // (column, old slot)
+      std::vector<double> v(Pval + s, Pval + e);
+      for (int k = s; k < e; ++k) row.emplace_back(Pci[k], k);
+      std::sort(row.begin(), row.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
         return a.first < b.first;
       });
+      std::vector<int> moved_to((size_t)(e - s));
       for (int k = s; k < e; ++k) {
         Pci[k] = row[k - s].first;
-        Pval[k] = row[k - s].second;
+        Pval[k] = v[(size_t)(row[k - s].second - s)];
+        moved_to[(size_t)(row[k - s].second - s)] = k;
         if (Pci[k] == r) Pdiag[r] = k;
       }
+      if (keep_maps)
+        for (int k = 0; k < h->nnzP_in; ++k) {
+          int& lo = h->pslot_lo[(size_t)k];
+          int& hi = h->pslot_up[(size_t)k];
+          if (lo >= s && lo < e) lo = moved_to[(size_t)(lo - s)];
+          if (hi >= s && hi < e) hi = moved_to[(size_t)(hi - s)];
+        }
     }
   }
   (void)nnzP;
@@ -503,6 +591,11 @@ struct HProb {
   size_t dev_off = 0, stage_off = 0, res_off = 0;
   double t_req = 0.0, latency = 0.0;
   long long iters_since_add = 0;
+  // matrix values staged by osqp_batch_update_data_mat for the next resolve
+  std::vector<int> up_pslot, up_aslot, up_tslot;
+  std::vector<double> up_pval, up_aval;
+  bool has_upd = false;  // pinned record of the running cmd 3
+  size_t upd_off = 0;
 };
 
 constexpr int kBuckets = 6;
@@ -702,6 +795,12 @@ bool admit(Engine* E, HProb* p) {
   P->warm_y = h.warm_y ? (const double*)(st + L.st_y0) : nullptr;
   P->out_x = (double*)(rs + L.rs_x);
   P->out_y = (double*)(rs + L.rs_y);
+  P->out_pcert = (double*)(rs + L.rs_pc);
+  P->out_dcert = (double*)(rs + L.rs_dc);
+  // certificates start at zero, as osqp_setup's calloc'd solution does (a
+  // non-convex stop leaves them as they are)
+  memset(P->out_pcert, 0, sizeof(double) * (size_t)h.m);
+  memset(P->out_dcert, 0, sizeof(double) * (size_t)h.n);
   P->out = (BResult*)(rs + L.rs_res);
   memset(P->out, 0, sizeof(BResult));
   p->cmd = 1;
@@ -760,6 +859,15 @@ int prepare(const Engine* E, const OSQPCscMatrix* P, const OSQPFloat* q, const O
             const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, OSQPInt resident, HProb** out) {
   *out = nullptr;
   if (!P || !A || !q) return OSQP_DATA_VALIDATION_ERROR;
+  HostInputs H;
+  if (!host_inputs(P, q, A, l, u, x0, y0, &H)) return OSQP_DATA_VALIDATION_ERROR;
+  P = &H.P;
+  A = &H.A;
+  q = H.q;
+  l = H.l;
+  u = H.u;
+  x0 = H.x0;
+  y0 = H.y0;
   const int n = P->n, m = A->m;
   if (n <= 0 || m < 0 || P->m != n || A->n != n) return OSQP_DATA_VALIDATION_ERROR;
   if (m > 0 && (!l || !u)) return OSQP_DATA_VALIDATION_ERROR;
@@ -777,7 +885,7 @@ int prepare(const Engine* E, const OSQPCscMatrix* P, const OSQPFloat* q, const O
     if (l[i] > u[i]) return OSQP_DATA_VALIDATION_ERROR;
   HProb* p = new HProb();
   p->resident = resident ? 1 : 0;
-  p->L = build_host(&p->h, E->bits / 8, P, q, A, l, u, x0, y0);
+  p->L = build_host(&p->h, E->bits / 8, P, q, A, l, u, x0, y0, resident != 0);
   if (p->L.total > E->arena_size || p->L.st_total + p->L.rs_total > E->pool_size) {
     // could never be admitted: larger than the whole arena or staging pool
     fprintf(stderr, "osqp_batch: problem needs %zu bytes of device arena (of %zu) and %zu of staging (of %zu)\n",
@@ -820,8 +928,12 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
     fprintf(stderr, "osqp_batch: rho_is_vec = 1 and polishing = 0 are required\n");
     return OSQP_SETTINGS_VALIDATION_ERROR;
   }
-  if (s->adaptive_rho != OSQP_ADAPTIVE_RHO_UPDATE_DISABLED && s->adaptive_rho != OSQP_ADAPTIVE_RHO_UPDATE_ITERATIONS) {
-    fprintf(stderr, "osqp_batch: adaptive_rho must be disabled or iteration-based\n");
+  if (s->adaptive_rho < OSQP_ADAPTIVE_RHO_UPDATE_DISABLED || s->adaptive_rho > OSQP_ADAPTIVE_RHO_UPDATE_KKT_ERROR) {
+    fprintf(stderr, "osqp_batch: adaptive_rho must be 0, 1, 2 or 3\n");
+    return OSQP_SETTINGS_VALIDATION_ERROR;
+  }
+  if (s->time_limit <= 0 || s->adaptive_rho_fraction <= 0) {
+    fprintf(stderr, "osqp_batch: time_limit and adaptive_rho_fraction must be positive\n");
     return OSQP_SETTINGS_VALIDATION_ERROR;
   }
   if (s->rho <= 0 || s->sigma <= 0 || s->alpha <= 0 || s->alpha >= 2 || s->max_iter <= 0 || s->eps_abs < 0 ||
@@ -833,6 +945,20 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
     return OSQP_SETTINGS_VALIDATION_ERROR;
   }
   if (osqp_algebra_init_libs(s->device)) return OSQP_MEM_ALLOC_ERROR;
+  // time_limit in ticks of the device wall clock (wall_clock64); limits of a
+  // million seconds or more (the default is 1e10 s) are treated as none, so
+  // the default loop reads no clock
+  long long tl_ticks = 0;
+  if (s->time_limit < 1e6) {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) != hipSuccess || khz <= 0) {
+      fprintf(stderr, "osqp_batch: no device wall clock for time_limit\n");
+      osqp_algebra_free_libs();
+      return OSQP_SETTINGS_VALIDATION_ERROR;
+    }
+    tl_ticks = (long long)(s->time_limit * 1e3 * (double)khz);
+    if (tl_ticks < 1) tl_ticks = 1;
+  }
   OSQPBatch* b = new OSQPBatch();
   Engine* E = &b->e;
   BSettings& B = E->set;
@@ -848,11 +974,15 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
   B.max_iter = s->max_iter;
   B.scaling = s->scaling;
   B.check_term = s->check_termination;
-  B.adaptive = s->adaptive_rho == OSQP_ADAPTIVE_RHO_UPDATE_ITERATIONS;
+  B.adaptive = (int)s->adaptive_rho;
   B.ad_interval = s->adaptive_rho_interval;
-  if (B.adaptive && B.ad_interval == 0)  // osqp_setup's default (osqp_api.c:615-620)
+  B.ad_fraction = s->adaptive_rho_fraction;
+  // osqp_setup's default intervals (src/osqp_api.c:607-620); the time mode
+  // fixes its interval during the first solve
+  if (B.adaptive == OSQP_ADAPTIVE_RHO_UPDATE_ITERATIONS && B.ad_interval == 0)
     B.ad_interval = s->check_termination ? OSQP_ADAPTIVE_RHO_MULTIPLE_TERMINATION * s->check_termination
                                          : OSQP_ADAPTIVE_RHO_FIXED;
+  if (B.adaptive == OSQP_ADAPTIVE_RHO_UPDATE_KKT_ERROR && B.ad_interval == 0) B.ad_interval = 1;
   B.scaled_term = s->scaled_termination;
   B.check_gap = s->check_dualgap;
   B.warm = s->warm_starting;
@@ -863,6 +993,7 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, s->device) == hipSuccess && prop.multiProcessorCount > 0)
     E->cus = prop.multiProcessorCount;
+  B.tl_ticks = tl_ticks;
   bool good = ok(E, hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking), "stream");
   E->arena_size = arena_bytes > 0 ? (size_t)arena_bytes : ((size_t)4 << 30);
   good = good && ok(E, hipMalloc((void**)&E->arena, E->arena_size), "hipMalloc arena");
@@ -934,6 +1065,67 @@ OSQPInt osqp_batch_add_many(OSQPBatch* b, OSQPInt count, const OSQPCscMatrix* co
   return first_error;
 }
 
+OSQPInt osqp_batch_update_data_mat(OSQPBatch* b, OSQPInt id, const OSQPFloat* Px_new, const OSQPInt* Px_new_idx,
+                                   OSQPInt P_new_n, const OSQPFloat* Ax_new, const OSQPInt* Ax_new_idx,
+                                   OSQPInt A_new_n) {
+  if (!b) return -1;
+  Engine* E = &b->e;
+  auto it = E->probs.find(id);
+  if (it == E->probs.end()) return -1;
+  HProb* p = it->second;
+  if (!p->resident || p->state != kIdle || !p->has_dev || !p->has_res) return -1;
+  const HostData& h = p->h;
+  // the argument checks of src/osqp_api.c:1335-1366 (1: P, 2: A), plus
+  // index ranges, since the engine writes at the indexed slots
+  const int nnzP = h.nnzP_in, nnzA = h.nnzA;
+  if (P_new_n > nnzP || P_new_n < 0) return 1;
+  if (!Px_new_idx && P_new_n != 0 && P_new_n != nnzP) return 1;
+  if (P_new_n == 0) P_new_n = nnzP;
+  if (A_new_n > nnzA || A_new_n < 0) return 2;
+  if (!Ax_new_idx && A_new_n != 0 && A_new_n != nnzA) return 2;
+  if (A_new_n == 0) A_new_n = nnzA;
+  std::vector<double> pv, av;
+  std::vector<int> pi, ai;
+  if (Px_new) {
+    pv.resize((size_t)P_new_n);
+    copy_in(pv.data(), Px_new, sizeof(double) * pv.size());
+    if (Px_new_idx) {
+      pi.resize((size_t)P_new_n);
+      copy_in(pi.data(), Px_new_idx, sizeof(int) * pi.size());
+      for (int k : pi)
+        if (k < 0 || k >= nnzP) return 1;
+    }
+  }
+  if (Ax_new) {
+    av.resize((size_t)A_new_n);
+    copy_in(av.data(), Ax_new, sizeof(double) * av.size());
+    if (Ax_new_idx) {
+      ai.resize((size_t)A_new_n);
+      copy_in(ai.data(), Ax_new_idx, sizeof(int) * ai.size());
+      for (int k : ai)
+        if (k < 0 || k >= nnzA) return 2;
+    }
+  }
+  // each P triangle entry goes to its slot and, off the diagonal, to the
+  // mirrored one; each A entry to its CSR slot and to its A' (CSC) slot
+  for (size_t j = 0; j < pv.size(); ++j) {
+    const int k = pi.empty() ? (int)j : pi[j];
+    p->up_pslot.push_back(h.pslot_lo[(size_t)k]);
+    p->up_pval.push_back(pv[j]);
+    if (h.pslot_up[(size_t)k] >= 0) {
+      p->up_pslot.push_back(h.pslot_up[(size_t)k]);
+      p->up_pval.push_back(pv[j]);
+    }
+  }
+  for (size_t j = 0; j < av.size(); ++j) {
+    const int k = ai.empty() ? (int)j : ai[j];
+    p->up_aslot.push_back(h.aslot[(size_t)k]);
+    p->up_tslot.push_back(k);
+    p->up_aval.push_back(av[j]);
+  }
+  return 0;
+}
+
 OSQPInt osqp_batch_resolve(OSQPBatch* b, OSQPInt id, const OSQPFloat* q_new) {
   if (!b) return -1;
   Engine* E = &b->e;
@@ -941,12 +1133,43 @@ OSQPInt osqp_batch_resolve(OSQPBatch* b, OSQPInt id, const OSQPFloat* q_new) {
   if (it == E->probs.end()) return -1;
   HProb* p = it->second;
   if (!p->resident || p->state != kIdle || !p->has_dev || !p->has_res) return -1;
+  p->cmd = 2;
+  if (!p->up_pslot.empty() || !p->up_aslot.empty()) {
+    // the staged matrix values, as one host-mapped record read by the GPU
+    const size_t np = p->up_pslot.size(), na = p->up_aslot.size();
+    const size_t o_ps = align_up(sizeof(BMatUpd), 64), o_pv = align_up(o_ps + 4 * np, 64);
+    const size_t o_as = align_up(o_pv + 8 * np, 64), o_ts = align_up(o_as + 4 * na, 64);
+    const size_t o_av = align_up(o_ts + 4 * na, 64), bytes = o_av + 8 * na;
+    size_t off = 0;
+    if (!E->pool_alloc.alloc(bytes, &off)) return -2;
+    char* rec = E->pool + off;
+    memcpy(rec + o_ps, p->up_pslot.data(), 4 * np);
+    memcpy(rec + o_pv, p->up_pval.data(), 8 * np);
+    memcpy(rec + o_as, p->up_aslot.data(), 4 * na);
+    memcpy(rec + o_ts, p->up_tslot.data(), 4 * na);
+    memcpy(rec + o_av, p->up_aval.data(), 8 * na);
+    BMatUpd* u = (BMatUpd*)rec;
+    u->np = (int)np;
+    u->na = (int)na;
+    u->pslot = (const int*)(rec + o_ps);
+    u->pval = (const double*)(rec + o_pv);
+    u->aslot = (const int*)(rec + o_as);
+    u->tslot = (const int*)(rec + o_ts);
+    u->aval = (const double*)(rec + o_av);
+    p->up_pslot.clear();
+    p->up_pval.clear();
+    p->up_aslot.clear();
+    p->up_tslot.clear();
+    p->up_aval.clear();
+    p->upd_off = off;
+    p->has_upd = true;
+    p->cmd = 3;
+  }
   char* rs = E->pool + p->res_off;
   BResult* out = (BResult*)(rs + p->L.rs_res);
   memset(out, 0, sizeof(BResult));
   double* qb = (double*)(rs + p->L.rs_q);
-  if (q_new) memcpy(qb, q_new, sizeof(double) * (size_t)p->h.n);
-  p->cmd = 2;
+  if (q_new) copy_in(qb, q_new, sizeof(double) * (size_t)p->h.n);
   p->qnew = q_new ? 1 : 0;
   p->state = kActive;
   p->t_req = now_s();
@@ -983,6 +1206,10 @@ int finish_epoch(Engine* E) {
       p->iters_since_add += B.tab->e[i].iters_done;
       // setup done: the stage is no longer read
       if (p->cmd == 1) free_problem_memory(E, p, false, true, false);
+      if (p->cmd == 3 && p->has_upd) {
+        E->pool_alloc.release(p->upd_off);
+        p->has_upd = false;
+      }
       p->cmd = 0;
       const BResult* out = (const BResult*)(E->pool + p->res_off + p->L.rs_res);
       if (out->done) {
@@ -1070,7 +1297,8 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
       en.cmd = p->cmd;
       en.iters_done = 0;
       en.upd_q = nullptr;
-      if (p->cmd == 2 && p->qnew) en.upd_q = (const double*)(E->pool + p->res_off + p->L.rs_q);
+      if ((p->cmd == 2 || p->cmd == 3) && p->qnew) en.upd_q = (const double*)(E->pool + p->res_off + p->L.rs_q);
+      en.upd_mat = p->cmd == 3 ? (const BMatUpd*)(E->pool + p->upd_off) : nullptr;
     }
     const int c = B.trial < kNumCands ? B.trial : B.best;
     const int T = B.fixT ? B.fixT : kCands[c].T, tpc = B.fixT ? B.fixTpc : kCands[c].tpc;
@@ -1093,7 +1321,8 @@ OSQPInt osqp_batch_last_completed(const OSQPBatch* b, OSQPInt* ids, OSQPInt cap)
   return (OSQPInt)d.size();
 }
 
-OSQPInt osqp_batch_poll(OSQPBatch* b, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OSQPBatchInfo* info) {
+OSQPInt osqp_batch_poll(OSQPBatch* b, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OSQPFloat* prim_inf_cert,
+                        OSQPFloat* dual_inf_cert, OSQPBatchInfo* info) {
   if (!b) return -1;
   Engine* E = &b->e;
   auto it = E->probs.find(id);
@@ -1104,6 +1333,8 @@ OSQPInt osqp_batch_poll(OSQPBatch* b, OSQPInt id, OSQPFloat* x, OSQPFloat* y, OS
   const BResult* out = (const BResult*)(rs + p->L.rs_res);
   if (x) memcpy(x, rs + p->L.rs_x, sizeof(double) * (size_t)p->h.n);
   if (y && p->h.m) memcpy(y, rs + p->L.rs_y, sizeof(double) * (size_t)p->h.m);
+  if (prim_inf_cert && p->h.m) memcpy(prim_inf_cert, rs + p->L.rs_pc, sizeof(double) * (size_t)p->h.m);
+  if (dual_inf_cert) memcpy(dual_inf_cert, rs + p->L.rs_dc, sizeof(double) * (size_t)p->h.n);
   if (info) {
     info->status_val = out->status;
     info->iter = out->iter;

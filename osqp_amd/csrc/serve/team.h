@@ -52,6 +52,7 @@ enum : int {
   kDualInfeasible = 5,
   kDualInfeasibleInaccurate = 6,
   kMaxIterReached = 7,
+  kTimeLimit = 8,
   kNonCvx = 9,
   kUnsolved = 11
 };
@@ -66,6 +67,12 @@ constexpr double kRhoTol = 1e-4;
 constexpr double kRhoEqOverIneq = 1e3;
 constexpr double kZeroDeadzone = 1e-15;
 constexpr double kCgTolMin = 1e-7;
+constexpr int kCheckTermDefault = 25;  // OSQP_CHECK_TERMINATION
+// OSQP_NAN = (OSQPFloat)0x7fc00000 (include/public/osqp_api_constants.h:193):
+// the number 2143289344.0, not an IEEE NaN
+constexpr double kOsqpNan = 2143289344.0;
+// adaptive_rho modes (include/public/osqp_api_constants.h:125-128)
+enum : int { kAdaptOff = 0, kAdaptIters = 1, kAdaptTime = 2, kAdaptKkt = 3 };
 
 // CSR matrix with its row groups (rows of 1..kLongRow entries), long-row and
 // empty-row lists.  Slot (g, lane, e) of the groups sits at
@@ -107,12 +114,18 @@ struct BState {
   double rho;        // current scalar rho
   double obj_val, dual_obj_val, prim_res, dual_res, gap, primdual_int, rel_kkt, rho_estimate;
   double scaled_prim_res, scaled_dual_res, scaled_gap, xtPx, qtx, SC, last_rel_kkt;
+  // device wall-clock ticks: start of the solve's run time (setup start for a
+  // new problem, as the reference counts setup_time + solve time), start of
+  // the ADMM loop, and the setup phase's duration
+  long long t_start, t_solve, setup_ticks;
+  int ad_interval;   // rho adaptation interval (fixed by the time mode once)
 };
 
 // settings copied into every problem
 struct BSettings {
   double rho0, sigma, alpha, eps_abs, eps_rel, eps_pinf, eps_dinf;
-  double ad_tol, cg_tol_fraction;
+  double ad_tol, cg_tol_fraction, ad_fraction;
+  long long tl_ticks;  // time_limit in device wall-clock ticks, 0 = none
   int max_iter, scaling, check_term, adaptive, ad_interval, scaled_term, check_gap, warm;
   int cg_max_iter, cg_red_thresh;
 };
@@ -122,6 +135,18 @@ struct BResult {
   int status, iter, rho_updates, done;
   double obj_val, dual_obj_val, prim_res, dual_res, gap, rho_estimate, primdual_int, rel_kkt;
   long long cg_iters;
+};
+
+// new matrix values of a resident problem (osqp_update_data_mat), in host-
+// mapped memory: np values for slots of P's full CSR, then na values for A's
+// CSR slots with, at the same index, the A' slot of the same entry
+struct BMatUpd {
+  int np, na;
+  const int* pslot;
+  const double* pval;
+  const int* aslot;
+  const int* tslot;
+  const double* aval;
 };
 
 // one resident problem: device pointers into its arena block
@@ -145,6 +170,8 @@ struct BProb {
   const double* warm_y;
   double* out_x;         // results (pinned host)
   double* out_y;
+  double* out_pcert;     // primal / dual infeasibility certificates
+  double* out_dcert;
   BResult* out;
 };
 
@@ -155,7 +182,17 @@ struct Team {
   int lds_bytes;
   double* red;   // LDS, kMaxRedBlocks doubles
   double* red2;  // LDS, kMaxRedBlocks doubles (a second reduction in the same pass)
+  int* flag;     // LDS word for decisions that thread 0 takes for the team
   __device__ __forceinline__ void sync() const { __syncthreads(); }
+  // the team's common value of a decision taken by thread 0 (reads of the
+  // device clock differ between threads)
+  __device__ __forceinline__ int decide(int v0) const {
+    if (tid == 0) *flag = v0;
+    __syncthreads();
+    const int v = *flag;
+    __syncthreads();
+    return v;
+  }
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
@@ -997,7 +1034,7 @@ struct Solver {
     double ea = p.set.eps_abs, er = p.set.eps_rel, epi = p.set.eps_pinf, edi = p.set.eps_dinf;
     if ((S.prim_res > kInfty) || (S.dual_res > kInfty)) {
       S.status = kNonCvx;
-      S.obj_val = __longlong_as_double(0x7ff8000000000000LL);
+      S.obj_val = kOsqpNan;
       return 1;
     }
     if (approximate) {
@@ -1063,6 +1100,7 @@ struct Solver {
   // setup of a new problem (osqp_setup: scale_data, set_rho_vec, linsys
   // init, cold start) and an optional osqp_warm_start
   __device__ __forceinline__ void setup() {
+    S.t_start = (long long)wall_clock64();
     const int4* s4 = (const int4*)p.stage;
     int4* d4 = (int4*)p.arena_static;
     const long long n4 = p.stage_bytes / 16;
@@ -1095,12 +1133,56 @@ struct Solver {
     S.cg_total = 0;
     S.cg_eps_prev = 0.0;
     S.cg_red_factor = p.set.cg_tol_fraction;
+    S.ad_interval = p.set.ad_interval;
+    S.t_solve = (long long)wall_clock64();
+    S.setup_ticks = S.t_solve - S.t_start;
     S.phase = kPhaseRun;
+  }
+
+  // osqp_update_data_mat (src/osqp_api.c:1309-1409) on the GPU: the data
+  // are unscaled (src/scaling.c:169-192, same operation order), the new raw
+  // values written, the data scaled again from scratch and the
+  // preconditioner rebuilt; the iterates stay as they are
+  __device__ __forceinline__ void update_matrices(const BMatUpd& u) {
+    const int n = p.n, m = p.m;
+    if (p.set.scaling) {
+      const double ci = S.cinv;
+      for (int r = t.tid; r < n; r += t.T)
+        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) {
+          const int c = p.P.ci[k];
+          const int lo = c < r ? c : r, hi = c < r ? r : c;
+          p.P.val[k] = ((p.P.val[k] * ci) * p.Dinv[lo]) * p.Dinv[hi];
+        }
+      for (int r = t.tid; r < m; r += t.T)
+        for (int k = p.A.rp[r]; k < p.A.rp[r + 1]; ++k) p.A.val[k] = (p.A.val[k] * p.Einv[r]) * p.Dinv[p.A.ci[k]];
+      for (int r = t.tid; r < n; r += t.T) {
+        for (int k = p.At.rp[r]; k < p.At.rp[r + 1]; ++k)
+          p.At.val[k] = (p.At.val[k] * p.Einv[p.At.ci[k]]) * p.Dinv[r];
+        p.q[r] = (p.q[r] * ci) * p.Dinv[r];
+      }
+      for (int i = t.tid; i < m; i += t.T) {
+        p.l[i] = p.l[i] * p.Einv[i];
+        p.u[i] = p.u[i] * p.Einv[i];
+      }
+      t.sync();
+    }
+    for (int k = t.tid; k < u.np; k += t.T) p.P.val[u.pslot[k]] = u.pval[k];
+    for (int k = t.tid; k < u.na; k += t.T) {
+      p.A.val[u.aslot[k]] = u.aval[k];
+      p.At.val[u.tslot[k]] = u.aval[k];
+    }
+    t.sync();
+    if (p.set.scaling) scale_data();
+    fill_group_values();
+    update_precond();
+    S.ax_valid = 0;
+    t.sync();
   }
 
   // a new solve of a resident problem: osqp_update_data_vec(q) (if given),
   // reset_info, and osqp_solve's cold start when warm starting is off
   __device__ __forceinline__ void new_solve(const double* q_new) {
+    S.t_start = (long long)wall_clock64();
     if (q_new) {
       for (int i = t.tid; i < p.n; i += t.T) {
         double v = q_new[i];
@@ -1114,35 +1196,63 @@ struct Solver {
     S.iter = 0;
     S.can_check = 0;
     if (!p.set.warm) cold_start();
+    S.t_solve = (long long)wall_clock64();
     S.phase = kPhaseRun;
   }
 
-  // end of a solve: objective, final status, unscaled solution to the host
+  // end of a solve: objective, final status, unscaled solution and
+  // certificates to the host (src/osqp_api.c:908-948, src/auxil.c:598-674)
   __device__ __forceinline__ void finish() {
+    int iter_out = S.iter;
     if (!S.can_check) {
       update_info(S.iter);
       check_termination(0);
     }
+    // a time-limit stop reports the last completed iteration: the loop
+    // variable's value less one, as update_info(iter - 1) after the loop does
+    if (S.status == kTimeLimit) iter_out = S.iter - 1;
     if (has_solution()) compute_obj_gap(x(), p.y);
     if (S.status == kUnsolved) {
       if (!check_termination(1)) S.status = kMaxIterReached;
     }
+    if (S.status == kTimeLimit) {
+      if (!check_termination(1)) S.status = kTimeLimit;
+    }
     S.rho_estimate = rho_estimate();
-    const double nanv = __longlong_as_double(0x7ff8000000000000LL);
+    const bool prim_inf = S.status == kPrimalInfeasible || S.status == kPrimalInfeasibleInaccurate;
+    const bool dual_inf = S.status == kDualInfeasible || S.status == kDualInfeasibleInaccurate;
     if (has_solution()) {
-      for (int i = t.tid; i < p.n; i += t.T) p.out_x[i] = p.set.scaling ? x()[i] * p.D[i] : x()[i];
-      for (int i = t.tid; i < p.m; i += t.T)
+      for (int i = t.tid; i < p.n; i += t.T) {
+        p.out_x[i] = p.set.scaling ? x()[i] * p.D[i] : x()[i];
+        p.out_dcert[i] = kOsqpNan;
+      }
+      for (int i = t.tid; i < p.m; i += t.T) {
         p.out_y[i] = p.set.scaling ? (p.y[i] * p.E[i]) * S.cinv : p.y[i];
+        p.out_pcert[i] = kOsqpNan;
+      }
     } else {
-      for (int i = t.tid; i < p.n; i += t.T) p.out_x[i] = nanv;
-      for (int i = t.tid; i < p.m; i += t.T) p.out_y[i] = nanv;
+      for (int i = t.tid; i < p.n; i += t.T) p.out_x[i] = kOsqpNan;
+      for (int i = t.tid; i < p.m; i += t.T) p.out_y[i] = kOsqpNan;
+      // the certificate normalised by its inf-norm (mult_scalar by 1 / norm);
+      // the other one is OSQP_NAN; a non-convex stop writes neither
+      if (prim_inf) {
+        const double inv = 1.0 / norm_inf(p.dy, p.m);
+        for (int i = t.tid; i < p.m; i += t.T) p.out_pcert[i] = p.dy[i] * inv;
+        for (int i = t.tid; i < p.n; i += t.T) p.out_dcert[i] = kOsqpNan;
+      }
+      if (dual_inf) {
+        const double inv = 1.0 / norm_inf(p.dx, p.n);
+        for (int i = t.tid; i < p.n; i += t.T) p.out_dcert[i] = p.dx[i] * inv;
+        for (int i = t.tid; i < p.m; i += t.T) p.out_pcert[i] = kOsqpNan;
+      }
+      t.sync();
       cold_start();
     }
     t.sync();
     if (t.tid == 0) {
       BResult* o = p.out;
       o->status = S.status;
-      o->iter = S.iter;
+      o->iter = iter_out;
       o->rho_updates = S.rho_updates;
       o->obj_val = S.obj_val;
       o->dual_obj_val = S.dual_obj_val;
@@ -1204,12 +1314,43 @@ struct Solver {
       y[i] = y[i] + dyv;
     }
     t.sync();
+    // time limit (src/osqp_api.c:743-759), before this iteration's checks
+    if (p.set.tl_ticks > 0 && t.decide((long long)wall_clock64() - S.t_start >= p.set.tl_ticks)) {
+      S.status = kTimeLimit;
+      return 1;
+    }
     const int ct = p.set.check_term;
     const bool can_check = ct && (iter % ct == 0);
-    const bool can_adapt = p.set.adaptive && p.set.ad_interval && (iter % p.set.ad_interval == 0);
+    bool can_adapt = false;
+    switch (p.set.adaptive) {
+      case kAdaptTime:
+        // src/osqp_api.c:782-820: once the loop has run for the fraction of
+        // the setup time, the interval is fixed from the iteration count
+        // (with the reference's min(1, .) bound)
+        if (S.ad_interval == 0) {
+          if (!t.decide((double)((long long)wall_clock64() - S.t_solve) >
+                        p.set.ad_fraction * (double)S.setup_ticks))
+            break;
+          const double base = ct ? ct : kCheckTermDefault;
+          const double xv = (double)iter + 0.5 * base;
+          int iv = (int)(xv - fmod(xv, base));
+          iv = iv > ct ? iv : ct;
+          S.ad_interval = iv < 1 ? iv : 1;
+        }
+        // fall through
+      case kAdaptIters:
+      case kAdaptKkt:
+        can_adapt = S.ad_interval && (iter % S.ad_interval == 0);
+        break;
+      default:
+        break;
+    }
     S.can_check = can_check;
     if (can_check || can_adapt || iter == 1) update_info(iter);
     if (can_check && check_termination(0)) return 1;
+    // src/osqp_api.c:868-877: the KKT mode adapts only after the KKT error
+    // has fallen by the fraction since the last update
+    if (can_adapt && p.set.adaptive == kAdaptKkt) can_adapt = S.rel_kkt <= p.set.ad_fraction * S.last_rel_kkt;
     if (can_adapt) {
       const double rho_new = rho_estimate();
       S.rho_estimate = rho_new;

@@ -207,6 +207,32 @@ def primal_infeasibility(seed=2):
     return dict(P=P, q=q, A=A.tocsc(), l=l, u=u, status="primal infeasible")
 
 
+def update_matrices_problem():
+    """The data of the reference's matrix-update test
+    (tests/update_matrices/generate_problem.py:6-64, PCG64(1) draws through
+    scipy.sparse.random in the same order), with the solutions it holds
+    (:86-111) for the original problem and after new P, new A, and both."""
+    from numpy.random import PCG64, Generator
+    rg = Generator(PCG64(1))
+    n, m, dens = 5, 8, 0.7
+    A = sp.random(m, n, density=dens, format="csc", random_state=rg)
+    P = sp.random(n, n, density=dens, random_state=rg)
+    P = (P @ P.T).tocsc() + sp.eye(n, format="csc")
+    Pu = sp.triu(P, format="csc")
+    A_new = A.copy()
+    A_new.data += rg.standard_normal(A_new.nnz)
+    Pu_new = Pu.copy()
+    Pu_new.data += 0.1 * rg.standard_normal(Pu_new.nnz)
+    q = rg.standard_normal(n)
+    l = -30 + rg.standard_normal(m)
+    u = 30 + rg.standard_normal(m)
+    x = np.array([-0.34282147, 0.0260615, 0.27197987, -0.01202531, 0.44052732])
+    x_P = np.array([-0.36033252, 0.10729745, 0.22965285, 0.06323582, 0.42301398])
+    return dict(P=Pu, q=q, A=A, l=l, u=u, Pu_new=Pu_new, A_new=A_new,
+                x=x, obj=-0.37221, x_P_new=x_P, obj_P_new=-0.36765, x_A_new=x, obj_A_new=-0.37221,
+                x_PA_new=x_P, obj_PA_new=-0.36765, y=np.zeros(m))
+
+
 def load_fixture(path):
     """Reference data set extracted to JSON (tests/golden/*.json)."""
     import json

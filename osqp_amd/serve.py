@@ -44,11 +44,13 @@ def _bind(lib):
     lib.osqp_batch_add_many.restype = C.c_int
     lib.osqp_batch_resolve.argtypes = [vp, C.c_int, vp]
     lib.osqp_batch_resolve.restype = C.c_int
+    lib.osqp_batch_update_data_mat.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int]
+    lib.osqp_batch_update_data_mat.restype = C.c_int
     lib.osqp_batch_step.argtypes = [vp, C.POINTER(C.c_int)]
     lib.osqp_batch_step.restype = C.c_int
     lib.osqp_batch_last_completed.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
     lib.osqp_batch_last_completed.restype = C.c_int
-    lib.osqp_batch_poll.argtypes = [vp, C.c_int, vp, vp, C.POINTER(BatchInfo)]
+    lib.osqp_batch_poll.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.POINTER(BatchInfo)]
     lib.osqp_batch_poll.restype = C.c_int
     lib.osqp_batch_release.argtypes = [vp, C.c_int]
     lib.osqp_batch_release.restype = C.c_int
@@ -65,6 +67,22 @@ def _bind(lib):
     lib.osqp_batch_destroy.restype = C.c_int
     _bound = True
     return lib
+
+
+def _check_lengths(Ph, Ah, q, l, u, x0, y0):
+    """The C entry points read q, x0 over n entries and l, u, y0 over m."""
+    n, m = Ph.n, Ah.m
+    if Ah.n != n:
+        raise ValueError(f"A has {Ah.n} columns, P has {n}")
+    if q is None or len(q) != n:
+        raise ValueError(f"q must have n = {n} entries")
+    for name, v in (("l", l), ("u", u)):
+        if v is None or len(v) != m:
+            raise ValueError(f"{name} must have m = {m} entries")
+    if x0 is not None and len(x0) != n:
+        raise ValueError(f"x0 must have n = {n} entries")
+    if y0 is not None and len(y0) != m:
+        raise ValueError(f"y0 must have m = {m} entries")
 
 
 class BatchEngine:
@@ -88,6 +106,7 @@ class BatchEngine:
         Ph, Ah = T.CscHolder(P), T.CscHolder(A)
         q, l, u = T.as_f64(q), T.as_f64(l), T.as_f64(u)  # l, u are clamped to +-OSQP_INFTY by the engine
         x0, y0 = T.as_f64(x0), T.as_f64(y0)
+        _check_lengths(Ph, Ah, q, l, u, x0, y0)
         pid = C.c_int(0)
         rc = self.lib.osqp_batch_add(self._b, Ph.ptr, T.addr(q), Ah.ptr, T.addr(l), T.addr(u), T.addr(x0),
                                      T.addr(y0), 1 if resident else 0, C.byref(pid))
@@ -110,6 +129,7 @@ class BatchEngine:
             Ph, Ah = T.CscHolder(p["P"]), T.CscHolder(p["A"])
             q, l, u = T.as_f64(p["q"]), T.as_f64(p["l"]), T.as_f64(p["u"])
             x0, y0 = T.as_f64(p.get("x0")), T.as_f64(p.get("y0"))
+            _check_lengths(Ph, Ah, q, l, u, x0, y0)
             keep.append((Ph, Ah, q, l, u, x0, y0))
             Pp[i], Ap[i] = C.addressof(Ph.s), C.addressof(Ah.s)
             qp[i], lp[i], up[i], xp[i], yp[i] = T.addr(q), T.addr(l), T.addr(u), T.addr(x0), T.addr(y0)
@@ -126,9 +146,23 @@ class BatchEngine:
 
     def resolve(self, pid, q=None):
         q = T.as_f64(q)
+        if q is not None and pid in self._dims and len(q) != self._dims[pid][0]:
+            raise ValueError(f"q must have n = {self._dims[pid][0]} entries")
         rc = self.lib.osqp_batch_resolve(self._b, int(pid), T.addr(q))
         if rc:
             raise RuntimeError(f"osqp_batch_resolve({pid}) failed: not resident, unknown or busy")
+
+    def update_mat(self, pid, Px=None, Px_idx=None, Ax=None, Ax_idx=None):
+        """Stages new P (upper-triangle CSC order) / A (CSC order) values of a
+        resident, polled problem; the next resolve applies them."""
+        Pxn, Axn = T.as_f64(Px), T.as_f64(Ax)
+        Pin = None if Px_idx is None else np.ascontiguousarray(Px_idx, dtype=np.int32)
+        Ain = None if Ax_idx is None else np.ascontiguousarray(Ax_idx, dtype=np.int32)
+        rc = self.lib.osqp_batch_update_data_mat(
+            self._b, int(pid), T.addr(Pxn), T.addr(Pin), 0 if Pxn is None else len(Pxn), T.addr(Axn),
+            T.addr(Ain), 0 if Axn is None else len(Axn))
+        if rc:
+            raise ValueError(f"osqp_batch_update_data_mat({pid}) failed ({rc})")
 
     def step(self):
         """Runs one epoch; returns the ids whose solves ended in it."""
@@ -147,15 +181,18 @@ class BatchEngine:
         n, m = self._dims[pid]
         x = np.empty(n)
         y = np.empty(max(m, 1))
+        pc = np.empty(max(m, 1))
+        dc = np.empty(n)
         info = BatchInfo()
-        rc = self.lib.osqp_batch_poll(self._b, int(pid), T.addr(x), T.addr(y), C.byref(info))
+        rc = self.lib.osqp_batch_poll(self._b, int(pid), T.addr(x), T.addr(y), T.addr(pc), T.addr(dc),
+                                      C.byref(info))
         if rc < 0:
             raise KeyError(pid)
         if rc == 0:
             return None
         inf = SimpleNamespace(**{f: getattr(info, f) for f, _ in BatchInfo._fields_})
         inf.status = T.STATUS_NAMES.get(inf.status_val, "unknown")
-        return SimpleNamespace(x=x, y=y[:m], info=inf)
+        return SimpleNamespace(x=x, y=y[:m], prim_inf_cert=pc[:m], dual_inf_cert=dc, info=inf)
 
     def release(self, pid):
         rc = self.lib.osqp_batch_release(self._b, int(pid))

@@ -70,8 +70,48 @@ def load_library():
         lib.osqp_amd_pcg_bytes.restype = C.c_int
         lib.osqp_algebra_device_name.argtypes = [C.c_char_p, C.c_int]
         lib.osqp_amd_last_error.restype = C.c_int
+        lib.osqp_amd_profiler_counts.argtypes = [C.POINTER(C.c_longlong)] * 3
+        lib.osqp_amd_profiler_counts.restype = C.c_int
+        lib.osqp_amd_matrix_export.argtypes = [vp] + [vp] * 11
+        lib.osqp_amd_matrix_export.restype = C.c_int
+        lib.osqp_amd_solver_matrices.argtypes = [C.POINTER(T.OSQPSolver), C.POINTER(vp), C.POINTER(vp)]
+        lib.osqp_amd_solver_matrices.restype = C.c_int
+        lib.osqp_amd_solver_scaling.argtypes = [C.POINTER(T.OSQPSolver), vp, vp, C.POINTER(C.c_double), vp, vp, vp]
+        lib.osqp_amd_solver_scaling.restype = C.c_int
         _lib = lib
         return lib
+
+
+def profiler_counts():
+    """roctx ranges pushed / popped and events marked so far in this process."""
+    lib = load_library()
+    a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
+    lib.osqp_amd_profiler_counts(C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def matrix_export(handle, m, n, is_triu):
+    """Host copies of an OSQPMatrix's backend structures (see
+    osqp_amd_matrix_export in include/osqp_amd_algebra.h)."""
+    lib = load_library()
+    nc, ni = C.c_int(), C.c_int()
+    lib.osqp_amd_matrix_export(handle, C.addressof(nc), C.addressof(ni), *([None] * 9))
+    rows = n if is_triu else m
+    nnz, nin = nc.value, ni.value
+    out = dict(rp=np.zeros(rows + 1, np.int32), ci=np.zeros(max(nnz, 1), np.int32), val=np.zeros(max(nnz, 1)),
+               csc2csr=np.zeros(max(nin, 1), np.int32))
+    if is_triu:
+        out.update(mirror=np.zeros(max(nin, 1), np.int32), diag=np.zeros(max(n, 1), np.int32))
+    else:
+        out.update(trp=np.zeros(n + 1, np.int32), tci=np.zeros(max(nin, 1), np.int32), tval=np.zeros(max(nin, 1)))
+    g = lambda k: out[k].ctypes.data if k in out else None
+    lib.osqp_amd_matrix_export(handle, None, None, g("rp"), g("ci"), g("val"), g("trp"), g("tci"), g("tval"),
+                               g("csc2csr"), g("mirror"), g("diag"))
+    for k, ln in (("ci", nnz), ("val", nnz), ("csc2csr", nin), ("mirror", nin), ("diag", n), ("tci", nin),
+                  ("tval", nin)):
+        if k in out:
+            out[k] = out[k][:ln]
+    return out
 
 
 def default_settings():
@@ -216,6 +256,24 @@ class OSQP:
         if self.lib.osqp_amd_pcg_bytes(self._solver, C.byref(a), C.byref(b)):
             raise RuntimeError("osqp_amd_pcg_bytes failed")
         return dict(per_cg_iter=a.value, per_solve=b.value)
+
+    def scaling(self):
+        """Ruiz scaling after setup: D, E, c and the scaled q, l, u (None
+        when the solver runs without scaling)."""
+        n, m = self.n, self.m
+        D, E, q, l, u = np.zeros(n), np.zeros(max(m, 1)), np.zeros(n), np.zeros(max(m, 1)), np.zeros(max(m, 1))
+        c = C.c_double()
+        if self.lib.osqp_amd_solver_scaling(self._solver, D.ctypes.data, E.ctypes.data, C.byref(c), q.ctypes.data,
+                                            l.ctypes.data, u.ctypes.data):
+            return None
+        return D, E[:m], c.value, q, l[:m], u[:m]
+
+    def structures(self):
+        """The backend's CSR structures of P (full symmetric) and A, with
+        their maps from the input CSC entries (osqp_amd_matrix_export)."""
+        Ph, Ah = C.c_void_p(), C.c_void_p()
+        self.lib.osqp_amd_solver_matrices(self._solver, C.byref(Ph), C.byref(Ah))
+        return dict(P=matrix_export(Ph.value, self.n, self.n, True), A=matrix_export(Ah.value, self.m, self.n, False))
 
     def close(self):
         if self._solver:

@@ -1,0 +1,335 @@
+"""The batched serving engine against the CPU oracle, and its result contract.
+
+The engine has no counterpart in the reference, but each problem it solves is
+an OSQP problem: its per-problem arithmetic restates the host core's ADMM
+step and the PCG linear solver.  So its results are checked here against the
+oracle (oracle/osqp_oracle.c, FP64), not against the same library's
+single-problem engine:
+  * a seeded MPC stream (nx = 12, nu = 4, N ~ U{10..50}, 256 requests) and
+    portfolios of 500, 2000 and 5000 assets with new mu per request
+    (osqp_update_data_vec semantics, warm-started re-solves): every status
+    equals the oracle's PCG path (linsys_solver = 2, the same algorithm in
+    FP64), every objective is within 1e-4 relative of it, and x, y are within
+    the stated bounds;
+  * OSQP's result contract (src/auxil.c:598-674): infeasible problems return
+    x = y = OSQP_NAN and the normalised certificate, equal to the
+    single-problem engine's; a non-convex problem returns obj = OSQP_NAN;
+  * time_limit (src/osqp_api.c:743-759) stops a solve with status 8;
+  * osqp_update_data_mat (src/osqp_api.c:1309-1409) on a resident problem
+    reaches the reference's held solutions (tests/update_matrices).
+"""
+import numpy as np
+import pytest
+
+from oracle import OracleSolver
+from osqp_amd import OSQP, OSQP_NAN
+from osqp_amd import problems as pr
+from osqp_amd.serve import BatchEngine
+
+pytestmark = pytest.mark.gpu
+
+ST = dict(eps_abs=1e-4, eps_rel=1e-4, max_iter=4000)
+OBJ_TOL = 1e-4
+
+
+def oracle(p, linsys=2, **kw):
+    o = OracleSolver()
+    st = dict(ST)
+    st.update(kw)
+    assert o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=linsys, **st) == 0
+    o.solve()
+    return o
+
+
+def rel(a, b):
+    return abs(a - b) / max(1.0, abs(b))
+
+
+def vec_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(b)))) if len(b) else 0.0
+
+
+def run_all(eng, probs, resident=False):
+    ids = eng.add_many(probs, resident=resident)
+    got = {}
+    while eng.pending():
+        for i in eng.step():
+            got[i] = eng.poll(i)
+    return ids, [got.get(i) for i in ids]
+
+
+def mpc_stream(k, seed=7):
+    rng = np.random.default_rng(seed)
+    return [pr.mpc(int(N), seed=1000 * seed + j) for j, N in enumerate(rng.integers(10, 51, k))]
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_mpc_stream_vs_oracle(precision):
+    probs = mpc_stream(256)
+    eng = BatchEngine(precision=precision, **ST)
+    _, res = run_all(eng, probs)
+    eng.close()
+    worst = dict(obj=0.0, x=0.0, y=0.0, obj_direct=0.0)
+    for k, (p, r) in enumerate(zip(probs, res)):
+        o = oracle(p)
+        assert r.info.status_val == o.info.status_val, (k, r.info.status, o.info.status)
+        if o.info.status_val == 1:
+            worst["obj"] = max(worst["obj"], rel(r.info.obj_val, o.info.obj_val))
+            worst["x"] = max(worst["x"], vec_err(r.x, o.x))
+            worst["y"] = max(worst["y"], vec_err(r.y, o.y))
+    # the direct (LDL') path, the reference's default, on a sample: statuses
+    for p, r in list(zip(probs, res))[::16]:
+        od = oracle(p, linsys=1)
+        assert r.info.status_val == od.info.status_val
+        if od.info.status_val == 1:
+            worst["obj_direct"] = max(worst["obj_direct"], rel(r.info.obj_val, od.info.obj_val))
+    print(f"batch{precision} MPC vs oracle: {worst}")
+    assert worst["obj"] <= OBJ_TOL and worst["obj_direct"] <= OBJ_TOL
+    assert worst["x"] <= X_TOL_SERVE and worst["y"] <= Y_TOL_SERVE
+
+
+X_TOL_SERVE = 1e-3
+Y_TOL_SERVE = 1e-3
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("n", [500, 2000, 5000])
+def test_portfolio_updates_vs_oracle(n, precision):
+    """A resident portfolio re-solved after two new mu (osqp_update_data_vec
+    + warm-started osqp_solve in the oracle)."""
+    p = pr.portfolio(n, seed=40 + n)
+    k = p["k"]
+    rng = np.random.default_rng(n)
+    qs = [pr.portfolio_q(rng.standard_normal(n), k) for _ in range(2)]
+    o = oracle(p)
+    ref = [(o.info.status_val, o.info.obj_val, o.x, o.y)]
+    for q in qs:
+        o.update_data_vec(q=q)
+        o.solve()
+        ref.append((o.info.status_val, o.info.obj_val, o.x, o.y))
+    eng = BatchEngine(precision=precision, **ST)
+    pid = eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=True)
+    got = []
+    for j in range(len(qs) + 1):
+        if j:
+            eng.resolve(pid, qs[j - 1])
+        while eng.pending():
+            eng.step()
+        got.append(eng.poll(pid))
+    eng.close()
+    for j, (r, (st, obj, x, y)) in enumerate(zip(got, ref)):
+        print(f"portfolio{n} batch{precision} solve {j}: {r.info.status} it={r.info.iter} "
+              f"obj_rel={rel(r.info.obj_val, obj):.2e} x={vec_err(r.x, x):.2e} y={vec_err(r.y, y):.2e}")
+        assert r.info.status_val == st
+        assert rel(r.info.obj_val, obj) <= OBJ_TOL
+        assert vec_err(r.x, x) <= X_TOL_SERVE
+        assert vec_err(r.y, y) <= Y_TOL_SERVE
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("case", [1, 2, 3, 4])
+def test_infeasibility_contract(case, precision):
+    """Statuses, OSQP_NAN fills and certificates equal the single-problem
+    engine's (src/auxil.c:598-674)."""
+    p = pr.primal_dual_infeasibility(case)
+    eng = BatchEngine(precision=precision, **ST)
+    _, (rb,) = run_all(eng, [p])
+    eng.close()
+    s = OSQP(precision=precision)
+    assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **ST) == 0
+    rs = s.solve()
+    s.close()
+    assert rb.info.status == p["status"] == rs.info.status
+    if rs.info.status_val in (1, 2):
+        assert np.all(rb.prim_inf_cert == OSQP_NAN) and np.all(rb.dual_inf_cert == OSQP_NAN)
+        return
+    assert np.all(rb.x == OSQP_NAN) and np.all(rb.y == OSQP_NAN)
+    if "primal" in p["status"]:
+        cert, other = rb.prim_inf_cert, rb.dual_inf_cert
+        ref = rs.prim_inf_cert
+    else:
+        cert, other = rb.dual_inf_cert, rb.prim_inf_cert
+        ref = rs.dual_inf_cert
+    assert abs(np.max(np.abs(cert)) - 1.0) < 1e-12
+    assert np.all(other == OSQP_NAN)
+    np.testing.assert_allclose(cert, ref, rtol=0, atol=1e-6)
+
+
+def test_non_convex_obj_is_osqp_nan():
+    """src/auxil.c:831-840: diverging residuals end with status non-convex and
+    obj_val = OSQP_NAN (2143289344.0), x = y = OSQP_NAN."""
+    p = pr.non_cvx()
+    eng = BatchEngine(precision=64, eps_abs=1e-5, eps_rel=1e-5, max_iter=2000, adaptive_rho=0, sigma=5.0)
+    _, (r,) = run_all(eng, [p])
+    eng.close()
+    assert r.info.status_val == 9
+    assert r.info.obj_val == OSQP_NAN
+    assert np.all(r.x == OSQP_NAN) and np.all(r.y == OSQP_NAN)
+
+
+def test_time_limit_in_batch():
+    p = pr.portfolio(2000, seed=3)
+    eng = BatchEngine(precision=64, eps_abs=1e-12, eps_rel=1e-12, max_iter=10**8, check_termination=0,
+                      time_limit=0.05)
+    _, (r,) = run_all(eng, [p])
+    eng.close()
+    assert r.info.status_val == 8, r.info.status
+    assert 0 < r.info.iter < 10**8
+    # a problem that ends before the limit is unaffected
+    q = pr.mpc(10, seed=1)
+    eng = BatchEngine(precision=64, time_limit=30.0, **ST)
+    _, (r1,) = run_all(eng, [q])
+    eng.close()
+    eng = BatchEngine(precision=64, **ST)
+    _, (r2,) = run_all(eng, [q])
+    eng.close()
+    assert r1.info.status_val == r2.info.status_val == 1
+    assert r1.info.iter == r2.info.iter and r1.info.obj_val == r2.info.obj_val
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_adaptive_rho_time_and_kkt_modes(mode):
+    """The time-based and KKT-error-based rho adaptation (src/osqp_api.c:
+    776-877) run in the batch and reach the oracle's status and objective
+    (the oracle runs the same mode; the time mode's interval depends on
+    timing, so only the answer is compared)."""
+    probs = mpc_stream(8, seed=3) + [pr.portfolio(500, seed=9)]
+    eng = BatchEngine(precision=64, adaptive_rho=mode, **ST)
+    _, res = run_all(eng, probs)
+    eng.close()
+    for p, r in zip(probs, res):
+        o = oracle(p, adaptive_rho=mode)
+        assert r.info.status_val == o.info.status_val == 1
+        assert rel(r.info.obj_val, o.info.obj_val) <= OBJ_TOL
+
+
+SECTIONS = [("P", "idx"), ("P", "all"), ("A", "idx"), ("A", "all"), ("PA", "idx"), ("PA", "all")]
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("which,mode", SECTIONS)
+def test_update_data_mat_resident(which, mode, precision):
+    """tests/update_matrices/test_update_matrices.cpp:138-384 on a resident
+    problem: new P and / or A values, then a re-solve, reach the held
+    solutions (TESTS_TOL 1e-4), and equal the oracle after the same update."""
+    p = pr.update_matrices_problem()
+    st = dict(rho=0.1, alpha=1.6, max_iter=1000, scaling=1, eps_abs=1e-5, eps_rel=1e-5)
+    eng = BatchEngine(precision=precision, **st)
+    pid = eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=True)
+    while eng.pending():
+        eng.step()
+    r0 = eng.poll(pid)
+    assert r0.info.status_val == 1
+    assert np.max(np.abs(r0.x - p["x"])) < 1e-4
+    kw = {}
+    if "P" in which:
+        kw["Px"] = p["Pu_new"].data
+        if mode == "idx":
+            kw["Px_idx"] = np.arange(p["Pu_new"].nnz)
+    if "A" in which:
+        kw["Ax"] = p["A_new"].data
+        if mode == "idx":
+            kw["Ax_idx"] = np.arange(p["A_new"].nnz)
+    eng.update_mat(pid, **kw)
+    eng.resolve(pid)
+    while eng.pending():
+        eng.step()
+    r = eng.poll(pid)
+    eng.close()
+    key = {"P": "P_new", "A": "A_new", "PA": "PA_new"}[which]
+    assert r.info.status_val == 1
+    assert np.max(np.abs(r.x - p["x_" + key])) < 1e-4
+    assert np.max(np.abs(r.y - p["y"])) < 1e-4
+    assert abs(r.info.obj_val - p["obj_" + key]) < 1e-4
+    o = OracleSolver()
+    assert o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=2, **st) == 0
+    o.solve()
+    assert o.update_data_mat(**kw) == 0
+    o.solve()
+    assert o.info.status_val == 1
+    assert abs(r.info.obj_val - o.info.obj_val) <= 1e-4 * max(1.0, abs(o.info.obj_val))
+
+
+def test_update_data_mat_partial_and_rejected():
+    """A partial update (a few indexed values) equals a fresh problem with
+    those values; bad counts and indices are rejected (1: P, 2: A) and leave
+    the problem as it was."""
+    p = pr.portfolio(300, seed=12)
+    A2 = p["A"].copy()
+    idx = np.arange(0, A2.nnz, 7)
+    A2.data[idx] = A2.data[idx] * 1.5
+    eng = BatchEngine(precision=64, **ST)
+    pid = eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=True)
+    while eng.pending():
+        eng.step()
+    eng.poll(pid)
+    with pytest.raises(ValueError):
+        eng.update_mat(pid, Ax=np.ones(3))  # partial without indices
+    with pytest.raises(ValueError):
+        eng.update_mat(pid, Ax=np.ones(2), Ax_idx=np.array([0, A2.nnz]))  # index out of range
+    with pytest.raises(ValueError):
+        eng.update_mat(pid, Px=np.ones(p["P"].nnz + 1))
+    eng.update_mat(pid, Ax=A2.data[idx], Ax_idx=idx)
+    eng.resolve(pid)
+    while eng.pending():
+        eng.step()
+    r = eng.poll(pid)
+    eng.close()
+    o = oracle(dict(p, A=A2))
+    assert r.info.status_val == o.info.status_val == 1
+    assert rel(r.info.obj_val, o.info.obj_val) <= OBJ_TOL
+
+
+def test_device_pointer_ingestion():
+    """Problem arrays in device memory (torch tensors on the GPU) are taken
+    as osqp_batch_add and osqp_setup take host arrays: same results, bit for
+    bit."""
+    import ctypes as C
+
+    import torch
+
+    from osqp_amd import _types as T
+    from osqp_amd.serve import _bind
+    from osqp_amd.solver import load_library
+    p = pr.mpc(15, seed=8)
+    Ph, Ah = T.CscHolder(p["P"]), T.CscHolder(p["A"])
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    arrays = dict(Pp=dev(Ph.p), Pi=dev(Ph.i), Px=dev(Ph.x), Ap=dev(Ah.p), Ai=dev(Ah.i), Ax=dev(Ah.x),
+                  q=dev(p["q"]), l=dev(np.clip(p["l"], -1e30, 1e30)), u=dev(np.clip(p["u"], -1e30, 1e30)))
+    torch.cuda.synchronize()
+    Pd = T.OSQPCscMatrix(Ph.m, Ph.n, arrays["Pp"].data_ptr(), arrays["Pi"].data_ptr(), arrays["Px"].data_ptr(),
+                         int(p["P"].nnz), -1, 0)
+    Ad = T.OSQPCscMatrix(Ah.m, Ah.n, arrays["Ap"].data_ptr(), arrays["Ai"].data_ptr(), arrays["Ax"].data_ptr(),
+                         int(p["A"].nnz), -1, 0)
+    lib = _bind(load_library())
+    eng = BatchEngine(precision=64, **ST)
+    pid = C.c_int()
+    assert lib.osqp_batch_add(eng._b, C.addressof(Pd), arrays["q"].data_ptr(), C.addressof(Ad),
+                              arrays["l"].data_ptr(), arrays["u"].data_ptr(), None, None, 0, C.byref(pid)) == 0
+    eng._dims[pid.value] = (Ph.n, Ah.m)
+    got = {}
+    while eng.pending():
+        for i in eng.step():
+            got[i] = eng.poll(i)
+    rd = got[pid.value]
+    _, (rh,) = run_all(eng, [p])
+    eng.close()
+    assert rd.info.status_val == rh.info.status_val == 1
+    assert rd.info.obj_val == rh.info.obj_val
+    np.testing.assert_array_equal(rd.x, rh.x)
+    # osqp_setup with the same device arrays
+    s = OSQP(precision=64)
+    s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **ST)
+    rs_host = s.solve()
+    s.close()
+    settings = T.default_settings()
+    T.apply_settings(settings, **ST)
+    solver = C.POINTER(T.OSQPSolver)()
+    assert lib.osqp_setup(C.byref(solver), C.addressof(Pd), arrays["q"].data_ptr(), C.addressof(Ad),
+                          arrays["l"].data_ptr(), arrays["u"].data_ptr(), Ah.m, Ph.n, C.addressof(settings)) == 0
+    lib.osqp_solve(solver)
+    info = solver.contents.info.contents
+    x = np.ctypeslib.as_array(solver.contents.solution.contents.x, shape=(Ph.n,)).copy()
+    assert info.status_val == 1 and info.obj_val == rs_host.info.obj_val
+    np.testing.assert_array_equal(x, rs_host.x)
+    lib.osqp_cleanup(solver)

@@ -172,6 +172,71 @@ def test_warm_start_from_solution():
     assert s.info.iter == 1
 
 
+UPDATE_SECTIONS = [
+    # (name, P update, A update, held solution key): tests/update_matrices/
+    # test_update_matrices.cpp:138-384, with index vectors, with all values
+    # (null index, n = nnz) and with n = 0 meaning all
+    ("P idx", ("idx",), None, "P_new"), ("P all", ("all",), None, "P_new"), ("P nnz0", ("zero",), None, "P_new"),
+    ("A idx", None, ("idx",), "A_new"), ("A all", None, ("all",), "A_new"), ("A nnz0", None, ("zero",), "A_new"),
+    ("PA idx", ("idx",), ("idx",), "PA_new"), ("PA all", ("all",), ("all",), "PA_new"),
+    ("PA nnz0", ("zero",), ("zero",), "PA_new"),
+]
+
+
+def update_args(p, Pmode, Amode):
+    """Keyword arguments of update_data_mat for one section."""
+    kw = {}
+    for key, mode, new in (("P", Pmode, p["Pu_new"]), ("A", Amode, p["A_new"])):
+        if mode is None:
+            continue
+        kw[key + "x"] = new.data
+        if mode[0] == "idx":
+            kw[key + "x_idx"] = np.arange(new.nnz, dtype=np.int32)
+        elif mode[0] == "zero":
+            kw[key + "x_n0"] = True
+    return kw
+
+
+@pytest.mark.parametrize("linsys", [1, 2])
+@pytest.mark.parametrize("section", range(len(UPDATE_SECTIONS)))
+def test_update_data_mat_held_solutions(linsys, section):
+    """oracle_update_data_mat (src/osqp_api.c:1309-1409): after the update the
+    solve reaches the solution the reference's test holds (TESTS_TOL), and the
+    same as a fresh setup on the new matrices."""
+    name, Pm, Am, key = UPDATE_SECTIONS[section]
+    p = pr.update_matrices_problem()
+    st = dict(FIXTURE, max_iter=1000)
+    s, rc = _solve(p, linsys, **st)
+    assert rc == 0
+    s.solve()
+    assert s.info.status_val == 1
+    assert np.max(np.abs(s.x - p["x"])) < TESTS_TOL
+    assert np.max(np.abs(s.y - p["y"])) < TESTS_TOL
+    assert s.update_data_mat(**update_args(p, Pm, Am)) == 0
+    s.solve()
+    assert s.info.status_val == 1, name
+    assert np.max(np.abs(s.x - p["x_" + key])) < TESTS_TOL, name
+    assert np.max(np.abs(s.y - p["y"])) < TESTS_TOL, name
+    assert abs(s.info.obj_val - p["obj_" + key]) < TESTS_TOL, name
+    fresh = dict(p, P=p["Pu_new"] if Pm else p["P"], A=p["A_new"] if Am else p["A"])
+    f, _ = _solve(fresh, linsys, **st)
+    f.solve()
+    assert np.max(np.abs(s.x - f.x)) < TESTS_TOL
+    assert abs(s.info.obj_val - f.info.obj_val) < TESTS_TOL
+
+
+def test_update_data_mat_rejects_bad_sizes():
+    """src/osqp_api.c:1335-1366: more values than nonzeros, or a partial
+    update without an index vector, return 1 (P) or 2 (A)."""
+    p = pr.update_matrices_problem()
+    s, _ = _solve(p)
+    nP, nA = p["P"].nnz, p["A"].nnz
+    assert s.update_data_mat(Px=np.ones(nP + 1)) == 1
+    assert s.update_data_mat(Px=np.ones(nP - 1)) == 1
+    assert s.update_data_mat(Ax=np.ones(nA + 1)) == 2
+    assert s.update_data_mat(Ax=np.ones(nA - 1)) == 2
+
+
 def test_time_limit():
     p = pr.basic_qp()
     s, _ = _solve(p, rho=20, adaptive_rho=0)
