@@ -202,6 +202,34 @@ void       osqp_amd_red_scaled_norm_inf(const OSQPVectorf* S, const OSQPVectorf*
 void       osqp_amd_red_dot_prod(const OSQPVectorf* a, const OSQPVectorf* b, OSQPFloat* dst);
 void       osqp_amd_red_dot_prod_signed(const OSQPVectorf* a, const OSQPVectorf* b, OSQPInt sign, OSQPFloat* dst);
 void       osqp_amd_red_wait(void);
+/* Reductions that allocate nothing.  No counterpart in the reference: this is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * 1..16 reductions, each as the queued osqp_amd_red_* call of the same kind
+ * runs it (same kernel, same bits), in one launch on the caller's `stream`
+ * (a hipStream_t; null is the default stream), and writes reduction k's
+ * value to out[k], out being device memory of `count` doubles.  a and b are
+ * device pointers of n doubles; n = 0 gives 0.  The per-block partials and
+ * tickets live in a fixed 64.1 KiB device array per GPU, held until the
+ * process ends: calls that overlap on one GPU race.  Returns 0, or
+ * OSQP_DATA_VALIDATION_ERROR (count outside 1..16, unknown kind, negative n,
+ * null operand) without launching anything.  Needs no osqp_algebra_init_libs. */
+enum osqp_amd_reduction_kind {
+  OSQP_AMD_RED_NORM_INF = 0,     /* max |a_i|            (OSQPVectorf_norm_inf)          */
+  OSQP_AMD_RED_SCALED_NORM_INF,  /* max |a_i b_i|        (OSQPVectorf_scaled_norm_inf)   */
+  OSQP_AMD_RED_NORM_INF_DIFF,    /* max |a_i - b_i|      (OSQPVectorf_norm_inf_diff)     */
+  OSQP_AMD_RED_NORM_1,           /* sum |a_i|            (OSQPVectorf_norm_1)            */
+  OSQP_AMD_RED_DOT,              /* sum a_i b_i          (OSQPVectorf_dot_prod)          */
+  OSQP_AMD_RED_DOT_POS,          /* sum a_i max(b_i, 0)  (OSQPVectorf_dot_prod_signed 1) */
+  OSQP_AMD_RED_DOT_NEG           /* sum a_i min(b_i, 0)  (OSQPVectorf_dot_prod_signed -1)*/
+};
+typedef struct {
+  OSQPInt kind;  /* enum osqp_amd_reduction_kind */
+  OSQPInt n;
+  const OSQPFloat* a;
+  const OSQPFloat* b;  /* unused by NORM_INF and NORM_1 */
+} OSQPAmdReduction;
+OSQPInt    osqp_amd_reduce_noalloc(const OSQPAmdReduction* items, OSQPInt count, OSQPFloat* out, void* stream);
 /* Last HIP error seen by the backend (0 if none); cleared on read. */
 OSQPInt    osqp_amd_last_error(void);
 /* 1 when p points to device memory (hipMalloc), 0 for host memory, and a

@@ -9,6 +9,8 @@
 #include <math.h>
 #include <time.h>
 
+#include <mutex>
+
 #include "hip_internal.h"
 
 using namespace osqp_hip;
@@ -140,6 +142,9 @@ __device__ inline double red_elem(int op, const RedArgs& r, long long i) {
 // timing, so results are deterministic -- writes the result to host-visible
 // memory (out[0], then the sequence number `seq` in out[1]) and re-arms the
 // ticket for the next reduction on the stream.
+// With to_device, the result goes to out[0] in device memory alone (no
+// sequence number, no system-scope fence): the caller orders on its stream.
+template <bool to_device = false>
 __device__ void finish_reduce_n(bool mx, double t, double* partials, unsigned* ticket, double* out, double seq,
                                 double* lds, int nblocks) {
   __shared__ int last;
@@ -161,8 +166,10 @@ __device__ void finish_reduce_n(bool mx, double t, double* partials, unsigned* t
   if (threadIdx.x == 0) {
     *ticket = 0;
     out[0] = r;
-    __threadfence_system();  // the result reaches the host before its sequence number
-    ((volatile double*)out)[1] = seq;
+    if (!to_device) {
+      __threadfence_system();  // the result reaches the host before its sequence number
+      ((volatile double*)out)[1] = seq;
+    }
   }
 }
 
@@ -211,6 +218,8 @@ struct RedBatch {
   RedTask t[kMultiMax];
 };
 
+// With to_device, reduction k's result goes to out[k] in device memory.
+template <bool to_device>
 __global__ void k_reduce_multi(RedBatch B, double* partials, unsigned* tickets, double* out, double seq0) {
   const int k = blockIdx.y;
   const int op = B.t[k].op, n = B.t[k].n, g = B.t[k].g;
@@ -236,7 +245,8 @@ __global__ void k_reduce_multi(RedBatch B, double* partials, unsigned* tickets, 
     }
   }
   const double t = mx ? block_max(acc, lds) : block_sum(acc, lds);
-  finish_reduce_n(mx, t, partials + (long long)k * 512, tickets + 1 + k, out + 2 * (k + 1), seq0 + k, lds, g);
+  finish_reduce_n<to_device>(mx, t, partials + (long long)k * 512, tickets + 1 + k,
+                             to_device ? out + k : out + 2 * (k + 1), seq0 + k, lds, g);
 }
 
 // The host waits for reduction `seq` by watching its sequence slot (a stream
@@ -287,8 +297,8 @@ void red_drain(Runtime& R) {
   for (int j = 0; j < k; ++j) gmax = Q.b.t[j].g > gmax ? Q.b.t[j].g : gmax;
   const double seq0 = R.red_seq + 1;
   R.red_seq += k;
-  hipLaunchKernelGGL(k_reduce_multi, dim3(gmax, k), dim3(kBlock), 0, R.stream, Q.b, R.d_partials, R.d_ticket,
-                     R.dh_scalars, seq0);
+  hipLaunchKernelGGL(k_reduce_multi<false>, dim3(gmax, k), dim3(kBlock), 0, R.stream, Q.b, R.d_partials,
+                     R.d_ticket, R.dh_scalars, seq0);
   if (LAUNCHCHK()) {
     for (int j = 0; j < k; ++j) *Q.dst[j] = 0.0;
     return;
@@ -693,3 +703,108 @@ void osqp_amd_admm_update(OSQPFloat alpha, OSQPVectorf* x, OSQPVectorf* delta_x,
 }
 
 }  // extern "C"
+
+// ---- reductions that allocate nothing ----------------------------------------
+// osqp_amd_reduce_noalloc runs up to kMultiMax of the queued reductions' kinds
+// in one k_reduce_multi launch, exactly as osqp_amd_red_wait runs them (same
+// grids, same order, same bits), but its per-block partials and tickets live in
+// a file-scope device array rather than in memory that osqp_algebra_init_libs
+// allocates, and its results go to the caller's device memory on the caller's
+// stream.  It never calls hipMalloc / hipFree, so code that may not allocate
+// (a stream under HIP graph capture, a tight serving loop) can call it.
+//
+// scratch: kMultiMax blocks of 512 partials, then the tickets (slot 1 + k is
+// reduction k's, as in k_reduce_multi).  Each GPU holds its own copy for the
+// life of the process; its address comes from hipGetSymbolAddress once per
+// device.  Every call on a device shares it, so calls that overlap on one GPU
+// (two streams, or two host threads) race.
+namespace {
+constexpr int kNoAllocPartials = kMultiMax * 512;
+constexpr int kNoAllocTickets = 32;  // unsigned slots used: 1 + kMultiMax
+static __device__ __align__(256) double scratch[kNoAllocPartials + kNoAllocTickets / 2];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// The tickets must start at zero: every finishing block re-arms its own, but
+// a call that never finished (a launch error) would leave one armed.  A fill
+// kernel, not hipMemsetAsync, so a captured graph replays it correctly.
+__global__ void k_clear_tickets(unsigned* t, int n) {
+  if ((int)threadIdx.x < n) t[threadIdx.x] = 0u;
+}
+
+double* scratch_ptr_for_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    double* p = nullptr;
+    if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" OSQPInt osqp_amd_reduce_noalloc(const OSQPAmdReduction* items, OSQPInt count, OSQPFloat* out,
+                                           void* stream) {
+  if (count < 1 || count > kMultiMax) {
+    fprintf(stderr, "osqp_amd_reduce_noalloc: %d reductions requested; the scratch holds 1 to %d per call\n",
+            (int)count, kMultiMax);
+    return OSQP_DATA_VALIDATION_ERROR;
+  }
+  if (!items || !out) {
+    fprintf(stderr, "osqp_amd_reduce_noalloc: null items or out\n");
+    return OSQP_DATA_VALIDATION_ERROR;
+  }
+  RedBatch B;
+  int gmax = 1;
+  for (int k = 0; k < count; ++k) {
+    const OSQPAmdReduction& it = items[k];
+    int op = -1;
+    switch (it.kind) {
+      case OSQP_AMD_RED_NORM_INF: op = R_NORM_INF; break;
+      case OSQP_AMD_RED_SCALED_NORM_INF: op = R_SCALED_NORM_INF; break;
+      case OSQP_AMD_RED_NORM_INF_DIFF: op = R_NORM_INF_DIFF; break;
+      case OSQP_AMD_RED_NORM_1: op = R_NORM_1; break;
+      case OSQP_AMD_RED_DOT: op = R_DOT; break;
+      case OSQP_AMD_RED_DOT_POS: op = R_DOT_POS; break;
+      case OSQP_AMD_RED_DOT_NEG: op = R_DOT_NEG; break;
+      default: break;
+    }
+    const bool two = op == R_SCALED_NORM_INF || op == R_NORM_INF_DIFF || op == R_DOT || op == R_DOT_POS ||
+                     op == R_DOT_NEG;
+    if (op < 0 || it.n < 0 || (it.n > 0 && (!it.a || (two && !it.b)))) {
+      fprintf(stderr, "osqp_amd_reduce_noalloc: item %d: bad kind %d, length %d or null operand\n", k, (int)it.kind,
+              (int)it.n);
+      return OSQP_DATA_VALIDATION_ERROR;
+    }
+    RedTask& T = B.t[k];
+    T.op = op;
+    T.n = (int)it.n;
+    T.g = red_grid((int)it.n);  // 1 block for n = 0: it writes 0, as the host-side path returns
+    T.pad = 0;
+    T.r = RedArgs{it.a, two ? it.b : nullptr, nullptr, nullptr, 0, 0};
+    gmax = T.g > gmax ? T.g : gmax;
+  }
+  for (int k = count; k < kMultiMax; ++k) B.t[k] = RedTask{};
+  double* ptr = scratch_ptr_for_current_device();
+  if (!ptr) {
+    fprintf(stderr, "osqp_amd_reduce_noalloc: no device address for the scratch array on this device\n");
+    return OSQP_ALGEBRA_LOAD_ERROR;
+  }
+  unsigned* tickets = reinterpret_cast<unsigned*>(ptr + kNoAllocPartials);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_clear_tickets, dim3(1), dim3(64), 0, s, tickets, 1 + kMultiMax);
+  hipLaunchKernelGGL(k_reduce_multi<true>, dim3(gmax, count), dim3(kBlock), 0, s, B, ptr, tickets, out, 0.0);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "osqp_amd_reduce_noalloc: launch failed: %s\n", hipGetErrorString(e));
+    return OSQP_ALGEBRA_LOAD_ERROR;
+  }
+  return 0;
+}
