@@ -498,10 +498,15 @@ struct Solver {
     for (int i = t.tid; i < m; i += t.T) p.E[i] = 1.0;
     t.sync();
     for (int it = 0; it < p.set.scaling; ++it) {
-      // inf-norms of the KKT columns: P columns (= full rows), A columns (= A' rows), A rows
+      // inf-norms of the KKT columns: P columns, A columns (= A' rows), A
+      // rows.  A column norm of P is taken over the stored upper-triangular
+      // column, as the builtin algebra's OSQPMatrix_col_norm_inf does
+      // (algebra/builtin/matrix.c:194-197, csc_math.c:292-308): column r of
+      // triu(P) is row r of the full P restricted to columns <= r.
       for (int r = t.tid; r < n; r += t.T) {
         double a = 0.0, b = 0.0;
-        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) a = dmax(a, fabs(p.P.val[k]));
+        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k)
+          if (p.P.ci[k] <= r) a = dmax(a, fabs(p.P.val[k]));
         for (int k = p.At.rp[r]; k < p.At.rp[r + 1]; ++k) b = dmax(b, fabs(p.At.val[k]));
         Dt[r] = a;
         DtA[r] = b;
@@ -525,10 +530,11 @@ struct Solver {
       }
       for (int i = t.tid; i < m; i += t.T) p.E[i] = p.E[i] * Et[i];
       t.sync();
-      // cost scaling: mean of P's column norms vs ||q||
+      // cost scaling: mean of P's (upper-triangular) column norms vs ||q||
       for (int r = t.tid; r < n; r += t.T) {
         double a = 0.0;
-        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k) a = dmax(a, fabs(p.P.val[k]));
+        for (int k = p.P.rp[r]; k < p.P.rp[r + 1]; ++k)
+          if (p.P.ci[k] <= r) a = dmax(a, fabs(p.P.val[k]));
         Dt[r] = a;
       }
       t.sync();
