@@ -11,7 +11,9 @@ library) on the same data:
     differences (450 to 700 iterations between summation orders) while the
     objective and the KKT certificate hold;
   * FP32 engine (FP32 CG, FP64 residuals) vs FP64 single-problem engine on the
-    MPC stream: identical statuses, objective to 1e-3 relative;
+    MPC stream and the mixed set: identical statuses, objective to 1e-4
+    relative (the comparison with the oracle itself is in
+    test_gpu_serve_oracle.py);
   * each result is bit-identical alone and in a mix of other problems, and for
     any team size (the engine's arithmetic is fixed by the problem's data);
   * resident problems with q updates and warm-started re-solves follow
@@ -71,8 +73,7 @@ def test_batch_matches_single_problem_engine(precision):
         print(f"[{k}] batch{precision}: {rb.info.status} it={rb.info.iter} obj={rb.info.obj_val:.10e} | "
               f"single64: {rs.info.status} it={rs.info.iter} obj={rs.info.obj_val:.10e}", flush=True)
         assert rb.info.status_val == rs.info.status_val
-        tol = 1e-4 if precision == 64 else 1e-3
-        assert abs(rb.info.obj_val - rs.info.obj_val) <= tol * max(1.0, abs(rs.info.obj_val))
+        assert abs(rb.info.obj_val - rs.info.obj_val) <= 1e-4 * max(1.0, abs(rs.info.obj_val))
         if precision == 64 and k != len(probs) - 1:
             assert abs(rb.info.iter - rs.info.iter) <= max(25, 0.15 * rs.info.iter)
         c = pr.kkt_certificate(p, rb.x, rb.y)
@@ -82,7 +83,7 @@ def test_batch_matches_single_problem_engine(precision):
 
 def test_mpc_stream_statuses_vs_fp64_plain():
     """FP32 batched engine on an MPC stream (N ~ U{10..50}) against the plain
-    FP64 single-problem engine: identical statuses, objectives to 1e-3."""
+    FP64 single-problem engine: identical statuses, objectives to 1e-4."""
     rng = np.random.default_rng(7)
     probs = [pr.mpc(int(N), seed=int(s)) for N, s in zip(rng.integers(10, 51, 24), range(24))]
     eng = BatchEngine(precision=32, **ST)
@@ -92,7 +93,7 @@ def test_mpc_stream_statuses_vs_fp64_plain():
         rs = single(p, 64)
         assert rb.info.status_val == rs.info.status_val
         if rs.info.status_val == 1:
-            assert abs(rb.info.obj_val - rs.info.obj_val) <= 1e-3 * max(1.0, abs(rs.info.obj_val))
+            assert abs(rb.info.obj_val - rs.info.obj_val) <= 1e-4 * max(1.0, abs(rs.info.obj_val))
 
 
 def same(a, b):
@@ -294,6 +295,8 @@ def test_settings_rejected():
     with pytest.raises(RuntimeError):
         BatchEngine(precision=32, polishing=1)
     with pytest.raises(RuntimeError):
-        BatchEngine(precision=32, adaptive_rho=2)
+        BatchEngine(precision=32, adaptive_rho=5)
+    with pytest.raises(RuntimeError):
+        BatchEngine(precision=32, rho_is_vec=0)
     with pytest.raises(RuntimeError):
         BatchEngine(precision=16)
