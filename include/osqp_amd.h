@@ -288,6 +288,10 @@ OSQPInt osqp_amd_solver_stats(OSQPSolver* solver, OSQPInt* pcg_iters_total,
 OSQPInt osqp_amd_time_pcg_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_av, double* ms_kv,
                                   double* bytes_av, double* bytes_kv);
 
+/* Applies this solver's linear-system operator to the host vector x (see
+ * osqp_amd_linsys_apply in osqp_amd_algebra.h; the scaled data). */
+OSQPInt osqp_amd_apply_operator(OSQPSolver* solver, OSQPInt which, const OSQPFloat* x, OSQPFloat* y);
+
 /* Times the FP64 panel SpMVs of the PCG's defect correction (see
  * osqp_amd_linsys_time_fp64_kernels in osqp_amd_algebra.h). */
 OSQPInt osqp_amd_time_fp64_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_a, double* ms_k,

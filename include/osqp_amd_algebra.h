@@ -163,6 +163,16 @@ OSQPInt    osqp_amd_linsys_time_kernels(LinSysSolver* s, OSQPInt reps, double* m
  * panelled.  Extension: the reference has one precision per build. */
 OSQPInt    osqp_amd_linsys_time_fp64_kernels(LinSysSolver* s, OSQPInt reps, double* ms_a, double* ms_k,
                                              double* bytes_a, double* bytes_k);
+/* Applies one of the linsys' operators to the host vector x through the
+ * kernels its solves use (column panels, hybrids or row-block CSR), for checks
+ * of each operator form against a reference product (no counterpart in the
+ * reference):
+ *   which 0: y (n) = (P + sigma I + A' diag(rho) A) x in the CG precision;
+ *   which 1: y (n) = the same with the defect correction's FP64-vector operator;
+ *   which 2: y (m) = A x, the defect correction's z = A x product.
+ * P, A and rho are the linsys' own (scaled) data.  Resets the CG state.
+ * Returns 0, or 1 for a bad argument or a HIP error. */
+OSQPInt    osqp_amd_linsys_apply(LinSysSolver* s, OSQPInt which, const OSQPFloat* x, OSQPFloat* y);
 /* Algorithmic bytes (each array element touched once per pass) of one CG
  * iteration and of the per-solve work outside the CG loop (cached A x path:
  * w = rho o (A x - b2), FP64 residual, correction, z = A x) of this linsys. */

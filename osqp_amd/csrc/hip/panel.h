@@ -55,12 +55,18 @@ struct PanelMatrix {
   double* val64;   // nnz + pad, optional
   long long ver;   // source-value version the copies were made from
   int grid;        // workgroups per launch
+  int* pcol;       // np: column block (of W columns) of each panel; null when every block is a panel
 };
 
 // Host builder from a host CSR structure (R x C): allocates and uploads the
 // structure for the FP32 (bits 32) or FP64 (bits 64) kernel; values are
-// filled by panel_refresh_values.  Returns 0 on success.
-int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int* ci);
+// filled by panel_refresh_values.  With `sel` (one flag per block of W
+// columns), only the selected blocks become panels and the entries of the
+// others are left out (the caller handles them).  Returns 0 on success.
+int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int* ci,
+                const unsigned char* sel = nullptr);
+// columns per panel for the FP32 (bits 32) and FP64 (bits 64) kernels
+int panel_width(int bits);
 void panel_free(PanelMatrix* P);
 // Whether panels pay for an R x C operator with nnz nonzeros: one panel, or
 // at least 4 nonzeros per (panel, row) on average (the row pointers and
@@ -73,10 +79,51 @@ void panel_refresh_values(PanelMatrix* P, const double* csr_val);
 void panel_spmv(const PanelMatrix& P, const int* active, const float* x0, int n0, const float* x1, float* partial);
 void panel_spmv(const PanelMatrix& P, const int* active, const double* x0, int n0, const double* x1,
                 double* partial);
+// The same product over FP64-built panels (4096 columns, 6 per lane) with
+// FP32 copies of their values: x, the products and the sums in FP64.  val32
+// comes from panel_alloc_f32vals (nnz plus the zeroed padding) and
+// panel_fill_f32vals (val32[k] = (float)csr_val[src[k]]).
+void panel_spmv_f32vals(const PanelMatrix& P, const float* val32, const int* active, const double* x0, int n0,
+                        const double* x1, double* partial);
+float* panel_alloc_f32vals(const PanelMatrix& P);
+void panel_fill_f32vals(const PanelMatrix& P, const double* csr_val, float* val32);
 // bytes one panel_spmv launch moves at minimum: values + 16-bit indices,
 // np (R + 1) row pointers, x once, one partial per (panel, row), 24-byte task
 // descriptors
 double panel_spmv_bytes(const PanelMatrix& P, int value_bytes);
+
+// ---- hybrid operators: panels for the dense part, CSR for the rest ----------
+// An operator whose nonzeros concentrate in some column blocks and rows (the
+// lasso's A = [[D, -I, 0], [I, 0, -I], [I, 0, I]]: D's 20000 columns hold
+// 94 % of the entries; its K = [P A'] likewise in its first 20000 rows) is
+// split into
+//   * panels over the rows [0, Rp) and the column blocks holding at least one
+//     entry per such row on average (Rp: where the trailing run of rows with
+//     at most 4 entries begins), and
+//   * a remainder CSR of every other entry (original columns), which the
+//     combine kernel adds to the row's panel partials.
+// Used where plain panels do not pay (panel_worth) and the split moves at
+// least half the entries into panels and is estimated 20 % cheaper than the
+// row-block CSR kernel (streams at ~5 TB/s, random gathers at ~1.5 TB/s).
+struct HybridOp {
+  PanelMatrix pan;  // rows [0, Rp), selected column blocks
+  int R, C, Rp;
+  long long rnnz;   // remainder entries
+  int* rrp;         // R + 1 remainder row pointers
+  int* rci;         // rnnz: original column
+  int* rsrc;        // rnnz: CSR slot the value comes from
+  float* rv32;      // remainder values (FP32; also the FP32 copy of an FP64 hybrid)
+  double* rv64;     // remainder values (FP64 hybrids)
+  float* dc32;      // FP32 copy of an FP64 hybrid's panel values (defect correction), optional
+  long long ver;
+};
+// 0: built; 1: does not pay (H zeroed); -1: allocation failure.
+int hybrid_build(HybridOp* H, int R, int C, int bits, const int* rp, const int* ci);
+// An FP64 hybrid also keeps FP32 copies of its values (panel and remainder).
+int hybrid_add_f32(HybridOp* H);
+void hybrid_free(HybridOp* H);
+// values from the source CSR's FP64 value array (device), on the backend stream
+void hybrid_refresh_values(HybridOp* H, const double* csr_val);
 
 // FP64 panels of a general (non-triu) matrix M (transpose = 0: m x n) or of
 // its transpose (transpose = 1: n x m), owned by M: built on the first call

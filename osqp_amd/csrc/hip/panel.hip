@@ -33,6 +33,7 @@ __device__ __forceinline__ T ld_nt(const T* p) {
 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 // Entries per lane per load group: a lane loads V consecutive (index, value)
@@ -41,15 +42,19 @@ typedef double f64x2 __attribute__((ext_vector_type(2)));
 template <typename TV>
 constexpr int panel_vec() { return sizeof(TV) == 4 ? 4 : 2; }
 
-// One task per wavefront at a time; see panel.h for the scheme.
-template <typename TV, typename TX, int W, int NW, int PER>
+// One task per wavefront at a time; see panel.h for the scheme.  V is the
+// vector width the panels were built for (their task cap); FP32 values on
+// FP64-built panels (V = 2) are loaded as 8-byte pairs and multiplied and
+// summed in TX.
+template <typename TV, typename TX, int W, int NW, int PER, int V, bool SEL>
 __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int ntask, const int4* __restrict__ task,
                                                         const int2* __restrict__ tpanel, const int* __restrict__ rp,
                                                         const uint16_t* __restrict__ ci, const TV* __restrict__ val,
-                                                        int R, int C, const TX* __restrict__ x0, int n0,
+                                                        int R, int C, const int* __restrict__ pcol,
+                                                        const TX* __restrict__ x0, int n0,
                                                         const TX* __restrict__ x1, TX* __restrict__ partial) {
   if (active && !*active) return;
-  constexpr int V = panel_vec<TV>();
+  static_assert(V == 4 ? (sizeof(TV) == 4 && sizeof(TX) == 4) : V == 2, "vector widths: 4 (FP32) or 2");
   constexpr int NG = PER / V;          // load groups of 64 V entries
   constexpr int CAP = 64 * PER - (V - 1);  // entries of a task (the aligned window holds 64 PER)
   static_assert(PER % V == 0, "PER must be a multiple of the vector width");
@@ -70,7 +75,7 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
     // goes as 16-byte vectors (thread i takes vectors i, i + 1024, ...);
     // otherwise element by element.  The choice is the same for the whole
     // workgroup.
-    const int c0 = p * W;
+    const int c0 = (SEL ? pcol[p] : p) * W;  // SEL: panels of selected column blocks only
     const int w = (C - c0) < W ? (C - c0) : W;
     constexpr int XV = 16 / (int)sizeof(TX);  // elements per vector
     constexpr int XG = XL / XV;               // vectors per thread
@@ -158,9 +163,15 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
             vv[g][1] = v4.y;
             vv[g][2] = v4.z;
             vv[g][3] = v4.w;
-          } else {
+          } else if constexpr (sizeof(TV) == 8) {
             const unsigned w1 = ld_nt((const unsigned*)(ci + at));
             const f64x2 v2 = ld_nt((const f64x2*)(val + at));
+            cw[g][0] = w1;
+            vv[g][0] = v2.x;
+            vv[g][1] = v2.y;
+          } else {
+            const unsigned w1 = ld_nt((const unsigned*)(ci + at));
+            const f32x2 v2 = ld_nt((const f32x2*)(val + at));
             cw[g][0] = w1;
             vv[g][0] = v2.x;
             vv[g][1] = v2.y;
@@ -227,12 +238,17 @@ __global__ __launch_bounds__(NW * 64) void k_panel_spmv(const int* active, int n
   }
 }
 
-template <typename TV, typename TX, int W, int PER>
+template <typename TV, typename TX, int W, int PER, int V = panel_vec<TV>()>
 void launch(const PanelMatrix& P, const int* active, const TV* val, const TX* x0, int n0, const TX* x1,
             TX* partial) {
-  hipLaunchKernelGGL((k_panel_spmv<TV, TX, W, kPanelWaves, PER>), dim3(P.grid), dim3(kPanelWaves * 64), 0,
-                     rt().stream, active, P.ntask, (const int4*)P.task, (const int2*)P.tpanel, (const int*)P.rp,
-                     (const uint16_t*)P.ci, val, P.R, P.C, x0, n0, x1, partial);
+  if (P.pcol)
+    hipLaunchKernelGGL((k_panel_spmv<TV, TX, W, kPanelWaves, PER, V, true>), dim3(P.grid), dim3(kPanelWaves * 64),
+                       0, rt().stream, active, P.ntask, (const int4*)P.task, (const int2*)P.tpanel, (const int*)P.rp,
+                       (const uint16_t*)P.ci, val, P.R, P.C, (const int*)P.pcol, x0, n0, x1, partial);
+  else
+    hipLaunchKernelGGL((k_panel_spmv<TV, TX, W, kPanelWaves, PER, V, false>), dim3(P.grid), dim3(kPanelWaves * 64),
+                       0, rt().stream, active, P.ntask, (const int4*)P.task, (const int2*)P.tpanel, (const int*)P.rp,
+                       (const uint16_t*)P.ci, val, P.R, P.C, (const int*)nullptr, x0, n0, x1, partial);
 }
 
 // FP32 operators: 8192-column panels, 12 nonzeros per lane (80 KB of LDS per
@@ -244,25 +260,41 @@ constexpr int kW32 = 8192, kPer32 = 12, kW64 = 4096, kPer64 = 6;
 
 }  // namespace
 
-int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int* ci) {
+int panel_width(int bits) { return bits == 32 ? kW32 : kW64; }
+
+int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int* ci, const unsigned char* sel) {
   memset(P, 0, sizeof(*P));
   if (bits != 32 && bits != 64) return 1;
   const int W = bits == 32 ? kW32 : kW64;
   const int per = bits == 32 ? kPer32 : kPer64;
   const int want32 = bits == 32, want64 = bits == 64;
-  const long long nnz = rp[R];
-  const int np = C > 0 ? (C + W - 1) / W : 1;
+  const int nblk = C > 0 ? (C + W - 1) / W : 1;
+  // column block -> panel (-1: not selected; its entries are skipped)
+  std::vector<int> pan_of(nblk), pcol;
+  int np = 0;
+  for (int b = 0; b < nblk; ++b) {
+    pan_of[b] = (!sel || sel[b]) ? np++ : -1;
+    if (pan_of[b] >= 0) pcol.push_back(b);
+  }
+  if (np == 0) return 1;
   const long long pad = 64LL * per;
   P->R = R;
   P->C = C;
   P->W = W;
   P->per = per;
   P->np = np;
-  P->nnz = nnz;
   // per (panel, row) counts -> absolute row pointers, panel-major
   std::vector<int> rpa((size_t)np * (R + 1), 0);
+  long long nnz = 0;
   for (int r = 0; r < R; ++r)
-    for (int k = rp[r]; k < rp[r + 1]; ++k) rpa[(size_t)(ci[k] / W) * (R + 1) + r + 1]++;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int pp = pan_of[ci[k] / W];
+      if (pp >= 0) {
+        rpa[(size_t)pp * (R + 1) + r + 1]++;
+        nnz++;
+      }
+    }
+  P->nnz = nnz;
   long long base = 0;
   for (int p = 0; p < np; ++p) {
     int* q = rpa.data() + (size_t)p * (R + 1);
@@ -275,9 +307,10 @@ int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int
   std::vector<int> next(rpa);
   for (int r = 0; r < R; ++r)
     for (int k = rp[r]; k < rp[r + 1]; ++k) {
-      const int p = ci[k] / W;
-      const int pos = next[(size_t)p * (R + 1) + r]++;
-      lci[(size_t)pos] = (uint16_t)(ci[k] - p * W);
+      const int b = ci[k] / W, pp = pan_of[b];
+      if (pp < 0) continue;
+      const int pos = next[(size_t)pp * (R + 1) + r]++;
+      lci[(size_t)pos] = (uint16_t)(ci[k] - b * W);
       src[(size_t)pos] = k;
     }
   // tasks: per panel, greedy row ranges with <= cap nonzeros and <= 63 rows;
@@ -314,6 +347,10 @@ int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int
   P->val32 = want32 ? (float*)dev_alloc(sizeof(float) * (size_t)(nnz + pad)) : nullptr;
   if (!P->rp || !P->ci || !P->src || !P->task || !P->tpanel || (want64 && !P->val64) || (want32 && !P->val32))
     return 1;
+  if (np < nblk) {
+    if (!(P->pcol = (int*)dev_alloc(sizeof(int) * (size_t)np))) return 1;
+    upload(P->pcol, pcol.data(), sizeof(int) * (size_t)np);
+  }
   upload(P->rp, rpa.data(), sizeof(int) * rpa.size());
   upload(P->ci, lci.data(), sizeof(uint16_t) * lci.size());
   upload(P->src, src.data(), sizeof(int) * src.size());
@@ -343,6 +380,7 @@ void panel_free(PanelMatrix* P) {
   dev_free(P->tpanel);
   dev_free(P->val64);
   dev_free(P->val32);
+  dev_free(P->pcol);
   memset(P, 0, sizeof(*P));
 }
 
@@ -375,6 +413,129 @@ void panel_spmv(const PanelMatrix& P, const int* active, const double* x0, int n
   LAUNCHCHK();
 }
 
+void panel_spmv_f32vals(const PanelMatrix& P, const float* val32, const int* active, const double* x0, int n0,
+                        const double* x1, double* partial) {
+  if (P.ntask <= 0) return;
+  if (P.W != kW64 || P.per != kPer64 || !val32) {
+    check(hipErrorInvalidValue, "panel_spmv_f32vals: not an FP64-built panel matrix", __FILE__, __LINE__);
+    return;
+  }
+  launch<float, double, kW64, kPer64, 2>(P, active, val32, x0, n0, x1, partial);
+  LAUNCHCHK();
+}
+
+float* panel_alloc_f32vals(const PanelMatrix& P) {
+  const long long pad = 64LL * P.per;
+  float* v = (float*)dev_alloc(sizeof(float) * (size_t)(P.nnz + pad));
+  // the padding past nnz is read (and multiplied by zero): keep it finite
+  if (v) HIPCHK(hipMemsetAsync(v + P.nnz, 0, sizeof(float) * (size_t)pad, rt().stream));
+  return v;
+}
+
+void panel_fill_f32vals(const PanelMatrix& P, const double* csr_val, float* val32) {
+  if (P.nnz <= 0) return;
+  const int g = grid_for(P.nnz, kBlock, 16384);
+  hipLaunchKernelGGL(k_panel_gather, dim3(g), dim3(kBlock), 0, rt().stream, P.nnz, (const int*)P.src, csr_val,
+                     (double*)nullptr, val32);
+  LAUNCHCHK();
+}
+
+int hybrid_build(HybridOp* H, int R, int C, int bits, const int* rp, const int* ci) {
+  memset(H, 0, sizeof(*H));
+  if (R <= 0 || C <= 0 || rp[R] <= 0) return 1;
+  const long long total = rp[R];
+  // Rp: start of the trailing run of rows with at most 4 entries
+  int Rp = R;
+  while (Rp > 0 && rp[Rp] - rp[Rp - 1] <= 4) --Rp;
+  if (Rp == 0) return 1;
+  const int W = panel_width(bits);
+  const int nblk = (C + W - 1) / W;
+  std::vector<long long> cnt(nblk, 0);
+  for (long long k = 0; k < rp[Rp]; ++k) cnt[ci[k] / W]++;
+  std::vector<unsigned char> sel(nblk, 0);
+  long long pan_nnz = 0;
+  int np = 0;
+  for (int b = 0; b < nblk; ++b)
+    if (cnt[b] >= Rp) {
+      sel[b] = 1;
+      pan_nnz += cnt[b];
+      np++;
+    }
+  if (np == 0 || 2 * pan_nnz < total) return 1;
+  const double vb = bits / 8.0;
+  const double t_rb = total * (vb + 4.0) / 1.5;
+  const double t_h = (pan_nnz * (vb + 2.0) + np * (Rp + 1.0) * 4.0 + 2.0 * np * (double)Rp * vb) / 5.0 +
+                     (total - pan_nnz) * (vb + 8.0) / 1.5 + R * 8.0 / 5.0;
+  if (t_h > 0.8 * t_rb) return 1;
+  if (panel_build(&H->pan, Rp, C, bits, rp, ci, sel.data())) {
+    panel_free(&H->pan);
+    return -1;
+  }
+  // remainder CSR: entries of rows >= Rp and of unselected blocks, in order
+  std::vector<int> rrp((size_t)R + 1, 0), rci, rsrc;
+  rci.reserve((size_t)(total - pan_nnz));
+  rsrc.reserve((size_t)(total - pan_nnz));
+  for (int r = 0; r < R; ++r) {
+    for (int k = rp[r]; k < rp[r + 1]; ++k)
+      if (r >= Rp || !sel[ci[k] / W]) {
+        rci.push_back(ci[k]);
+        rsrc.push_back(k);
+      }
+    rrp[(size_t)r + 1] = (int)rci.size();
+  }
+  H->R = R;
+  H->C = C;
+  H->Rp = Rp;
+  H->rnnz = (long long)rci.size();
+  const size_t rn = (size_t)(H->rnnz > 0 ? H->rnnz : 1);
+  H->rrp = (int*)dev_alloc(sizeof(int) * rrp.size());
+  H->rci = (int*)dev_alloc(sizeof(int) * rn);
+  H->rsrc = (int*)dev_alloc(sizeof(int) * rn);
+  if (bits == 32) H->rv32 = (float*)dev_alloc(sizeof(float) * rn);
+  else H->rv64 = (double*)dev_alloc(sizeof(double) * rn);
+  if (!H->rrp || !H->rci || !H->rsrc || (bits == 32 ? !H->rv32 : !H->rv64)) {
+    hybrid_free(H);
+    return -1;
+  }
+  upload(H->rrp, rrp.data(), sizeof(int) * rrp.size());
+  if (H->rnnz) {
+    upload(H->rci, rci.data(), sizeof(int) * rci.size());
+    upload(H->rsrc, rsrc.data(), sizeof(int) * rsrc.size());
+  }
+  H->ver = -1;
+  return 0;
+}
+
+int hybrid_add_f32(HybridOp* H) {
+  if (!H->pan.val64) return 1;
+  if (!H->dc32 && !(H->dc32 = panel_alloc_f32vals(H->pan))) return 1;
+  if (!H->rv32 && !(H->rv32 = (float*)dev_alloc(sizeof(float) * (size_t)(H->rnnz > 0 ? H->rnnz : 1)))) return 1;
+  H->ver = -1;
+  return 0;
+}
+
+void hybrid_free(HybridOp* H) {
+  panel_free(&H->pan);
+  dev_free(H->rrp);
+  dev_free(H->rci);
+  dev_free(H->rsrc);
+  dev_free(H->rv32);
+  dev_free(H->rv64);
+  dev_free(H->dc32);
+  memset(H, 0, sizeof(*H));
+}
+
+void hybrid_refresh_values(HybridOp* H, const double* csr_val) {
+  panel_refresh_values(&H->pan, csr_val);
+  if (H->dc32) panel_fill_f32vals(H->pan, csr_val, H->dc32);
+  if (H->rnnz > 0) {
+    const int g = grid_for(H->rnnz, kBlock, 16384);
+    hipLaunchKernelGGL(k_panel_gather, dim3(g), dim3(kBlock), 0, rt().stream, H->rnnz, (const int*)H->rsrc, csr_val,
+                       H->rv64, H->rv32);
+    LAUNCHCHK();
+  }
+}
+
 int panel_worth(long long nnz, int R, int C, int bits) {
   const int W = bits == 32 ? kW32 : kW64;
   const long long np = C > 0 ? (C + W - 1) / W : 1;
@@ -383,7 +544,9 @@ int panel_worth(long long nnz, int R, int C, int bits) {
 }
 
 double panel_spmv_bytes(const PanelMatrix& P, int value_bytes) {
-  return (double)P.nnz * (value_bytes + 2.0) + (double)P.np * (P.R + 1.0) * 4.0 + (double)P.C * value_bytes +
+  // x: the columns of the panels (every column when all blocks are panels)
+  const double xcols = P.pcol ? (double)P.np * P.W : (double)P.C;
+  return (double)P.nnz * (value_bytes + 2.0) + (double)P.np * (P.R + 1.0) * 4.0 + xcols * value_bytes +
          (double)P.np * P.R * value_bytes + (double)P.ntask * 24.0;
 }
 

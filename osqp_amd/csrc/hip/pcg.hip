@@ -45,6 +45,8 @@
 
 #include <vector>
 
+#include <type_traits>
+
 #include "hip_internal.h"
 #include "spmv_rb.h"
 #include "panel.h"
@@ -325,6 +327,87 @@ __global__ void k_comb_z(const int* when, int m, int n, int np, const double* pa
   if (when && !*when) return;
   COMB_ROWS(i, j, m, G) {
     const double s = panel_sum<double, G>(partial, np, m, i, j);
+    if (j == 0) {
+      if (ax) ax[i] = s;
+      b[n + i] = polishing ? (s - b[n + i]) * rho_pol : s;
+    }
+  }
+}
+
+// ---- hybrid operators: panel partials of rows < Rp plus the remainder CSR ------
+// A group of G lanes serves a row: lane j adds its share of the row's panel
+// partials (as panel_sum) and the remainder entries j, j + G, ... in order;
+// a fixed butterfly adds the G sums.  Deterministic.
+template <typename TR>
+struct Rem {
+  int Rp;          // rows with panel partials (stride of the partials)
+  const int* rp;   // remainder CSR (R + 1 row pointers)
+  const int* ci;   // original columns
+  const TR* val;
+};
+
+template <typename TX, int G, typename TR, typename Gather>
+__device__ __forceinline__ TX hyb_sum(const TX* __restrict__ partial, int np, const Rem<TR>& rm, int row, int j,
+                                      Gather gather) {
+  TX s = 0;
+  if (row < rm.Rp) {
+    const int p0 = (np * j) / G, p1 = (np * (j + 1)) / G;
+    for (int p = p0; p < p1; ++p) s += partial[(long long)p * rm.Rp + row];
+  }
+  const int k1 = rm.rp[row + 1];
+  for (int k = rm.rp[row] + j; k < k1; k += G) s += (TX)rm.val[k] * gather(rm.ci[k]);
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) s += __shfl_xor(s, off, G);
+  return s;
+}
+
+// w = rho o (A v)
+template <typename TX, int G, typename TR>
+__global__ void k_hyb_av(const PcgState* st, int m, int np, const TX* partial, Rem<TR> rm, const TX* v,
+                         const double* rho, double rho_sc, TX* w) {
+  if (st && !st->active) return;
+  COMB_ROWS(i, j, m, G) {
+    const TX s = hyb_sum<TX, G>(partial, np, rm, i, j, [&](int c) { return v[c]; });
+    if (j == 0) w[i] = (TX)rho_at(rho, rho_sc, i) * s;
+  }
+}
+
+// Kv = sigma v + [P A'] [v; w] (- sub); partials of v'Kv or of max|Kv - sub|
+template <typename TX, int G, typename TR>
+__global__ void k_hyb_kv(const PcgState* st, int n, int np, const TX* partial, Rem<TR> rm, double sigma, const TX* v,
+                         const TX* w, const double* sub, TX* Kv, double* partials) {
+  if (!sub && !st->active) return;  // the residual (sub given) runs outside the CG loop
+  __shared__ double lds[kBlock / kWave];
+  const TX sg = (TX)sigma;
+  Acc<TX> dot;
+  double mx = 0.0;
+  COMB_ROWS(i, j, n, G) {
+    const TX s = hyb_sum<TX, G>(partial, np, rm, i, j, [&](int c) { return c < n ? v[c] : w[c - n]; });
+    if (j == 0) {
+      const TX vi = v[i];
+      TX kv = sg * vi + s;
+      if (sub) {
+        kv -= (TX)sub[i];
+        const double a = fabs((double)kv);
+        mx = a > mx ? a : mx;
+      } else {
+        dot.add(vi * kv);
+      }
+      Kv[i] = kv;
+    }
+  }
+  const double t = sub ? block_max(mx, lds) : block_sum(dot.value(), lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// z = A x (FP64 sums): b[n:n+m] (polishing: rho_pol (A x - b2)) and the A x
+// cache; skipped while *when == 0 (when given)
+template <int G, typename TR>
+__global__ void k_hyb_z(const int* when, int m, int n, int np, const double* partial, Rem<TR> rm, const double* x,
+                        double* b, double* ax, int polishing, double rho_pol) {
+  if (when && !*when) return;
+  COMB_ROWS(i, j, m, G) {
+    const double s = hyb_sum<double, G>(partial, np, rm, i, j, [&](int c) { return x[c]; });
     if (j == 0) {
       if (ax) ax[i] = s;
       b[n + i] = polishing ? (s - b[n + i]) * rho_pol : s;
@@ -642,6 +725,8 @@ __global__ void k_recip_to(int n, const double* d, V* dinv, int none) {
   GRID_STRIDE(i, n) dinv[i] = none ? (V)1 : (V)(1.0 / d[i]);
 }
 
+__global__ void k_widen(int n, const float* a, double* b) { GRID_STRIDE(i, n) b[i] = (double)a[i]; }
+
 template <typename V>
 __global__ void k_cast(long long n, const double* a, V* b) {
   GRID_STRIDE(i, n) b[i] = (V)a[i];
@@ -705,7 +790,31 @@ struct HipPcg {
   PanelMatrix pA32, pK32, pK64;
   const PanelMatrix* pA64;
   void *partA, *partK;  // per-panel partial sums (sized for the largest use)
+  // FP32 mode: FP32 copies of the FP64 panels' values.  The defect
+  // correction's products (r = K x64 - rhs, z = A x64) read them with x, the
+  // products and the sums in FP64: 6 instead of 10 bytes per nonzero.  The
+  // correction then solves K~ x = rhs for K~ = P~ + sigma I + A~' R A~ built
+  // from the FP32-rounded P and A (relative change <= 2^-24 per entry, below
+  // the CG's 1e-7 tolerance floor), and z~ = A~ x, as the CG does; the
+  // termination checks still use the FP64 matrices.
+  float *dcA32, *dcK32;
+  long long dcA_ver, dcK_ver;
+  // hybrid operators (panels for the dense part, remainder CSR) where plain
+  // panels do not pay: FP32 ones for the CG, FP64 ones for the FP64 CG or the
+  // defect correction (with FP32 copies of their values in FP32 mode)
+  HybridOp hA32, hK32, hA64, hK64;
 };
+
+template <typename V>
+const HybridOp* hyb_A(const HipPcg* s) {
+  const HybridOp* H = sizeof(V) == 4 ? &s->hA32 : &s->hA64;
+  return H->pan.ntask > 0 ? H : nullptr;
+}
+template <typename V>
+const HybridOp* hyb_K(const HipPcg* s) {
+  const HybridOp* H = sizeof(V) == 4 ? &s->hK32 : &s->hK64;
+  return H->pan.ntask > 0 ? H : nullptr;
+}
 
 // the panel copy of A / K for value type V, or null when CSR kernels are used
 template <typename V>
@@ -762,6 +871,54 @@ void refresh_panels(HipPcg* s) {
       panel_refresh_values(P, s->kval64);
       P->ver = kv;
     }
+  if (s->dcA32 && s->pA64 && s->dcA_ver != s->A->version) {
+    panel_fill_f32vals(*s->pA64, s->A->val, s->dcA32);
+    s->dcA_ver = s->A->version;
+  }
+  if (s->dcK32 && s->dcK_ver != kv) {
+    panel_fill_f32vals(s->pK64, s->kval64, s->dcK32);
+    s->dcK_ver = kv;
+  }
+  HybridOp* HA[2] = {&s->hA32, &s->hA64};
+  for (HybridOp* H : HA)
+    if (H->pan.ntask > 0 && H->ver != s->A->version) {
+      hybrid_refresh_values(H, s->A->val);
+      H->ver = s->A->version;
+    }
+  HybridOp* HK[2] = {&s->hK32, &s->hK64};
+  for (HybridOp* H : HK)
+    if (H->pan.ntask > 0 && H->ver != kv) {
+      hybrid_refresh_values(H, s->kval64);
+      H->ver = kv;
+    }
+}
+
+// The FP64-vector products over FP64-built panels: FP32 values in FP32 mode
+// (the defect correction), FP64 values otherwise.
+void dc_panel_spmv(HipPcg* s, const PanelMatrix& P, const int* active, const double* x0, int n0, const double* x1,
+                   double* partial) {
+  const float* v32 = &P == s->pA64 ? s->dcA32 : (&P == &s->pK64 ? s->dcK32 : nullptr);
+  if (v32) panel_spmv_f32vals(P, v32, active, x0, n0, x1, partial);
+  else panel_spmv(P, active, x0, n0, x1, partial);
+}
+// algorithmic bytes of a hybrid operator application without its epilogue:
+// the panel SpMV (values of pv bytes, vectors and partials of xb bytes), the
+// partials read back, and the remainder CSR (values, 32-bit columns, row
+// pointers, one gathered element per entry)
+double hyb_bytes(const HybridOp& H, double pv, double xb) {
+  const PanelMatrix& P = H.pan;
+  const double rv = H.rv32 && (pv == 4.0 || !H.rv64) ? 4.0 : 8.0;
+  return panel_spmv_bytes(P, (int)xb) - (xb - pv) * (double)P.nnz + (double)P.np * H.Rp * xb +
+         (double)H.rnnz * (rv + 4.0 + xb) + (H.R + 1.0) * 4.0;
+}
+// value bytes of a hybrid's panel SpMV for vectors of xb bytes
+double hyb_pv(const HybridOp& H, double xb) { return xb == 8.0 && H.dc32 ? 4.0 : xb; }
+
+// algorithmic bytes of one dc_panel_spmv launch: values of 4 or 8 bytes, the
+// rest as panel_spmv_bytes with FP64 x and partials
+double dc_panel_bytes(const HipPcg* s, const PanelMatrix& P) {
+  const bool f32 = (&P == s->pA64 && s->dcA32) || (&P == &s->pK64 && s->dcK32);
+  return panel_spmv_bytes(P, 8) - (f32 ? 4.0 * (double)P.nnz : 0.0);
 }
 
 int build_panels(HipPcg* s) {
@@ -801,8 +958,43 @@ int build_panels(HipPcg* s) {
     const size_t b = (size_t)s->pA64->np * (size_t)m * sizeof(double);
     bytesA = b > bytesA ? b : bytesA;
   }
+  // hybrids where plain panels do not pay
+  for (int bits : prec) {
+    if (bits == 32 && s->bits != 32) continue;
+    const bool haveK = bits == 32 ? s->pK32.ntask > 0 : s->pK64.ntask > 0;
+    const bool haveA = bits == 32 ? s->pA32.ntask > 0 : (s->pA64 && s->pA64->ntask > 0);
+    if (!haveK && n > 0) {
+      std::vector<int> hrp((size_t)n + 1), hci((size_t)(knnz > 0 ? knnz : 1));
+      download(hrp.data(), s->krp, sizeof(int) * hrp.size());
+      if (knnz) download(hci.data(), s->kci, sizeof(int) * (size_t)knnz);
+      HybridOp* H = bits == 32 ? &s->hK32 : &s->hK64;
+      const int rc = hybrid_build(H, n, n + m, bits, hrp.data(), hci.data());
+      if (rc < 0 || (rc == 0 && bits == 64 && s->bits == 32 && hybrid_add_f32(H))) return 1;
+      if (rc == 0) {
+        const size_t b = (size_t)H->pan.np * (size_t)H->Rp * (bits / 8);
+        bytesK = b > bytesK ? b : bytesK;
+      }
+    }
+    if (!haveA && m > 0 && nnzA > 0) {
+      std::vector<int> arp((size_t)m + 1), aci((size_t)nnzA);
+      download(arp.data(), s->A->rp, sizeof(int) * arp.size());
+      download(aci.data(), s->A->ci, sizeof(int) * (size_t)nnzA);
+      HybridOp* H = bits == 32 ? &s->hA32 : &s->hA64;
+      const int rc = hybrid_build(H, m, n, bits, arp.data(), aci.data());
+      if (rc < 0 || (rc == 0 && bits == 64 && s->bits == 32 && hybrid_add_f32(H))) return 1;
+      if (rc == 0) {
+        const size_t b = (size_t)H->pan.np * (size_t)H->Rp * (bits / 8);
+        bytesA = b > bytesA ? b : bytesA;
+      }
+    }
+  }
   if (bytesA && !(s->partA = dev_alloc(bytesA))) return 1;
   if (bytesK && !(s->partK = dev_alloc(bytesK))) return 1;
+  s->dcA_ver = s->dcK_ver = -1;
+  if (s->bits == 32) {
+    if (s->pA64 && s->pA64->ntask > 0 && !(s->dcA32 = panel_alloc_f32vals(*s->pA64))) return 1;
+    if (s->pK64.ntask > 0 && !(s->dcK32 = panel_alloc_f32vals(s->pK64))) return 1;
+  }
   refresh_panels(s);
   return 0;
 }
@@ -909,14 +1101,44 @@ const V* csr_vals_K(HipPcg* s) {
 }
 const int* active_flag(HipPcg* s) { return (const int*)((const char*)s->st + offsetof(PcgState, active)); }
 
+// ---- hybrid dispatch: value types and lane-group sizes ------------------------
+// The panel SpMV of a hybrid: FP32 values for the FP32 CG; for FP64 vectors
+// the FP32 copies where the hybrid has them (FP32 mode: defect correction),
+// else the FP64 values.
+template <typename V>
+void hyb_panel(const HybridOp& H, const int* active, const V* x0, int n0, const V* x1, V* partial) {
+  if constexpr (sizeof(V) == 4) panel_spmv(H.pan, active, x0, n0, x1, partial);
+  else if (H.dc32) panel_spmv_f32vals(H.pan, H.dc32, active, x0, n0, x1, partial);
+  else panel_spmv(H.pan, active, x0, n0, x1, partial);
+}
+// f(Rem<TR>) with the remainder values that go with hyb_panel's
+template <typename V, typename F>
+void with_rem(const HybridOp& H, F f) {
+  if constexpr (sizeof(V) == 4) f(Rem<float>{H.Rp, H.rrp, H.rci, H.rv32});
+  else if (H.dc32) f(Rem<float>{H.Rp, H.rrp, H.rci, H.rv32});
+  else f(Rem<double>{H.Rp, H.rrp, H.rci, H.rv64});
+}
+template <typename F>
+void with_G(int G, F f) {
+  if (G == 8) f(std::integral_constant<int, 8>());
+  else if (G == 4) f(std::integral_constant<int, 4>());
+  else if (G == 2) f(std::integral_constant<int, 2>());
+  else f(std::integral_constant<int, 1>());
+}
+template <typename R>
+using rem_t = std::remove_const_t<std::remove_pointer_t<decltype(R::val)>>;
+
 // w = rho o (A v)  (skipped while the CG state is inactive), in two launches
-// where A is panelled: the panel SpMV (av_spmv) and the combine (av_finish);
-// the row-block CSR kernel does both in av_spmv.
+// where A is panelled (or hybrid): the panel SpMV (av_spmv) and the combine
+// (av_finish); the row-block CSR kernel does both in av_spmv.
 template <typename V>
 void av_spmv(HipPcg* s, const V* v, V* w) {
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
   if (const PanelMatrix* P = panels_A<V>(s)) {
-    panel_spmv(*P, active_flag(s), v, s->n, (const V*)nullptr, (V*)s->partA);
+    if constexpr (sizeof(V) == 8) dc_panel_spmv(s, *P, active_flag(s), v, s->n, nullptr, (double*)s->partA);
+    else panel_spmv(*P, active_flag(s), v, s->n, (const V*)nullptr, (V*)s->partA);
+  } else if (const HybridOp* H = hyb_A<V>(s)) {
+    hyb_panel<V>(*H, active_flag(s), v, s->n, (const V*)nullptr, (V*)s->partA);
   } else {
     const int g = s->A->rb_n < kRowGridCap ? s->A->rb_n : kRowGridCap;
     hipLaunchKernelGGL((k_rb_Av_rho<V, V>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st,
@@ -925,9 +1147,20 @@ void av_spmv(HipPcg* s, const V* v, V* w) {
   }
 }
 template <typename V>
-void av_finish(HipPcg* s, V* w) {
+void av_finish(HipPcg* s, const V* v, V* w) {
   const int m = s->m;
   const double* rho = s->rho_vec ? s->rho_vec->d : nullptr;
+  if (const HybridOp* H = panels_A<V>(s) ? nullptr : hyb_A<V>(s)) {
+    const int G = comb_group(H->pan.np, m);
+    const dim3 g(grid_for((long long)m * G, kBlock, 8192));
+    with_rem<V>(*H, [&](auto rm) {
+      with_G(G, [&](auto Gc) {
+        hipLaunchKernelGGL((k_hyb_av<V, decltype(Gc)::value, rem_t<decltype(rm)>>), g, dim3(kBlock), 0, S(),
+                           (const PcgState*)s->st, m, H->pan.np, (const V*)s->partA, rm, v, rho, s->rho_sc, w);
+      });
+    });
+    return;
+  }
   const PanelMatrix* P = panels_A<V>(s);
   if (!P) return;
   const int G = comb_group(P->np, m);
@@ -948,7 +1181,7 @@ void av_finish(HipPcg* s, V* w) {
 template <typename V>
 void op_av_rho(HipPcg* s, const V* v, V* w) {
   av_spmv<V>(s, v, w);
-  av_finish<V>(s, w);
+  av_finish<V>(s, v, w);
 }
 
 // Kv = sigma v + [P A'] [v; w] (- sub) and block partials (v'Kv, or
@@ -957,7 +1190,8 @@ template <typename V>
 int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* parts) {
   const int n = s->n;
   if (const PanelMatrix* P = panels_K<V>(s)) {
-    panel_spmv(*P, sub ? nullptr : active_flag(s), v, n, w, (V*)s->partK);
+    if constexpr (sizeof(V) == 8) dc_panel_spmv(s, *P, sub ? nullptr : active_flag(s), v, n, w, (double*)s->partK);
+    else panel_spmv(*P, sub ? nullptr : active_flag(s), v, n, w, (V*)s->partK);
     constexpr int RB = 16 / (int)sizeof(V);
     if (n % RB == 0) {
       const int nq = n / RB;
@@ -993,6 +1227,19 @@ int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* p
                          (const V*)s->partK, s->sigma, v, sub, Kv, parts);
     return g;
   }
+  if (const HybridOp* H = hyb_K<V>(s)) {
+    hyb_panel<V>(*H, sub ? nullptr : active_flag(s), v, n, w, (V*)s->partK);
+    const int G = comb_group(H->pan.np, n);
+    const int g = grid_for((long long)n * G, kBlock, kMaxReduceBlocks);
+    with_rem<V>(*H, [&](auto rm) {
+      with_G(G, [&](auto Gc) {
+        hipLaunchKernelGGL((k_hyb_kv<V, decltype(Gc)::value, rem_t<decltype(rm)>>), dim3(g), dim3(kBlock), 0, S(),
+                           (const PcgState*)s->st, n, H->pan.np, (const V*)s->partK, rm, s->sigma, v, w, sub, Kv,
+                           parts);
+      });
+    });
+    return g;
+  }
   const int g = s->k_nb < kMaxReduceBlocks ? s->k_nb : kMaxReduceBlocks;
   hipLaunchKernelGGL((k_rb_Kv<V, V>), dim3(g), dim3(kBlock), 0, S(), (const PcgState*)s->st, (const int*)s->k_rows,
                      s->k_nb, (const int*)s->krp, (const int*)s->kci, csr_vals_K<V>(s), n, s->sigma, v, w, sub, Kv,
@@ -1005,7 +1252,7 @@ int op_kv(HipPcg* s, const V* v, const V* w, const double* sub, V* Kv, double* p
 void op_finish_z(HipPcg* s, double* b, const int* when) {
   const int n = s->n, m = s->m;
   if (const PanelMatrix* P = panels_A<double>(s)) {
-    panel_spmv(*P, when, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
+    dc_panel_spmv(s, *P, when, (const double*)s->x64, n, nullptr, (double*)s->partA);
     const int G = comb_group(P->np, m);
     const dim3 g(grid_for((long long)m * G, kBlock, 8192));
     if (G == 8)
@@ -1020,6 +1267,17 @@ void op_finish_z(HipPcg* s, double* b, const int* when) {
     else
       hipLaunchKernelGGL(k_comb_z<1>, g, dim3(kBlock), 0, S(), when, m, n, P->np, (const double*)s->partA, b,
                          s->ax64, s->polishing, s->rho_sc);
+  } else if (const HybridOp* H = hyb_A<double>(s)) {
+    hyb_panel<double>(*H, when, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
+    const int G = comb_group(H->pan.np, m);
+    const dim3 g(grid_for((long long)m * G, kBlock, 8192));
+    with_rem<double>(*H, [&](auto rm) {
+      with_G(G, [&](auto Gc) {
+        hipLaunchKernelGGL((k_hyb_z<decltype(Gc)::value, rem_t<decltype(rm)>>), g, dim3(kBlock), 0, S(), when, m, n,
+                           H->pan.np, (const double*)s->partA, rm, (const double*)s->x64, b, s->ax64, s->polishing,
+                           s->rho_sc);
+      });
+    });
   } else {
     if (when) {
       check(hipErrorInvalidValue, "op_finish_z: a guarded tail needs panels", __FILE__, __LINE__);
@@ -1103,7 +1361,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   // loop.  host_should_stop's answers are definitive, so the host knows which
   // of the two happened.
   const bool ahead = s->lookahead == 0 && m > 0;
-  const bool spec = ahead && s->h_flags && panels_A<double>(s) != nullptr;
+  const bool spec = ahead && s->h_flags && (panels_A<double>(s) != nullptr || hyb_A<double>(s) != nullptr);
   const int* done = (const int*)((const char*)s->st + offsetof(PcgState, done));
   auto tail = [&](bool guarded) {
     hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st,
@@ -1121,7 +1379,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
     if (ahead) {
       // the look-ahead SpMV was left out where the loop was predicted to stop
       if (it == spec_at) av_spmv<V>(s, (const V*)p, w);
-      av_finish<V>(s, w);
+      av_finish<V>(s, (const V*)p, w);
     } else if (m) {
       op_av_rho<V>(s, (const V*)p, w);
     }
@@ -1222,6 +1480,8 @@ void pcg_free(LinSysSolver* self) {
   dev_free(s->st); dev_free(s->parts);
   dev_free(s->krp); dev_free(s->kci); dev_free(s->kval64); dev_free(s->kval32); dev_free(s->k_rows);
   panel_free(&s->pA32); panel_free(&s->pK32); panel_free(&s->pK64);
+  dev_free(s->dcA32); dev_free(s->dcK32);
+  hybrid_free(&s->hA32); hybrid_free(&s->hK32); hybrid_free(&s->hA64); hybrid_free(&s->hK64);
   dev_free(s->partA); dev_free(s->partK);
   if (s->h_st) (void)hipHostFree(s->h_st);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
@@ -1408,10 +1668,17 @@ OSQPInt osqp_amd_linsys_time_kernels(LinSysSolver* ls, OSQPInt reps, double* ms_
   // partial per (panel, row), the multiplied vector, task descriptors.  CSR:
   // values + 32-bit indices, row pointers, the vector, the output (+ rho).
   const double nnzA = (double)s->A->nnz_in, nnzK = (double)s->P->nnz + (m ? nnzA : 0.0);
+  // Hybrids: the panel SpMV and its combine (hyb_bytes + the epilogue).
+  const HybridOp* HA = s->bits == 32 ? hyb_A<float>(s) : hyb_A<double>(s);
+  const HybridOp* HK = s->bits == 32 ? hyb_K<float>(s) : hyb_K<double>(s);
   if (bytes_av)
-    *bytes_av = PA ? panel_spmv_bytes(*PA, (int)vb) : nnzA * (vb + 4.0) + (m + 1.0) * 4.0 + n * vb + m * (vb + 8.0);
+    *bytes_av = PA ? panel_spmv_bytes(*PA, (int)vb)
+                   : HA ? hyb_bytes(*HA, vb, vb) + m * (vb + 8.0)
+                        : nnzA * (vb + 4.0) + (m + 1.0) * 4.0 + n * vb + m * (vb + 8.0);
   if (bytes_kv)
-    *bytes_kv = PK ? panel_spmv_bytes(*PK, (int)vb) : nnzK * (vb + 4.0) + (n + 1.0) * 4.0 + (n + (double)m) * vb + 2.0 * n * vb;
+    *bytes_kv = PK ? panel_spmv_bytes(*PK, (int)vb)
+                   : HK ? hyb_bytes(*HK, vb, vb) + 2.0 * n * vb
+                        : nnzK * (vb + 4.0) + (n + 1.0) * 4.0 + (n + (double)m) * vb + 2.0 * n * vb;
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
   HIPCHK(hipEventDestroy(e2));
@@ -1445,11 +1712,10 @@ OSQPInt osqp_amd_linsys_time_fp64_kernels(LinSysSolver* ls, OSQPInt reps, double
                      (const double*)s->dtmp, s->w64, 1);
   HIPCHK(hipStreamSynchronize(S()));
   HIPCHK(hipEventRecord(e0, S()));
-  for (int r = 0; r < reps; ++r)
-    panel_spmv(*PA, nullptr, (const double*)s->x64, n, (const double*)nullptr, (double*)s->partA);
+  for (int r = 0; r < reps; ++r) dc_panel_spmv(s, *PA, nullptr, (const double*)s->x64, n, nullptr, (double*)s->partA);
   HIPCHK(hipEventRecord(e1, S()));
   for (int r = 0; r < reps; ++r)
-    panel_spmv(*PK, nullptr, (const double*)s->x64, n, (const double*)s->w64, (double*)s->partK);
+    dc_panel_spmv(s, *PK, nullptr, (const double*)s->x64, n, (const double*)s->w64, (double*)s->partK);
   HIPCHK(hipEventRecord(e2, S()));
   HIPCHK(hipEventSynchronize(e2));
   float t01 = 0.f, t12 = 0.f;
@@ -1457,14 +1723,63 @@ OSQPInt osqp_amd_linsys_time_fp64_kernels(LinSysSolver* ls, OSQPInt reps, double
   HIPCHK(hipEventElapsedTime(&t12, e1, e2));
   if (ms_a) *ms_a = (double)t01 / reps;
   if (ms_k) *ms_k = (double)t12 / reps;
-  if (bytes_a) *bytes_a = panel_spmv_bytes(*PA, 8);
-  if (bytes_k) *bytes_k = panel_spmv_bytes(*PK, 8);
+  if (bytes_a) *bytes_a = dc_panel_bytes(s, *PA);
+  if (bytes_k) *bytes_k = dc_panel_bytes(s, *PK);
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
   HIPCHK(hipEventDestroy(e2));
   // w64 and the A x cache are rebuilt by the next solve
   s->ax_valid = 0;
   return 0;
+}
+
+// Applies one of the linsys' operators to a host vector, through the same
+// kernels the solve uses (panels, hybrids or row-block CSR):
+//   which 0: y = K x with K = P + sigma I + A' diag(rho) A, in the CG precision
+//            (x rounded to it first; y returned in FP64);
+//   which 1: y = K x with the defect correction's FP64-vector operator;
+//   which 2: y = A x (m entries) with the FP64-vector product of z = A x.
+// The CG state is reset afterwards (the next solve starts fresh).
+OSQPInt osqp_amd_linsys_apply(LinSysSolver* ls, OSQPInt which, const OSQPFloat* x, OSQPFloat* y) {
+  HipPcg* s = (HipPcg*)ls;
+  if (!s || which < 0 || which > 2 || !x || !y || s->polishing) return 1;
+  const int n = s->n, m = s->m;
+  if (which == 2 && m == 0) return 1;
+  update_precond(s);
+  hipLaunchKernelGGL(k_set_active, dim3(1), dim3(64), 0, S(), s->st, 1);
+  upload(s->x64, x, sizeof(double) * (size_t)n);
+  double* out = nullptr;
+  if (which == 0) {
+    if (s->bits == 32) {
+      hipLaunchKernelGGL(k_cast<float>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (long long)n,
+                         (const double*)s->x64, (float*)s->p);
+      if (m) op_av_rho<float>(s, (const float*)s->p, (float*)s->w);
+      op_kv<float>(s, (const float*)s->p, (const float*)s->w, nullptr, (float*)s->Kp, s->parts);
+      hipLaunchKernelGGL(k_widen, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), n, (const float*)s->Kp,
+                         s->r64);
+    } else {
+      if (m) op_av_rho<double>(s, (const double*)s->x64, (double*)s->w);
+      op_kv<double>(s, (const double*)s->x64, (const double*)s->w, nullptr, s->r64, s->parts);
+    }
+    out = s->r64;
+  } else if (which == 1) {
+    if (m) op_av_rho<double>(s, (const double*)s->x64, s->w64);
+    op_kv<double>(s, (const double*)s->x64, (const double*)s->w64, nullptr, s->r64, s->parts);
+    out = s->r64;
+  } else {
+    double* b = (double*)dev_alloc(sizeof(double) * (size_t)(n + m));
+    if (!b) return 1;
+    op_finish_z(s, b, nullptr);
+    download(y, b + n, sizeof(double) * (size_t)m);
+    dev_free(b);
+  }
+  if (out) download(y, out, sizeof(double) * (size_t)n);
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, S(), s->st, s->tol_fraction);
+  s->ax_valid = 0;
+  s->host_solves = 0;
+  HIPCHK(hipStreamSynchronize(S()));
+  if (s->h_flags) memset(s->h_flags, 0, sizeof(int) * (size_t)s->flags_len);
+  return osqp_amd_last_error() ? 1 : 0;
 }
 
 OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per_solve) {
@@ -1477,11 +1792,15 @@ OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per
   auto av = [&](const PanelMatrix* P, double b) {
     const double epi = m * (8.0 + b);  // rho read, w written
     if (P) return panel_spmv_bytes(*P, (int)b) + (double)P->np * m * b + epi;
+    const HybridOp* H = b == 4.0 ? hyb_A<float>(s) : hyb_A<double>(s);
+    if (H) return hyb_bytes(*H, hyb_pv(*H, b), b) + epi;
     return nnzA * (b + 4.0) + (m + 1.0) * 4.0 + n * b + epi;
   };
   auto kv = [&](const PanelMatrix* P, double b) {
     const double epi = 3.0 * n * b;  // p (sigma p), sub or nothing, Kp written
     if (P) return panel_spmv_bytes(*P, (int)b) + (double)P->np * n * b + epi;
+    const HybridOp* H = b == 4.0 ? hyb_K<float>(s) : hyb_K<double>(s);
+    if (H) return hyb_bytes(*H, hyb_pv(*H, b), b) + epi;
     return nnzK * (b + 4.0) + (n + 1.0) * 4.0 + (n + m) * b + epi;
   };
   const PanelMatrix* A32 = s->bits == 32 ? panels_A<float>(s) : panels_A<double>(s);
@@ -1495,8 +1814,17 @@ OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per
   // residual, CG init (r64, dinv read; r, y, p, d written), correction (d,
   // x64 read; x64, b1 written), z = A x64 (A panels; b2 and ax written: the
   // same 16 bytes per row as av's rho read and w write)
-  const double outer = 4.0 * m * 8.0 + kv(K64, 8.0) + n * (8.0 + vb) + 4.0 * n * vb + n * (vb + 8.0) +
-                       2.0 * n * 8.0 + (m ? av(A64, 8.0) : 0.0);
+  // the FP64-vector products (FP32 values in FP32 mode)
+  auto av64 = [&](const PanelMatrix* P) {
+    if (P) return dc_panel_bytes(s, *P) + (double)P->np * m * 8.0 + m * 16.0;
+    return av(P, 8.0);
+  };
+  auto kv64 = [&](const PanelMatrix* P) {
+    if (P) return dc_panel_bytes(s, *P) + (double)P->np * n * 8.0 + 3.0 * n * 8.0;
+    return kv(P, 8.0);
+  };
+  const double outer = 4.0 * m * 8.0 + kv64(K64) + n * (8.0 + vb) + 4.0 * n * vb + n * (vb + 8.0) +
+                       2.0 * n * 8.0 + (m ? av64(A64) : 0.0);
   if (per_cg_iter) *per_cg_iter = cg;
   if (per_solve) *per_solve = outer;
   return 0;

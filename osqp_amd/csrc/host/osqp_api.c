@@ -820,6 +820,11 @@ OSQPInt osqp_amd_time_pcg_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_a
   return osqp_amd_linsys_time_kernels(solver->work->linsys_solver, reps, ms_av, ms_kv, bytes_av, bytes_kv);
 }
 
+OSQPInt osqp_amd_apply_operator(OSQPSolver* solver, OSQPInt which, const OSQPFloat* x, OSQPFloat* y) {
+  if (!solver || !solver->work || !solver->work->linsys_solver) return 1;
+  return osqp_amd_linsys_apply(solver->work->linsys_solver, which, x, y);
+}
+
 OSQPInt osqp_amd_time_fp64_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_a, double* ms_k, double* bytes_a,
                                    double* bytes_k) {
   if (!solver || !solver->work || !solver->work->linsys_solver) return 1;

@@ -62,6 +62,8 @@ def load_library():
         lib.osqp_amd_solver_stats.argtypes = [C.POINTER(T.OSQPSolver), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                               C.POINTER(C.c_double)]
         lib.osqp_amd_solver_stats.restype = C.c_int
+        lib.osqp_amd_apply_operator.argtypes = [C.POINTER(T.OSQPSolver), C.c_int, vp, vp]
+        lib.osqp_amd_apply_operator.restype = C.c_int
         lib.osqp_amd_time_pcg_kernels.argtypes = [C.POINTER(T.OSQPSolver), C.c_int] + [C.POINTER(C.c_double)] * 4
         lib.osqp_amd_time_pcg_kernels.restype = C.c_int
         lib.osqp_amd_time_fp64_kernels.argtypes = [C.POINTER(T.OSQPSolver), C.c_int] + [C.POINTER(C.c_double)] * 4
@@ -231,6 +233,18 @@ class OSQP:
         it, sv, ms = C.c_int(), C.c_int(), C.c_double()
         self.lib.osqp_amd_solver_stats(self._solver, C.byref(it), C.byref(sv), C.byref(ms))
         return SimpleNamespace(pcg_iters=it.value, linsys_solves=sv.value, solve_ms=ms.value)
+
+    def apply_operator(self, which, x):
+        """The linear system's operator on x through its own kernels:
+        which 0: K x in the CG precision; 1: K x with the FP64-vector operator
+        of the defect correction; 2: A x (z = A x of the defect correction).
+        K = P + sigma I + A' diag(rho) A on the solver's scaled data."""
+        n, m = self.n, self.m
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(m if which == 2 else n)
+        if self.lib.osqp_amd_apply_operator(self._solver, int(which), x.ctypes.data, y.ctypes.data):
+            raise RuntimeError("osqp_amd_apply_operator failed")
+        return y
 
     def time_kernels(self, reps=20):
         """Mean ms and algorithmic bytes of one launch of each CG operator kernel."""
