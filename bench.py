@@ -6,7 +6,11 @@ the FP32-PCG engine (FP32 CG with FP64 defect correction; FP64 residuals and
 termination).  A "step" is one full cold-started osqp_solve of that QP with its
 data already resident in HBM (setup excluded).  The plain FP64 HIP backend is
 run on the same QP as the GPU comparison, and the CPU oracle's FP64 PCG path is
-timed on a bounded prefix of the same solve as the CPU baseline.
+timed over one termination-check interval (25 ADMM iterations) of the same
+solve as the CPU baseline.  Beside it: the random-QP, lasso, MPC-stream and
+portfolio-stream configurations, each with its error against the FP64 oracle
+and, for the streams, the oracle's direct path on a pool of the host's cores
+(CPU model and core count reported).
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--small]
 
@@ -33,6 +37,187 @@ def log(*a):
 
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+def host_cpu():
+    """CPU model and core counts of the host this runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff}
+
+
+def cpu_workers():
+    """Worker processes for the CPU pools: the cores this process may use,
+    at most 16 (a one-GPU job's share of the host)."""
+    return max(1, min(16, host_cpu()["affinity"] or 1))
+
+
+def lib_sync():
+    """Waits for the library's queued GPU work (its own stream)."""
+    from osqp_amd import load_library
+    load_library().osqp_amd_algebra_sync()
+
+
+def mpc_requests(seed, count):
+    """Horizons and problem seeds of the MPC stream: request k has horizon
+    Ns[k] ~ U{10..50} and problem seed seed * 100000 + k."""
+    rng = np.random.default_rng(seed + 17)
+    return rng.integers(10, 51, count), [seed * 100000 + k for k in range(count)]
+
+
+def portfolio_stream(seed, n_problems, n_req):
+    """The portfolio stream's problems (n ~ U{500..5000}) and requests
+    (problem index, new q) in order."""
+    from osqp_amd import problems as pr
+    rng = np.random.default_rng(seed + 29)
+    sizes = rng.integers(500, 5001, n_problems)
+    probs = [pr.portfolio(int(nn), seed=seed * 1000 + k) for k, nn in enumerate(sizes)]
+    ks = [int(rng.integers(n_problems)) for _ in range(n_req)]
+    reqs = [(k, pr.portfolio_q(rng.standard_normal(probs[k]["P"].shape[0] - probs[k]["k"]), probs[k]["k"]))
+            for k in ks]
+    return probs, reqs
+
+
+# ---- CPU baseline legs: the oracle (oracle/osqp_oracle.c, FP64) ----------------
+# These run in a child process started before this process touches the GPU;
+# the pools fork from that child, which never initialises the GPU.
+
+def _cpu_mpc_chunk(job):
+    """Oracle direct (LDL') path, setup + solve per request, one core."""
+    from oracle import OracleSolver
+    from osqp_amd import problems as pr
+    Ns, seeds = job
+    out = []
+    for N, sd in zip(Ns, seeds):
+        p = pr.mpc(int(N), seed=int(sd))
+        t0 = time.perf_counter()
+        o = OracleSolver()
+        o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=1, eps_abs=1e-4, eps_rel=1e-4, verbose=0)
+        o.solve()
+        out.append((int(o.info.status_val), float(o.info.obj_val), time.perf_counter() - t0))
+    return out
+
+
+def _cpu_portfolio_problem(job):
+    """One resident portfolio on the oracle's direct path: setup + first solve
+    (untimed), then its requests in stream order, each update_data_vec(q) +
+    warm-started solve (timed)."""
+    from oracle import OracleSolver
+    p, reqs = job
+    o = OracleSolver()
+    o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=1, eps_abs=1e-4, eps_rel=1e-4, verbose=0)
+    o.solve()
+    out = []
+    for j, q in reqs:
+        t0 = time.perf_counter()
+        o.update_data_vec(q=q)
+        o.solve()
+        out.append((j, int(o.info.status_val), float(o.info.obj_val), time.perf_counter() - t0))
+    return out
+
+
+def cpu_legs(seed, mpc_warm, mpc_req, pf_problems, pf_req, large_qp):
+    import multiprocessing as mp
+    from oracle import OracleSolver
+    from osqp_amd import problems as pr
+    W = cpu_workers()
+    out = {"host": host_cpu(), "workers": W}
+    ctx = mp.get_context("fork")
+    # MPC stream: every timed request, W worker processes
+    Ns, seeds = mpc_requests(seed, mpc_warm + mpc_req)
+    Ns, seeds = Ns[mpc_warm:], seeds[mpc_warm:]
+    chunks = [(Ns[i::W], seeds[i::W]) for i in range(W)]
+    t0 = time.perf_counter()
+    with ctx.Pool(W) as pool:
+        res = pool.map(_cpu_mpc_chunk, chunks)
+    wall = time.perf_counter() - t0
+    status, obj, svc = [None] * mpc_req, [None] * mpc_req, [None] * mpc_req
+    for i, rr in enumerate(res):
+        for j, (st, ob, dt) in enumerate(rr):
+            k = i + j * W
+            status[k], obj[k], svc[k] = st, ob, dt
+    out["mpc"] = {"qps": mpc_req / wall, "wall_s": wall, "status": status, "obj": obj,
+                  "solve_ms": {"p50": percentile(svc, 50) * 1e3, "p99": percentile(svc, 99) * 1e3},
+                  "single_core_qps": mpc_req / float(np.sum(svc))}
+    log(f"[cpu] MPC: {mpc_req} requests on {W} processes in {wall:.1f}s ({mpc_req / wall:.0f} QPs/s)")
+    # portfolio stream: one problem's requests in order per job
+    probs, reqs = portfolio_stream(seed, pf_problems, pf_req)
+    jobs = [(probs[k], [(j, q) for j, (kk, q) in enumerate(reqs) if kk == k]) for k in range(pf_problems)]
+    t0 = time.perf_counter()
+    with ctx.Pool(W) as pool:
+        res = pool.map(_cpu_portfolio_problem, jobs)
+    wall = time.perf_counter() - t0
+    pst, pob, psv = {}, {}, []
+    for rr in res:
+        for j, st, ob, dt in rr:
+            pst[j], pob[j] = st, ob
+            psv.append(dt)
+    busy = float(np.sum(psv))
+    out["portfolio"] = {"status": [pst[j] for j in range(pf_req)], "obj": [pob[j] for j in range(pf_req)],
+                        "qps_packed": pf_req / (busy / W), "single_core_qps": pf_req / busy,
+                        "solve_ms": {"p50": percentile(psv, 50) * 1e3, "p99": percentile(psv, 99) * 1e3},
+                        "wall_s_with_setups": wall}
+    log(f"[cpu] portfolio: {pf_req} re-solves, {busy:.1f}s of solves on {W} processes")
+    # large QP: the oracle's FP64 PCG path over one termination-check interval
+    if large_qp:
+        n, m = large_qp
+        p = pr.random_qp(n, m, 1e-4, 2e-3, seed=seed) if n == 50000 else pr.random_qp(n, m, 1e-3, 2e-2, seed=seed)
+        o = OracleSolver()
+        t0 = time.perf_counter()
+        assert o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], linsys_solver=2, max_iter=25, check_termination=25,
+                       eps_abs=1e-4, eps_rel=1e-4, verbose=0) == 0
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.solve()
+        t25 = time.perf_counter() - t0
+        out["large_qp"] = {"admm_iters": int(o.info.iter), "cg_iters": int(o.total_pcg_iters()), "solve_s": t25,
+                           "setup_s": t_setup}
+        log(f"[cpu] large QP: {o.info.iter} ADMM iterations ({o.total_pcg_iters()} CG) in {t25:.1f}s")
+    return out
+
+
+def run_cpu_legs(args, large_qp):
+    """Runs the CPU legs in a child process (this process has not touched the
+    GPU yet) and returns their results."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-legs", "--seed", str(args.seed),
+           "--mpc-requests", str(args.mpc_requests), "--mpc-warmup", str(args.mpc_warmup),
+           "--portfolio-problems", str(args.portfolio_problems), "--portfolio-requests",
+           str(args.portfolio_requests)]
+    if large_qp:
+        cmd += ["--cpu-large-qp", f"{large_qp[0]}x{large_qp[1]}"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU legs failed ({r.returncode})")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def oracle_fixture_errors(name, prob, x, y, obj):
+    """Errors of a GPU point against the oracle's FP64 PCG answer held in
+    tests/golden/<name>_oracle.npz (written by tests/golden/make_full_size_fixtures.py),
+    when the fixture is present and was made from the same data."""
+    path = os.path.join(ROOT, "tests", "golden", f"{name}_oracle.npz")
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    chk = np.array([prob["P"].nnz, prob["A"].nnz, np.sum(prob["P"].data), np.sum(prob["A"].data),
+                    np.sum(prob["q"]), np.sum(np.clip(prob["l"], -1e30, 1e30)),
+                    np.sum(np.clip(prob["u"], -1e30, 1e30))])
+    if not np.allclose(chk, z["checksums"], rtol=1e-12, atol=0):
+        return {"fixture": "different data"}
+    info = z["info"]
+    fo = float(info[3])
+    return {"oracle_status": int(info[0]), "oracle_iter": int(info[1]), "oracle_obj": fo,
+            "x_inf": float(np.max(np.abs(x - z["x"]))), "y_inf": float(np.max(np.abs(y - z["y"]))),
+            "obj_rel": abs(obj - fo) / max(1.0, abs(fo)), "oracle_solve_s_on_build_host": float(info[8])}
 
 
 def measured_traffic(kernel):
@@ -88,29 +273,32 @@ def _mpc_stream(eng, reqs, arrivals, label):
     return time.perf_counter() - t_start, lat, status, objs, iters
 
 
-def serve_mpc(n_req, seed):
+def serve_mpc(n_req, n_warm, seed, cpu):
     """MPC stream (nx=12, nu=4, N ~ U{10..50}) through the batched FP32-PCG
-    engine.  Throughput: Poisson arrivals at a rate far above saturation (up to
-    4096 in flight), so the engine is the bottleneck.  Latency under load: the
-    same requests again, Poisson arrivals at 70 % of that throughput.  Latency
-    = result seen by the host - arrival.  The plain FP64 engine (osqp_setup +
-    osqp_solve per request, one at a time) on a sample of the same requests is
-    the GPU comparison; statuses are compared on it."""
-    import torch
+    engine: n_warm distinct warm-up requests (code paths and the team-size
+    tuner), then n_req timed ones.  Throughput: Poisson arrivals at a rate far
+    above saturation (up to 4096 in flight), so the engine is the bottleneck.
+    Latency under load: the same requests again, Poisson arrivals at 70 % of
+    that throughput.  Latency = result seen by the host - arrival.  Every
+    timed request's status is compared with the CPU oracle's direct path
+    (the reference's default), and its objective likewise.  The plain FP64
+    engine (osqp_setup + osqp_solve per request, one at a time) on a sample
+    of the same requests is the GPU comparison."""
     from osqp_amd import OSQP
     from osqp_amd import problems as pr
     from osqp_amd.serve import BatchEngine
     st = dict(eps_abs=1e-4, eps_rel=1e-4)
-    rng = np.random.default_rng(seed + 17)
-    Ns = rng.integers(10, 51, n_req)
+    Ns, seeds = mpc_requests(seed, n_warm + n_req)
     t0 = time.time()
-    reqs = [pr.mpc(int(N), seed=seed * 100000 + k) for k, N in enumerate(Ns)]
-    log(f"[serve] generated {n_req} MPC requests in {time.time() - t0:.1f}s")
+    allreqs = [pr.mpc(int(N), seed=sd) for N, sd in zip(Ns, seeds)]
+    warm, reqs = allreqs[:n_warm], allreqs[n_warm:]
+    log(f"[serve] generated {n_warm} + {n_req} MPC requests in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(seed + 18)
     eng = BatchEngine(precision=32, max_active=4096, **st)
-    eng.solve_batch(reqs[:512])  # warm-up: code paths and the team-size tuner
+    eng.solve_batch(warm)
     rate = 1e6  # requests/s: far above saturation
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    torch.cuda.synchronize()
+    lib_sync()
     c0 = eng.counters()
     elapsed, lat, status, objs, iters = _mpc_stream(eng, reqs, arrivals, "saturated")
     c1 = eng.counters()
@@ -123,56 +311,63 @@ def serve_mpc(n_req, seed):
     same_runs = sum(int(status70[k] == status[k]) for k in range(n_req))
     # plain FP64 engine, one request at a time (setup + solve), on a sample
     k_plain = 24
-    same = 0
-    plain = []
+    same_plain = 0
     ts = time.perf_counter()
     for k in range(k_plain):
         p = reqs[k]
         s = OSQP(precision=64)
         assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
         r = s.solve()
-        same += int(r.info.status_val == status[k])
-        plain.append(r)
+        same_plain += int(r.info.status_val == status[k])
         s.close()
     plain_qps = k_plain / (time.perf_counter() - ts)
-    obj_err = max(abs(objs[k] - plain[k].info.obj_val) / max(1.0, abs(plain[k].info.obj_val)) for k in range(k_plain))
     pct = lambda v, q: float(np.percentile(v, q) * 1e3)
+    vs_oracle = None
+    if cpu is not None:
+        cst, cob = cpu["mpc"]["status"], cpu["mpc"]["obj"]
+        errs = [abs(objs[k] - cob[k]) / max(1.0, abs(cob[k])) for k in range(n_req) if cst[k] == 1 and status[k] == 1]
+        vs_oracle = {"requests": n_req, "statuses_identical": int(sum(int(status[k] == cst[k]) for k in range(n_req))),
+                     "obj_rel_err_vs_direct": {"max": float(np.max(errs)) if errs else None,
+                                               "p99": percentile(errs, 99), "p50": percentile(errs, 50)}}
     log(f"[serve] batched FP32: {qps:.0f} QPs/s, p50 {pct(lat, 50):.1f} ms, p99 {pct(lat, 99):.1f} ms; "
         f"at {rate70:.0f}/s: p50 {pct(lat70, 50):.1f} ms, p99 {pct(lat70, 99):.1f} ms; "
-        f"plain FP64: {plain_qps:.1f} QPs/s; statuses identical on {same}/{k_plain}")
-    return {"workload": f"MPC stream nx=12 nu=4 N~U{{10..50}}, {n_req} requests, Poisson arrivals at "
-                        f"{rate:.0e}/s, <=4096 in flight, eps 1e-4",
-            "qps": qps, "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
-            "at_70pct_load": {"arrival_rate_per_s": rate70, "qps": n_req / el70,
-                              "latency_ms": {"p50": pct(lat70, 50), "p99": pct(lat70, 99)},
-                              "statuses_same_as_saturated_run": same_runs},
-            "mean_admm_iters": float(np.mean(iters)),
-            "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
-            "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
-            "plain_sample": k_plain, "statuses_identical_on_sample": same,
-            "max_obj_rel_err_vs_fp64_on_sample": obj_err,
-            "engine": {"epochs": c1["epochs"] - c0["epochs"], "gpu_s": c1["gpu_s"] - c0["gpu_s"],
-                       "tuned_team_per_bucket": tuning}}
+        f"plain FP64: {plain_qps:.1f} QPs/s; vs oracle: {vs_oracle and vs_oracle['statuses_identical']}/{n_req}")
+    out = {"workload": f"MPC stream nx=12 nu=4 N~U{{10..50}}, {n_req} requests after {n_warm} distinct warm-up "
+                       f"requests, Poisson arrivals at {rate:.0e}/s, <=4096 in flight, eps 1e-4",
+           "qps": qps, "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+           "at_70pct_load": {"arrival_rate_per_s": rate70, "qps": n_req / el70,
+                             "latency_ms": {"p50": pct(lat70, 50), "p99": pct(lat70, 99)},
+                             "statuses_same_as_saturated_run": same_runs},
+           "mean_admm_iters": float(np.mean(iters)),
+           "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
+           "vs_cpu_oracle_direct": vs_oracle,
+           "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
+           "plain_sample": k_plain, "statuses_identical_to_plain_on_sample": same_plain,
+           "engine": {"epochs": c1["epochs"] - c0["epochs"], "gpu_s": c1["gpu_s"] - c0["gpu_s"],
+                      "tuned_team_per_bucket": tuning}}
+    if cpu is not None:
+        c = cpu["mpc"]
+        out["cpu_oracle"] = {"qps": c["qps"], "processes": cpu["workers"], "single_core_qps": c["single_core_qps"],
+                             "solve_ms": c["solve_ms"], "kind": "port",
+                             "sample": f"the same {n_req} timed requests, setup + solve each on the oracle's direct "
+                                       f"(LDL') path, {cpu['workers']} processes"}
+        out["ratio_vs_cpu_oracle"] = qps / c["qps"]
+    return out
 
 
-def serve_portfolio(n_problems, n_req, seed):
+def serve_portfolio(n_problems, n_req, seed, cpu):
     """Portfolio stream: n ~ U{500..5000} assets, problems resident on the GPU;
     each request draws a new mu, replaces q (osqp_update_data_vec semantics)
     and re-solves warm-started.  Closed loop: an idle problem takes its next
-    request as soon as its previous result is seen.  The plain FP64 engine,
-    one resident solver per problem doing update(q) + solve, on a sample of
-    the same requests, is the GPU comparison."""
+    request as soon as its previous result is seen.  Statuses and objectives
+    are compared with the CPU oracle's direct path on the same request
+    sequence; the plain FP64 engine, one resident solver per problem doing
+    update(q) + solve, on a sample of the same requests, is the GPU comparison."""
     from osqp_amd import OSQP
-    from osqp_amd import problems as pr
     from osqp_amd.serve import BatchEngine
     st = dict(eps_abs=1e-4, eps_rel=1e-4)
-    rng = np.random.default_rng(seed + 29)
-    sizes = rng.integers(500, 5001, n_problems)
     t0 = time.time()
-    probs = [pr.portfolio(int(nn), seed=seed * 1000 + k) for k, nn in enumerate(sizes)]
-    reqs = [(int(rng.integers(n_problems)), None) for _ in range(n_req)]
-    reqs = [(k, pr.portfolio_q(rng.standard_normal(probs[k]["P"].shape[0] - probs[k]["k"]), probs[k]["k"]))
-            for k, _ in reqs]
+    probs, reqs = portfolio_stream(seed, n_problems, n_req)
     log(f"[serve] generated {n_problems} portfolios, {n_req} requests in {time.time() - t0:.1f}s")
     eng = BatchEngine(precision=32, max_active=4096, **st)
     pid = [eng.add(p["P"], p["q"], p["A"], p["l"], p["u"], resident=True) for p in probs]
@@ -191,19 +386,21 @@ def serve_portfolio(n_problems, n_req, seed):
     for j, (k, q) in enumerate(reqs):
         queue[k].append(j)
     t_start = time.perf_counter()
-    issued, lat, status, busy, objs = {}, [], {}, {}, {}
+    issued, lat, status, objs = {}, [], {}, {}
     for k in range(n_problems):
         if queue[k]:
             j = queue[k].pop(0)
             eng.resolve(pid[k], reqs[j][1])
             issued[pid[k]] = (j, time.perf_counter())
     t_log = time.perf_counter()
+    capped = False
     while issued:
         if time.perf_counter() - t_log > 5.0:
             log(f"[serve] portfolio: {len(lat)}/{n_req} requests done")
             t_log = time.perf_counter()
-        if time.perf_counter() - t_start > 120.0:
+        if not capped and time.perf_counter() - t_start > 120.0:
             # bound the section's run time: stop issuing, drain what is in flight
+            capped = True
             for k in queue:
                 queue[k] = []
         for i in eng.step():
@@ -233,28 +430,39 @@ def serve_portfolio(n_problems, n_req, seed):
             assert s.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **st) == 0
             s.solve()
             solvers[k] = s
-    same = 0
-    obj_err = 0.0
     ts = time.perf_counter()
     for j in sample:
         k, q = reqs[j]
         solvers[k].update(q=q)
-        r = solvers[k].solve()
-        same += int(r.info.status_val == status[j])
-        obj_err = max(obj_err, abs(objs[j] - r.info.obj_val) / max(1.0, abs(r.info.obj_val)))
+        solvers[k].solve()
     plain_qps = k_plain / (time.perf_counter() - ts)
     for s in solvers.values():
         s.close()
-    log(f"[serve] portfolio batched FP32: {qps:.0f} QPs/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
+    vs_oracle = None
+    if cpu is not None:
+        cst, cob = cpu["portfolio"]["status"], cpu["portfolio"]["obj"]
+        done = sorted(status)
+        errs = [abs(objs[j] - cob[j]) / max(1.0, abs(cob[j])) for j in done if cst[j] == 1 and status[j] == 1]
+        vs_oracle = {"requests": len(done), "statuses_identical": int(sum(int(status[j] == cst[j]) for j in done)),
+                     "obj_rel_err_vs_direct": {"max": float(np.max(errs)) if errs else None,
+                                               "p50": percentile(errs, 50)}}
+    log(f"[serve] portfolio batched FP32: {qps:.1f} QPs/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
         f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms; plain FP64: {plain_qps:.1f} QPs/s")
-    return {"workload": f"portfolio stream, {n_problems} resident problems n~U{{500..5000}}, {n_done} mu updates "
-                        f"with warm-started re-solves, eps 1e-4, closed loop",
-            "qps": qps, "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
-                                       "p99": float(np.percentile(lat, 99) * 1e3)},
-            "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
-            "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps,
-            "plain_sample": k_plain, "statuses_identical_on_sample": same,
-            "max_obj_rel_err_vs_fp64_on_sample": obj_err}
+    out = {"workload": f"portfolio stream, {n_problems} resident problems n~U{{500..5000}}, {n_done} mu updates "
+                       f"with warm-started re-solves, eps 1e-4, closed loop",
+           "qps": qps, "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                      "p99": float(np.percentile(lat, 99) * 1e3)},
+           "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
+           "vs_cpu_oracle_direct": vs_oracle,
+           "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps, "plain_sample": k_plain}
+    if cpu is not None:
+        c = cpu["portfolio"]
+        out["cpu_oracle"] = {"qps": c["qps_packed"], "processes": cpu["workers"],
+                             "single_core_qps": c["single_core_qps"], "solve_ms": c["solve_ms"], "kind": "port",
+                             "sample": f"all {n_req} requests in stream order per problem, update_data_vec + "
+                                       f"warm-started solve on the oracle's direct path; QPs/s = requests / "
+                                       f"(solve seconds / {cpu['workers']} processes)"}
+    return out
 
 
 def random_qp_small_section(seed):
@@ -320,7 +528,9 @@ def lasso_section(seed):
             "fp64_plain": {"qps": 1.0 / t64, "status": r64.info.status, "iter": r64.info.iter},
             "err_vs_fp64": {"x_inf": float(np.max(np.abs(r32.x - r64.x))), "y_inf": float(np.max(np.abs(r32.y - r64.y))),
                             "obj_rel": abs(r32.info.obj_val - r64.info.obj_val) / max(1.0, abs(r64.info.obj_val))},
-            "kkt_certificate": {"prim": c["prim"], "dual": c["dual"]}}
+            "kkt_certificate": {"prim": c["prim"], "dual": c["dual"]},
+            "err_vs_oracle_fp64_pcg": {"fp32_pcg": oracle_fixture_errors("lasso", p, r32.x, r32.y, r32.info.obj_val),
+                                       "fp64_plain": oracle_fixture_errors("lasso", p, r64.x, r64.y, r64.info.obj_val)}}
 
 
 def main():
@@ -333,24 +543,47 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--mpc-requests", type=int, default=4096)
+    ap.add_argument("--mpc-requests", type=int, default=20000)
+    ap.add_argument("--mpc-warmup", type=int, default=2000)
+    ap.add_argument("--portfolio-problems", type=int, default=64)
+    ap.add_argument("--portfolio-requests", type=int, default=512)
+    ap.add_argument("--cpu-legs", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-large-qp", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--profile-kernels", action="store_true",
                     help="set up the QP, launch only the timed SpMV kernels (reps=20 each, A then K) and a "
                          "256 MiB device copy (counter calibration); for rocprofv3 runs")
+    ap.add_argument("--profile-workload", choices=["large_qp", "lasso"], default="large_qp",
+                    help="the QP --profile-kernels sets up: the large random QP or the full-size lasso")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (one MI355X per job); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    if args.cpu_legs:
+        lq = tuple(int(v) for v in args.cpu_large_qp.split("x")) if args.cpu_large_qp else None
+        print(json.dumps(cpu_legs(args.seed, args.mpc_warmup, args.mpc_requests, args.portfolio_problems,
+                                  args.portfolio_requests, lq)), flush=True)
+        return
+
+    n, m = (5000, 50000) if args.small else (50000, 500000)
+    # CPU legs first, in a child process, before this process touches the GPU
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        cpu = run_cpu_legs(args, (n, m))
+        log(f"[bench] CPU legs done in {time.time() - t0:.0f}s on {cpu['host']}")
+
     from osqp_amd import OSQP, device_name
     from osqp_amd import problems as pr
 
-    n, m = (5000, 50000) if args.small else (50000, 500000)
     dens_M, dens_A = (1e-3, 2e-2) if args.small else (1e-4, 2e-3)
     t0 = time.time()
-    prob = pr.random_qp(n, m, dens_M, dens_A, seed=args.seed)
+    if args.profile_kernels and args.profile_workload == "lasso":
+        prob = pr.lasso(20000, 200000, 1e-3, seed=args.seed)
+        n, m = prob["P"].shape[0], prob["A"].shape[0]
+    else:
+        prob = pr.random_qp(n, m, dens_M, dens_A, seed=args.seed)
     log(f"[bench] generated n={n} m={m} nnz(A)={prob['A'].nnz} nnz(P triu)={prob['P'].nnz} in {time.time() - t0:.1f}s")
     settings = dict(eps_abs=1e-4, eps_rel=1e-4, verbose=0)
 
@@ -359,18 +592,24 @@ def main():
     eng = OSQP(precision=32)
     rc = eng.setup(prob["P"], prob["q"], prob["A"], prob["l"], prob["u"], **settings)
     assert rc == 0, f"setup failed: {rc}"
-    torch.cuda.synchronize()
+    lib_sync()
     log(f"[bench] FP32-PCG setup {time.time() - t0:.2f}s on {device_name()}")
     if args.profile_kernels:
+        import ctypes as C
         kt = eng.time_kernels(reps=20)
         kt64 = eng.time_fp64_kernels(reps=20)
-        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
+        # a 256 MiB device-to-device copy, the counters' calibration
+        hip = C.CDLL("libamdhip64.so.7")
+        nb = 256 << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(a), C.c_size_t(nb)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nb)) == 0
+        assert hip.hipMemset(a, 1, C.c_size_t(nb)) == 0
         for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"profile_kernels": kt, "fp64_kernels": kt64, "copy_bytes_each_way": a.numel() * 4}),
-              flush=True)
+            assert hip.hipMemcpy(b, a, C.c_size_t(nb), 3) == 0  # device to device
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(json.dumps({"profile_kernels": kt, "fp64_kernels": kt64, "copy_bytes_each_way": nb}), flush=True)
         eng.close()
         return
     for w in range(args.warmup):
@@ -379,7 +618,7 @@ def main():
         log(f"[bench] warmup {w}: status={r.info.status} iter={r.info.iter} t={r.info.solve_time:.3f}s")
     lat, iters, statuses = [], [], []
     st0 = eng.stats()
-    torch.cuda.synchronize()
+    lib_sync()
     t_start = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
@@ -390,7 +629,7 @@ def main():
         statuses.append(r.info.status)
         log(f"[bench] step {k}: status={r.info.status} iter={r.info.iter} obj={r.info.obj_val:.8e} "
             f"lat={lat[-1] * 1e3:.1f}ms")
-    torch.cuda.synchronize()
+    lib_sync()
     elapsed = time.perf_counter() - t_start
     st1 = eng.stats()
     x32, y32, obj32 = r.x, r.y, r.info.obj_val
@@ -438,6 +677,7 @@ def main():
         r64 = e64.solve()
         t64 = time.perf_counter() - ts
         fp64 = dict(qps=1.0 / t64, status=r64.info.status, iter=r64.info.iter, obj=r64.info.obj_val,
+                    err_vs_oracle_fp64_pcg=oracle_fixture_errors("large_qp", prob, r64.x, r64.y, r64.info.obj_val),
                     err_x_inf=float(np.max(np.abs(x32 - r64.x))),
                     err_y_inf=float(np.max(np.abs(y32 - r64.y))),
                     err_obj_rel=float(abs(obj32 - r64.info.obj_val) / max(1.0, abs(r64.info.obj_val))))
@@ -445,32 +685,32 @@ def main():
             f"|dx|={fp64['err_x_inf']:.2e} |dobj|={fp64['err_obj_rel']:.2e}")
         e64.close()
 
-    # ---- CPU baseline: oracle FP64 PCG, bounded prefix of the same solve -------
-    cpu = None
-    if not args.no_cpu:
-        from oracle import OracleSolver
-        k_cpu = 3
-        o = OracleSolver()
-        t0 = time.time()
-        assert o.setup(prob["P"], prob["q"], prob["A"], prob["l"], prob["u"], linsys_solver=2,
-                       max_iter=k_cpu, check_termination=0, **settings) == 0
-        log(f"[bench] oracle setup {time.time() - t0:.1f}s")
-        ts = time.perf_counter()
-        o.solve()
-        tc = time.perf_counter() - ts
-        it_per_s = k_cpu / tc
+    # ---- CPU baseline: oracle FP64 PCG over one termination-check interval -----
+    # Cost model: an ADMM iteration of the PCG path streams A and A' about
+    # 2 + 2 k_cg times (right-hand side A'(rho b2), z = A x, and A, A' per CG
+    # iteration), so the sample's seconds per ADMM iteration are scaled by that
+    # count at the GPU solve's own k_cg, times the GPU solve's ADMM iterations.
+    cpu_baseline = None
+    if cpu is not None and "large_qp" in cpu:
+        c = cpu["large_qp"]
+        k_cpu = c["cg_iters"] / max(c["admm_iters"], 1)
         mean_iters = admm_iters / args.steps
-        cpu = dict(value=it_per_s / mean_iters, unit="QPs/s", cores=1, kind="port",
-                   sample=f"oracle FP64 PCG (oracle/osqp_oracle.c), first {k_cpu} ADMM iterations of the same "
-                          f"n={n} m={m} QP ({tc:.1f}s, single thread); QPs/s = ADMM iter/s / "
-                          f"{mean_iters:.0f} iterations per GPU solve")
-        log(f"[bench] cpu oracle: {k_cpu} ADMM iterations in {tc:.1f}s")
+        s_per_admm = c["solve_s"] / max(c["admm_iters"], 1) * (2 + 2 * cg_per_admm) / (2 + 2 * k_cpu)
+        cpu_baseline = dict(value=1.0 / (s_per_admm * mean_iters), unit="QPs/s", cores=1, kind="port",
+                            sample=f"oracle FP64 PCG (oracle/osqp_oracle.c, linsys_solver=2) on the same n={n} m={m} "
+                                   f"QP: {c['admm_iters']} ADMM iterations ({c['cg_iters']} CG) in {c['solve_s']:.1f}s, "
+                                   f"one thread; scaled to {mean_iters:.0f} ADMM iterations at the GPU's "
+                                   f"{cg_per_admm:.2f} CG per iteration (A/A' passes 2 + 2 k_cg)",
+                            host=cpu["host"])
+        log(f"[bench] cpu oracle: {c['admm_iters']} ADMM iterations in {c['solve_s']:.1f}s -> "
+            f"{cpu_baseline['value']:.2e} QPs/s")
 
     # ---- served MPC stream: batched FP32-PCG engine vs plain FP64 engine -------
     small = None if args.no_serving else random_qp_small_section(args.seed)
     lasso = None if args.no_serving else lasso_section(args.seed)
-    serving = None if args.no_serving else serve_mpc(args.mpc_requests, args.seed)
-    serving_pf = None if args.no_serving else serve_portfolio(64, 512, args.seed)
+    serving = None if args.no_serving else serve_mpc(args.mpc_requests, args.mpc_warmup, args.seed, cpu)
+    serving_pf = None if args.no_serving else serve_portfolio(args.portfolio_problems, args.portfolio_requests,
+                                                              args.seed, cpu)
 
     out = {
         "metric": METRIC,
@@ -504,8 +744,9 @@ def main():
                                       "ms_per_launch": min(kt["ms_kv"], kt["ms_av"]),
                                       "bytes_per_launch": kt["bytes_av"] if dom == "k_Kv" else kt["bytes_kv"]},
                      "fp64_residual_kernels": fp64_kernels},
+        "err_vs_oracle_fp64_pcg": oracle_fixture_errors("large_qp", prob, x32, y32, obj32),
         "gpu_fp64_plain": fp64,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_baseline,
         "random_qp_small": small,
         "lasso": lasso,
         "serving_mpc": serving,
