@@ -363,9 +363,14 @@ int panel_build(PanelMatrix* P, int R, int C, int bits, const int* rp, const int
   if (want32) HIPCHK(hipMemsetAsync(P->val32 + nnz, 0, sizeof(float) * (size_t)pad, rt().stream));
   // two resident workgroups per CU, a few rounds of them: 3 rounds for FP32
   // (its tasks are shorter), 4 for FP64 -- the fastest of 1..4 rounds measured
-  // on the 500000 x 50000 operator and on [P A'] of the same QP
+  // on the 500000 x 50000 operator and on [P A'] of the same QP.  An operator
+  // with fewer tasks gets about one task per wavefront (a workgroup's x-slice
+  // staging then serves 16 tasks, not a handful), but at least one workgroup
+  // per CU.
   const int cus = rt().num_cu > 0 ? rt().num_cu : 256;
   long long g = (bits == 32 ? 6LL : 8LL) * cus;
+  const long long per_wave = P->ntask / kPanelWaves;
+  if (per_wave < g) g = per_wave > cus ? per_wave : cus;
   if (g > P->ntask) g = P->ntask > 0 ? P->ntask : 1;
   P->grid = (int)g;
   P->ver = -1;
