@@ -175,7 +175,9 @@ OSQPInt    osqp_amd_linsys_time_fp64_kernels(LinSysSolver* s, OSQPInt reps, doub
 OSQPInt    osqp_amd_linsys_apply(LinSysSolver* s, OSQPInt which, const OSQPFloat* x, OSQPFloat* y);
 /* Algorithmic bytes (each array element touched once per pass) of one CG
  * iteration and of the per-solve work outside the CG loop (cached A x path:
- * w = rho o (A x - b2), FP64 residual, correction, z = A x) of this linsys. */
+ * w = rho o (A x - b2), FP64 residual, correction, z = A x) of this linsys.
+ * With the incremental z of the FP32 engine, both are averages over its
+ * refresh cycle (one exact z = A x64 product in eight solves). */
 OSQPInt    osqp_amd_linsys_bytes(LinSysSolver* s, double* per_cg_iter, double* per_solve);
 /* Fused ADMM steps for the rho-vector case (no counterpart in the reference,
  * which issues the same arithmetic as separate vector calls; results are

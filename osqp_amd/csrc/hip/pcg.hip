@@ -217,13 +217,18 @@ __global__ void k_kfill(int n, const int* prp, const int* pci, const double* pva
 }
 
 // w = rho o (A v)
+// (ap, when given, receives A v itself: the CG's products for the running
+// A d of the incremental z = A x64)
 template <typename TX, int G>
 __global__ void k_comb_av(const PcgState* st, int m, int np, const TX* partial, const double* rho, double rho_sc,
-                          TX* w) {
+                          TX* w, TX* ap) {
   if (st && !st->active) return;
   COMB_ROWS(i, j, m, G) {
     const TX s = panel_sum<TX, G>(partial, np, m, i, j);
-    if (j == 0) w[i] = (TX)rho_at(rho, rho_sc, i) * s;
+    if (j == 0) {
+      w[i] = (TX)rho_at(rho, rho_sc, i) * s;
+      if (ap) ap[i] = s;
+    }
   }
 }
 
@@ -364,11 +369,14 @@ __device__ __forceinline__ TX hyb_sum(const TX* __restrict__ partial, int np, co
 // w = rho o (A v)
 template <typename TX, int G, typename TR>
 __global__ void k_hyb_av(const PcgState* st, int m, int np, const TX* partial, Rem<TR> rm, const TX* v,
-                         const double* rho, double rho_sc, TX* w) {
+                         const double* rho, double rho_sc, TX* w, TX* ap) {
   if (st && !st->active) return;
   COMB_ROWS(i, j, m, G) {
     const TX s = hyb_sum<TX, G>(partial, np, rm, i, j, [&](int c) { return v[c]; });
-    if (j == 0) w[i] = (TX)rho_at(rho, rho_sc, i) * s;
+    if (j == 0) {
+      w[i] = (TX)rho_at(rho, rho_sc, i) * s;
+      if (ap) ap[i] = s;
+    }
   }
 }
 
@@ -506,7 +514,8 @@ __global__ void k_tol_w(PcgState* st, int admm_iter, double spr, double sdr, dou
 template <typename V>
 __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int n, const double* r64, const V* dinv,
                              V* r, V* y, V* p, V* d, double* part_rty, double* part_rn, int settol, int admm_iter,
-                             double spr, double sdr, double tol_fraction, int reduction_threshold) {
+                             double spr, double sdr, double tol_fraction, int reduction_threshold, int m,
+                             double* u) {
   __shared__ double lds2[2 * kBlock / kWave];
   double* lds = lds2;
   const double rn0 = parts_max(part_res, nres, lds);
@@ -531,11 +540,27 @@ __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int
     double a = fabs((double)ri);
     mx = a > mx ? a : mx;
   }
+  if (u) GRID_STRIDE(i, m) u[i] = 0.0;
   double t1 = dot.value(), t2 = mx;
   block_sum_max(t1, t2, lds2);
   if (threadIdx.x == 0) {
     part_rty[blockIdx.x] = t1;
     part_rn[blockIdx.x] = t2;
+  }
+}
+
+// The incremental z = A x64 (FP32 mode): z = ax + scale u, u = A d summed
+// over the CG iterations from their own products (u_i += alpha (A p)_i);
+// ax = z; b[n:n+m] = z.  With when: only once the loop has stopped.
+__global__ void k_z_incr(const PcgState* st, const int* when, int m, int n, const double* u, double* ax, double* b) {
+  if (when && !*when) return;
+  const double s = st->scale;
+  const int ran = st->iter > 0;
+  GRID_STRIDE(i, m) {
+    double z = ax[i];
+    if (ran) z += s * u[i];
+    ax[i] = z;
+    b[n + i] = z;
   }
 }
 
@@ -579,7 +604,8 @@ __device__ inline int loop_go(const PcgState* st, double rty, double rn, int j, 
 template <typename V>
 __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, int guard, int n, V* x, V* r, V* y,
                          const V* p, const V* Kp, const V* dinv, double* part_rty, double* part_rn,
-                         const double* init_rty, const double* init_rn, int ninit, int max_iter, int* flags) {
+                         const double* init_rty, const double* init_rn, int ninit, int max_iter, int* flags, int m,
+                         const V* ap, double* u) {
   if (!st->active) return;
   __shared__ double lds2[2 * kBlock / kWave];
   double* lds = lds2;
@@ -649,6 +675,9 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
     double a = fabs((double)ri);
     mx = a > mx ? a : mx;
   }
+  // u += alpha A p, the running A d (same rounded alpha as d's update)
+  if (u)
+    for (long long i = i0; i < m; i += stride) u[i] += (double)alpha * (double)ap[i];
   double t1 = dot.value(), t2 = mx;
   block_sum_max(t1, t2, lds2);
   if (threadIdx.x == 0) {
@@ -803,7 +832,16 @@ struct HipPcg {
   // panels do not pay: FP32 ones for the CG, FP64 ones for the FP64 CG or the
   // defect correction (with FP32 copies of their values in FP32 mode)
   HybridOp hA32, hK32, hA64, hK64;
+  // FP32 mode with panelled or hybrid A: z = A x64 from the CG's own products
+  // (u = A d accumulated as d is; k_z_incr) on all but every kZRefresh-th
+  // solve, which recomputes it exactly from x64.  ap: the last A p (FP32).
+  float* ap;
+  double* u;
+  int z_incr, z_count, z_use;
 };
+
+// solves between two exact recomputations of the cached A x64
+constexpr int kZRefresh = 8;
 
 template <typename V>
 const HybridOp* hyb_A(const HipPcg* s) {
@@ -991,6 +1029,12 @@ int build_panels(HipPcg* s) {
   if (bytesA && !(s->partA = dev_alloc(bytesA))) return 1;
   if (bytesK && !(s->partK = dev_alloc(bytesK))) return 1;
   s->dcA_ver = s->dcK_ver = -1;
+  if (s->bits == 32 && m > 0 && (panels_A<float>(s) || hyb_A<float>(s))) {
+    s->ap = (float*)dev_alloc(sizeof(float) * (size_t)m);
+    s->u = (double*)dev_alloc(sizeof(double) * (size_t)m);
+    if (!s->ap || !s->u) return 1;
+    s->z_incr = 1;
+  }
   if (s->bits == 32) {
     if (s->pA64 && s->pA64->ntask > 0 && !(s->dcA32 = panel_alloc_f32vals(*s->pA64))) return 1;
     if (s->pK64.ntask > 0 && !(s->dcK32 = panel_alloc_f32vals(s->pK64))) return 1;
@@ -1153,30 +1197,32 @@ void av_finish(HipPcg* s, const V* v, V* w) {
   if (const HybridOp* H = panels_A<V>(s) ? nullptr : hyb_A<V>(s)) {
     const int G = comb_group(H->pan.np, m);
     const dim3 g(grid_for((long long)m * G, kBlock, 8192));
+    V* ap = sizeof(V) == 4 && s->z_use ? (V*)s->ap : nullptr;
     with_rem<V>(*H, [&](auto rm) {
       with_G(G, [&](auto Gc) {
         hipLaunchKernelGGL((k_hyb_av<V, decltype(Gc)::value, rem_t<decltype(rm)>>), g, dim3(kBlock), 0, S(),
-                           (const PcgState*)s->st, m, H->pan.np, (const V*)s->partA, rm, v, rho, s->rho_sc, w);
+                           (const PcgState*)s->st, m, H->pan.np, (const V*)s->partA, rm, v, rho, s->rho_sc, w, ap);
       });
     });
     return;
   }
   const PanelMatrix* P = panels_A<V>(s);
   if (!P) return;
+  V* ap = sizeof(V) == 4 && s->z_use ? (V*)s->ap : nullptr;
   const int G = comb_group(P->np, m);
   const dim3 g(grid_for((long long)m * G, kBlock, 8192));
   if (G == 8)
     hipLaunchKernelGGL((k_comb_av<V, 8>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                       (const V*)s->partA, rho, s->rho_sc, w);
+                       (const V*)s->partA, rho, s->rho_sc, w, ap);
   else if (G == 4)
     hipLaunchKernelGGL((k_comb_av<V, 4>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                       (const V*)s->partA, rho, s->rho_sc, w);
+                       (const V*)s->partA, rho, s->rho_sc, w, ap);
   else if (G == 2)
     hipLaunchKernelGGL((k_comb_av<V, 2>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                       (const V*)s->partA, rho, s->rho_sc, w);
+                       (const V*)s->partA, rho, s->rho_sc, w, ap);
   else
     hipLaunchKernelGGL((k_comb_av<V, 1>), g, dim3(kBlock), 0, S(), (const PcgState*)s->st, m, P->np,
-                       (const V*)s->partA, rho, s->rho_sc, w);
+                       (const V*)s->partA, rho, s->rho_sc, w, ap);
 }
 template <typename V>
 void op_av_rho(HipPcg* s, const V* v, V* w) {
@@ -1318,6 +1364,9 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   const double sdr = s->scaled_dual_res ? *s->scaled_dual_res : 0.0;
   const bool cached = m > 0 && s->ax_valid && s->ax_Aver == s->A->version && admm_iter > 1 && !s->polishing;
   const bool w_given = cached && s->w_given;
+  // incremental z = A x64 on a cached solve, exact every kZRefresh-th one
+  s->z_use = s->z_incr && cached && s->z_count < kZRefresh - 1 && sizeof(V) == 4;
+  s->z_count = s->z_use ? s->z_count + 1 : 0;
   s->w_given = 0;
   int nres;
   if (!cached) {
@@ -1344,7 +1393,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   }
   hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nres, n,
                      (const double*)s->r64, dinv, r, y, p, d, part3, part4, w_given ? 1 : 0, admm_iter, spr, sdr,
-                     s->tol_fraction, s->reduction_threshold);
+                     s->tol_fraction, s->reduction_threshold, m, s->z_use ? s->u : nullptr);
   // one CG iteration: the operator (two SpMVs with their combines), the
   // update, and the next loop head fused with the search-direction update.
   // With lookahead 0 the first launch of iteration it + 1 (A's panel SpMV) is
@@ -1366,7 +1415,11 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   auto tail = [&](bool guarded) {
     hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st,
                        guarded ? 1 : 0, n, (const V*)d, s->x64, b->d);
-    if (m) op_finish_z(s, b->d, guarded ? done : nullptr);
+    if (m && s->z_use)
+      hipLaunchKernelGGL(k_z_incr, dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st,
+                         guarded ? done : nullptr, m, n, (const double*)s->u, s->ax64, b->d);
+    else if (m)
+      op_finish_z(s, b->d, guarded ? done : nullptr);
   };
   int spec_at = -1, stop_at = s->max_iter;
   if (ahead) av_spmv<V>(s, (const V*)p, w);
@@ -1387,7 +1440,8 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
     osqp_amd_sec_pop(OSQP_AMD_SEC_LINSYS_MVM);
     hipLaunchKernelGGL(k_update<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nk, it, guard, n,
                        d, r, y, (const V*)p, (const V*)Kp, dinv, part0, part1, (const double*)part3,
-                       (const double*)part4, gE, s->max_iter, s->d_flags);
+                       (const double*)part4, gE, s->max_iter, s->d_flags, m,
+                       s->z_use ? (const V*)s->ap : nullptr, s->z_use ? s->u : nullptr);
     hipLaunchKernelGGL(k_head_pupdate<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part0,
                        (const double*)part1, gE, it + 1, s->max_iter, guard, s->d_flags, n, p, (const V*)y);
     if (ahead && it + 1 < s->max_iter) {
@@ -1403,6 +1457,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   }
   s->last_cg = stop_at;
   if (spec_at < 0 || spec_at != stop_at) tail(false);
+  s->z_use = 0;
   if (m) {
     s->ax_valid = 1;
     s->ax_Aver = s->A->version;
@@ -1482,6 +1537,7 @@ void pcg_free(LinSysSolver* self) {
   panel_free(&s->pA32); panel_free(&s->pK32); panel_free(&s->pK64);
   dev_free(s->dcA32); dev_free(s->dcK32);
   hybrid_free(&s->hA32); hybrid_free(&s->hK32); hybrid_free(&s->hA64); hybrid_free(&s->hK64);
+  dev_free(s->ap); dev_free(s->u);
   dev_free(s->partA); dev_free(s->partK);
   if (s->h_st) (void)hipHostFree(s->h_st);
   if (s->h_flags) (void)hipHostFree(s->h_flags);
@@ -1809,7 +1865,11 @@ OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per
   const PanelMatrix* K64 = panels_K<double>(s);
   // CG iteration: A p, K p, update (d, p, r, Kp, dinv read; d, r, y written),
   // p update (p, y read; p written)
-  const double cg = (m ? av(A32, vb) : 0.0) + kv(K32, vb) + 8.0 * n * vb + 3.0 * n * vb;
+  double cg = (m ? av(A32, vb) : 0.0) + kv(K32, vb) + 8.0 * n * vb + 3.0 * n * vb;
+  // incremental z on kZRefresh - 1 of every kZRefresh solves: A p written by
+  // the combine, read by the update with u (u read and written)
+  const double f_incr = s->z_incr ? (kZRefresh - 1.0) / kZRefresh : 0.0;
+  cg += f_incr * m * (4.0 + 4.0 + 16.0);
   // per solve: w64 from the cached A x (ax, b2, rho read; w written), FP64
   // residual, CG init (r64, dinv read; r, y, p, d written), correction (d,
   // x64 read; x64, b1 written), z = A x64 (A panels; b2 and ax written: the
@@ -1823,8 +1883,11 @@ OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per
     if (P) return dc_panel_bytes(s, *P) + (double)P->np * n * 8.0 + 3.0 * n * 8.0;
     return kv(P, 8.0);
   };
+  // z: exactly (av64) on the refresh solves, else u zeroed at the CG init
+  // and z = ax + scale u (ax, u read; ax, b2 written)
+  const double z = m ? (1.0 - f_incr) * av64(A64) + f_incr * m * (8.0 + 32.0) : 0.0;
   const double outer = 4.0 * m * 8.0 + kv64(K64) + n * (8.0 + vb) + 4.0 * n * vb + n * (vb + 8.0) +
-                       2.0 * n * 8.0 + (m ? av64(A64) : 0.0);
+                       2.0 * n * 8.0 + z;
   if (per_cg_iter) *per_cg_iter = cg;
   if (per_solve) *per_solve = outer;
   return 0;
