@@ -87,7 +87,7 @@ __device__ void run_entry(const Team& t, BProb& P, const BEntry& E, int iters, i
 // spilling it at 128 (a 1024-thread bound); on the MPC stream that cut the GPU
 // time of the same work by 28 %.
 template <typename V>
-__global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters) {
+__global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters, int lds_bytes) {
   __shared__ double red[kMaxRedBlocks];
   __shared__ double red2[kMaxRedBlocks];
   extern __shared__ double dyn_lds[];
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters) {
   t.red2 = red2;
   t.flag = &decision;
   t.lds = (char*)dyn_lds;
-  t.lds_bytes = kTeamLdsBytes;
+  t.lds_bytes = lds_bytes;
   const int ep = tab->epoch;
   const int count = tab->count;
   const int iters = tab->iters;
@@ -633,6 +633,9 @@ struct Bucket {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int g_T = 0, g_tpc = 0;
+  hipStream_t stream = nullptr;  // buckets run side by side, each on its own stream
+  bool inflight = false;         // an epoch of this bucket is on the GPU
+  std::vector<int> incoming;     // admitted or re-solved while it was in flight
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // tuner
   int trial = 0;  // next candidate to try; kNumCands when tuned
@@ -648,7 +651,9 @@ struct Engine {
   int bits = 32;
   int max_active = 4096;
   int cus = 256;
-  hipStream_t stream = nullptr;
+  size_t lds_per_cu = 65536;
+  int lds_T[2] = {0, 0};  // team size and dynamic LDS bytes of the last lookup
+  int lds_B[2] = {0, 0};
   char* arena = nullptr;
   size_t arena_size = 0;
   RangeAlloc arena_alloc;
@@ -664,9 +669,9 @@ struct Engine {
   long long epochs = 0, admm_iters = 0;
   double gpu_s = 0.0;
   bool failed = false;
-  bool inflight = false;          // an epoch is running on the GPU
-  std::vector<int> resumed;       // resident problems re-solved since the last launch
-  bool launched[kBuckets] = {};   // the buckets it covers
+  std::vector<int> resumed;       // resident problems re-solved since the last step
+  hipEvent_t ref = nullptr;       // time origin of the busy-time account
+  double busy_until_ms = 0.0;     // end of the GPU busy intervals counted so far
 };
 
 namespace {
@@ -831,19 +836,51 @@ bool ensure_table(Engine* E, Bucket& B, size_t count) {
   return true;
 }
 
+// Dynamic LDS of a team of T threads: the CU's LDS shared by the teams of
+// that size its registers let it hold (static LDS of each taken off), at least
+// kTeamLdsBytes and at most what keeps that team count resident.  A problem
+// whose CG vectors fit keeps them there (same arithmetic either way: the
+// bits do not depend on it).  -1 on a HIP error.
+int team_lds_bytes(Engine* E, int T) {
+  const int w = E->bits == 32 ? 0 : 1;
+  if (E->lds_T[w] == T) return E->lds_B[w];
+  const void* fn = E->bits == 32 ? (const void*)k_epoch<float> : (const void*)k_epoch<double>;
+  hipFuncAttributes fa;
+  int per_cu = 0;
+  if (!ok(E, hipFuncGetAttributes(&fa, fn), "hipFuncGetAttributes") ||
+      !ok(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, T, kTeamLdsBytes), "occupancy"))
+    return -1;
+  int bytes = kTeamLdsBytes;
+  if (per_cu > 0) {
+    const long long share = (long long)(E->lds_per_cu / (size_t)per_cu) - (long long)fa.sharedSizeBytes - 1024;
+    const int cand = (int)(std::min<long long>(share, 1 << 20) / 1024 * 1024);
+    // a refused attribute leaves the default size (not an engine failure)
+    if (cand > bytes && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, cand) == hipSuccess) {
+      int per_cu2 = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, fn, T, cand) == hipSuccess && per_cu2 >= per_cu)
+        bytes = cand;
+    }
+  }
+  E->lds_T[w] = T;
+  E->lds_B[w] = bytes;
+  return bytes;
+}
+
 bool ensure_graph(Engine* E, Bucket& B, int T, int tpc) {
   if (B.exec && B.g_T == T && B.g_tpc == tpc) return true;
   if (B.exec) hipGraphExecDestroy(B.exec);
   if (B.graph) hipGraphDestroy(B.graph);
   B.exec = nullptr;
   B.graph = nullptr;
-  if (!ok(E, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
+  const int lds = team_lds_bytes(E, T);
+  if (lds < 0) return false;
+  if (!ok(E, hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
   const dim3 grid((unsigned)(tpc * E->cus)), block((unsigned)T);
   if (E->bits == 32)
-    hipLaunchKernelGGL(k_epoch<float>, grid, block, kTeamLdsBytes, E->stream, B.tab, B.counters);
+    hipLaunchKernelGGL(k_epoch<float>, grid, block, (unsigned)lds, B.stream, B.tab, B.counters, lds);
   else
-    hipLaunchKernelGGL(k_epoch<double>, grid, block, kTeamLdsBytes, E->stream, B.tab, B.counters);
-  if (!ok(E, hipStreamEndCapture(E->stream, &B.graph), "end capture")) return false;
+    hipLaunchKernelGGL(k_epoch<double>, grid, block, (unsigned)lds, B.stream, B.tab, B.counters, lds);
+  if (!ok(E, hipStreamEndCapture(B.stream, &B.graph), "end capture")) return false;
   if (!ok(E, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "graph instantiate")) return false;
   B.g_T = T;
   B.g_tpc = tpc;
@@ -993,8 +1030,10 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, s->device) == hipSuccess && prop.multiProcessorCount > 0)
     E->cus = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, s->device) == hipSuccess && prop.maxSharedMemoryPerMultiProcessor > 0)
+    E->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
   B.tl_ticks = tl_ticks;
-  bool good = ok(E, hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking), "stream");
+  bool good = true;
   E->arena_size = arena_bytes > 0 ? (size_t)arena_bytes : ((size_t)4 << 30);
   good = good && ok(E, hipMalloc((void**)&E->arena, E->arena_size), "hipMalloc arena");
   E->pool_size = std::max<size_t>(E->arena_size / 4, (size_t)64 << 20);
@@ -1002,10 +1041,17 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
   good = good && ok(E, hipHostMalloc((void**)&E->pool, E->pool_size, hipHostMallocDefault), "hipHostMalloc pool");
   for (int k = 0; k < kBuckets && good; ++k) {
     Bucket& Bk = E->buckets[k];
-    good = ok(E, hipMalloc((void**)&Bk.counters, sizeof(int) * kCounterRing), "hipMalloc counters") &&
+    good = ok(E, hipStreamCreateWithFlags(&Bk.stream, hipStreamNonBlocking), "stream") &&
+           ok(E, hipMalloc((void**)&Bk.counters, sizeof(int) * kCounterRing), "hipMalloc counters") &&
            ok(E, hipMemset(Bk.counters, 0, sizeof(int) * kCounterRing), "hipMemset counters") &&
            ok(E, hipEventCreate(&Bk.e0), "event") && ok(E, hipEventCreate(&Bk.e1), "event");
   }
+  if (!good) {
+    osqp_batch_destroy(b);
+    return OSQP_MEM_ALLOC_ERROR;
+  }
+  good = good && ok(E, hipEventCreate(&E->ref), "event") && ok(E, hipEventRecord(E->ref, E->buckets[0].stream), "event") &&
+         ok(E, hipEventSynchronize(E->ref), "event");
   if (!good) {
     osqp_batch_destroy(b);
     return OSQP_MEM_ALLOC_ERROR;
@@ -1184,61 +1230,93 @@ OSQPInt osqp_batch_resolve(OSQPBatch* b, OSQPInt id, const OSQPFloat* q_new) {
 namespace osqp_serve {
 namespace {
 
-// Ends the epoch in flight, if any: its results, the tuner, the counters.
-// Returns the number of solves that ended in it.
-int finish_epoch(Engine* E) {
-  if (!E->inflight) return 0;
-  E->inflight = false;
-  if (!ok(E, hipStreamSynchronize(E->stream), "epoch")) return -1;
+// Ends bucket k's epoch (finished on the GPU): its results, the tuner, the
+// counters.  Returns the number of solves that ended in it.
+int finish_bucket(Engine* E, int k) {
+  Bucket& B = E->buckets[k];
+  B.inflight = false;
+  float ms = 0.f, t0 = 0.f, t1 = 0.f;
+  hipEventElapsedTime(&ms, B.e0, B.e1);
+  // GPU busy time: the union of the buckets' epoch intervals (approximate
+  // where intervals are finished out of order)
+  if (hipEventElapsedTime(&t0, E->ref, B.e0) == hipSuccess && hipEventElapsedTime(&t1, E->ref, B.e1) == hipSuccess) {
+    const double a = std::max((double)t0, E->busy_until_ms);
+    if (t1 > a) E->gpu_s += (t1 - a) * 1e-3;
+    E->busy_until_ms = std::max(E->busy_until_ms, (double)t1);
+  }
   const double t_now = now_s();
   int done = 0;
-  for (int k = 0; k < kBuckets; ++k) {
-    if (!E->launched[k]) continue;
-    Bucket& B = E->buckets[k];
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, B.e0, B.e1);
-    long long its = 0;
-    std::vector<int> still;
-    still.reserve(B.active.size());
-    for (size_t i = 0; i < B.active.size(); ++i) {
-      HProb* p = E->probs[B.active[i]];
-      its += B.tab->e[i].iters_done;
-      p->iters_since_add += B.tab->e[i].iters_done;
-      // setup done: the stage is no longer read
-      if (p->cmd == 1) free_problem_memory(E, p, false, true, false);
-      if (p->cmd == 3 && p->has_upd) {
-        E->pool_alloc.release(p->upd_off);
-        p->has_upd = false;
-      }
-      p->cmd = 0;
-      const BResult* out = (const BResult*)(E->pool + p->res_off + p->L.rs_res);
-      if (out->done) {
-        p->state = kEnded;
-        p->latency = t_now - p->t_req;
-        E->n_active--;
-        E->last_done.push_back(p->id);
-        done++;
-      } else {
-        still.push_back(p->id);
-      }
+  long long its = 0;
+  std::vector<int> still;
+  still.reserve(B.active.size());
+  for (size_t i = 0; i < B.active.size(); ++i) {
+    HProb* p = E->probs[B.active[i]];
+    its += B.tab->e[i].iters_done;
+    p->iters_since_add += B.tab->e[i].iters_done;
+    // setup done: the stage is no longer read
+    if (p->cmd == 1) free_problem_memory(E, p, false, true, false);
+    if (p->cmd == 3 && p->has_upd) {
+      E->pool_alloc.release(p->upd_off);
+      p->has_upd = false;
     }
-    B.active.swap(still);
-    E->admm_iters += its;
-    E->gpu_s += ms * 1e-3;
-    // online tuner: iterations per second of each candidate on a full epoch
-    if (B.trial < kNumCands) {
-      B.rate[B.trial] = ms > 0 ? its / (double)ms : 0.0;
-      B.trial++;
-      if (B.trial == kNumCands) {
-        int best = 0;
-        for (int c = 1; c < kNumCands; ++c)
-          if (B.rate[c] > B.rate[best]) best = c;
-        B.best = best;
-      }
+    p->cmd = 0;
+    const BResult* out = (const BResult*)(E->pool + p->res_off + p->L.rs_res);
+    if (out->done) {
+      p->state = kEnded;
+      p->latency = t_now - p->t_req;
+      E->n_active--;
+      E->last_done.push_back(p->id);
+      done++;
+    } else {
+      still.push_back(p->id);
     }
   }
+  B.active.swap(still);
+  E->admm_iters += its;
   E->epochs++;
+  // online tuner: iterations per second of each candidate on a full epoch
+  if (B.trial < kNumCands) {
+    B.rate[B.trial] = ms > 0 ? its / (double)ms : 0.0;
+    B.trial++;
+    if (B.trial == kNumCands) {
+      int best = 0;
+      for (int c = 1; c < kNumCands; ++c)
+        if (B.rate[c] > B.rate[best]) best = c;
+      B.best = best;
+    }
+  }
   return done;
+}
+
+// Waits until at least one bucket in flight has ended (none in flight: no
+// wait) and finishes every bucket that has.  Returns the solves that ended,
+// or -1 on a HIP error.
+int finish_ready(Engine* E) {
+  for (;;) {
+    bool any = false, ready[kBuckets] = {};
+    int n_ready = 0;
+    for (int k = 0; k < kBuckets; ++k) {
+      Bucket& B = E->buckets[k];
+      if (!B.inflight) continue;
+      any = true;
+      const hipError_t q = hipEventQuery(B.e1);
+      if (q == hipSuccess) {
+        ready[k] = true;
+        n_ready++;
+      } else if (q != hipErrorNotReady) {
+        ok(E, q, "epoch");
+        return -1;
+      }
+    }
+    if (!any) return 0;
+    if (n_ready) {
+      int done = 0;
+      for (int k = 0; k < kBuckets; ++k)
+        if (ready[k]) done += finish_bucket(E, k);
+      return done;
+    }
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
 }
 
 }  // namespace
@@ -1246,11 +1324,15 @@ int finish_epoch(Engine* E) {
 
 extern "C" {
 
-// One step of the pipelined engine: queued problems are admitted (their
-// memory reserved and images written) while the epoch launched by the
-// previous step still runs, then that epoch is finished, then the next one is
-// launched and the call returns without waiting for it.  So the host work
-// between two calls (adds, polls, resolves, the caller's own) overlaps the GPU.
+// One step of the pipelined engine.  Each size bucket runs its epochs on its
+// own stream, independently of the others.  A step admits queued problems
+// (their memory reserved and images written) while epochs are in flight,
+// waits until at least one bucket's epoch has ended (none in flight: no
+// wait), finishes every bucket whose epoch has ended, launches the next epoch
+// of every bucket that is idle and has work, and returns without waiting for
+// them.  So the host work between two calls (adds, polls, resolves, the
+// caller's own) overlaps the GPU, and a bucket of small problems does not
+// wait for the epochs of a bucket of large ones.
 OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
   if (completed) *completed = 0;
   if (!b) return OSQP_MEM_ALLOC_ERROR;
@@ -1270,20 +1352,23 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
     }
   }
   E->queue.resize(kept);
-  const int done = finish_epoch(E);
+  const int done = finish_ready(E);
   if (done < 0) return OSQP_MEM_ALLOC_ERROR;
   for (int id : fresh) {
     HProb* p = E->probs[id];
     p->state = kActive;
-    E->buckets[p->bucket].active.push_back(id);
+    E->buckets[p->bucket].incoming.push_back(id);
     E->n_active++;
   }
-  for (int id : E->resumed) E->buckets[E->probs[id]->bucket].active.push_back(id);
+  for (int id : E->resumed) E->buckets[E->probs[id]->bucket].incoming.push_back(id);
   E->resumed.clear();
-  // launch every bucket with work
+  // launch every idle bucket with work (a bucket's table and active list are
+  // left alone while its epoch is in flight)
   for (int k = 0; k < kBuckets; ++k) {
-    E->launched[k] = false;
     Bucket& B = E->buckets[k];
+    if (B.inflight) continue;
+    B.active.insert(B.active.end(), B.incoming.begin(), B.incoming.end());
+    B.incoming.clear();
     if (B.active.empty()) continue;
     if (!ensure_table(E, B, B.active.size())) return OSQP_MEM_ALLOC_ERROR;
     B.tab->count = (int)B.active.size();
@@ -1303,11 +1388,10 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
     const int c = B.trial < kNumCands ? B.trial : B.best;
     const int T = B.fixT ? B.fixT : kCands[c].T, tpc = B.fixT ? B.fixTpc : kCands[c].tpc;
     if (!ensure_graph(E, B, T, tpc)) return OSQP_MEM_ALLOC_ERROR;
-    if (!ok(E, hipEventRecord(B.e0, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
-    if (!ok(E, hipGraphLaunch(B.exec, E->stream), "graph launch")) return OSQP_MEM_ALLOC_ERROR;
-    if (!ok(E, hipEventRecord(B.e1, E->stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
-    E->launched[k] = true;
-    E->inflight = true;
+    if (!ok(E, hipEventRecord(B.e0, B.stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipGraphLaunch(B.exec, B.stream), "graph launch")) return OSQP_MEM_ALLOC_ERROR;
+    if (!ok(E, hipEventRecord(B.e1, B.stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
+    B.inflight = true;
   }
   if (completed) *completed = done;
   return 0;
@@ -1409,7 +1493,8 @@ OSQPInt osqp_batch_counters(const OSQPBatch* b, long long* epochs, long long* it
 OSQPInt osqp_batch_destroy(OSQPBatch* b) {
   if (!b) return 0;
   Engine* E = &b->e;
-  if (E->stream) hipStreamSynchronize(E->stream);
+  for (int k = 0; k < kBuckets; ++k)
+    if (E->buckets[k].stream) hipStreamSynchronize(E->buckets[k].stream);
   for (auto& kv : E->probs) delete kv.second;
   E->probs.clear();
   for (int k = 0; k < kBuckets; ++k) {
@@ -1420,10 +1505,11 @@ OSQPInt osqp_batch_destroy(OSQPBatch* b) {
     if (B.counters) hipFree(B.counters);
     if (B.e0) hipEventDestroy(B.e0);
     if (B.e1) hipEventDestroy(B.e1);
+    if (B.stream) hipStreamDestroy(B.stream);
   }
+  if (E->ref) hipEventDestroy(E->ref);
   if (E->arena) hipFree(E->arena);
   if (E->pool) hipHostFree(E->pool);
-  if (E->stream) hipStreamDestroy(E->stream);
   delete b;
   osqp_algebra_free_libs();
   return 0;

@@ -37,6 +37,7 @@ constexpr int kMaxRedBlocks = 512; // vectors up to 131072 entries
 constexpr int kLongRow = 64;       // rows longer than this go to whole wavefronts
 constexpr int kGroupPer = 8;       // entries per lane of a row group
 constexpr int kGroupSlots = kWave * kGroupPer;  // entries per row group
+constexpr int kLongPer = 16;     // strides of a long row whose loads are issued together
 constexpr int kMaxTeamWaves = 8;   // teams of up to 512 threads
 constexpr int kTeamLdsBytes = 32768;  // dynamic LDS per team: CG vectors of problems with 6n + m small
 
@@ -175,6 +176,50 @@ struct BProb {
   BResult* out;
 };
 
+// pointers with their address space stated (global: the arena and host-
+// mapped staging; LDS: the workgroup's dynamic LDS), so that loads are
+// global_load / ds_read rather than flat (a flat load's wait also waits for
+// every LDS access issued before it, and the other way round)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+using lptr = __attribute__((address_space(3))) T*;
+
+// ---- DPP lane moves (GFX9 data-parallel primitives) --------------------------
+// v_mov_b32_dpp: each lane reads a register of another lane of its wavefront
+// in the same instruction, without an LDS round trip.  Lanes without a source
+// (or outside row_mask) keep `old`.  Controls: row_shr:d within rows of 16
+// lanes, wave_shr:1 / wave_shl:1 across the wavefront, row_bcast:15 (lane 15
+// of each row to the next row), row_bcast:31 (lane 31 to rows 2 and 3).
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138, kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp_move(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float dpp_move(float old, float v) {
+  return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_move(double old, double v) {
+  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const int lo = dpp_move<CTRL, ROWS>((int)(o & 0xffffffff), (int)(x & 0xffffffff));
+  const int hi = dpp_move<CTRL, ROWS>((int)(o >> 32), (int)(x >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// one step of the inclusive scan of the maps x -> b + a x (a in {0, 1}): the
+// lane's map after the one read from another lane (identity: a = 1, b = 0)
+template <int CTRL, int ROWS = 0xf, typename TX>
+__device__ __forceinline__ void scan_step(int& a, TX& b) {
+  const int ap = dpp_move<CTRL, ROWS>(1, a);
+  const TX bp = dpp_move<CTRL, ROWS>((TX)0, b);
+  if (a) b = b + bp;
+  a = a & ap;
+}
+
 // ---- team primitives -------------------------------------------------------
 struct Team {
   int tid, T, lane, wv, W;
@@ -305,49 +350,60 @@ __device__ __forceinline__ void team_dot_max(const Team& t, int n, F body, doubl
 
 // y-rows of a CSR product: for each row r, epi(r, sum_k elem(val_k, col_k)),
 // epi called exactly once per row by one lane.  Row groups go to wavefronts
-// (one group per wavefront at a time):
+// (one group per wavefront at a time, the next group's slots loaded while
+// this one is reduced):
 //   1. each lane loads its 8 slots, forms the products and adds them in order,
 //      row by row: a row wholly inside the lane is finished there; the lane's
 //      first row (head) and last row (tail) may continue across lanes;
 //   2. C_l, the part of lane l's head row from lanes before l, follows
 //      C_l = cont_l ? ts_{l-1} + (single_{l-1} ? C_{l-1} : 0) : 0 (ts: tail sum,
 //      single: the lane holds one row), an affine recurrence solved by a
-//      6-step shuffle scan of fixed shape;
+//      6-step DPP scan of fixed shape (rows of 16 lanes, then row broadcasts);
 //   3. the lane where a row ends adds C_l to its part and runs the epilogue.
 // Long rows take one wavefront each; empty rows get epi(r, 0).  The matrix is
 // taken by value so its pointers live in registers.  Caller syncs.
 template <typename TX, typename Elem, typename Epi>
 __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem, Epi epi) {
-  const int* __restrict__ grow = M.grow;
-  const int* __restrict__ gci = M.gci;
-  const double* __restrict__ gval = M.gval;
-  const double* __restrict__ val = M.val;
+  const gptr<const int> grow = (gptr<const int>)M.grow;
+  const gptr<const int> gci = (gptr<const int>)M.gci;
+  const gptr<const double> gval = (gptr<const double>)M.gval;
+  const gptr<const double> val = (gptr<const double>)M.val;
   const int lane = t.lane;
   static_assert(kGroupPer == 8, "a lane's slots are loaded as two int4 / four double2 vectors");
   typedef int i32x4 __attribute__((ext_vector_type(4)));
   typedef double f64x2 __attribute__((ext_vector_type(2)));
-  for (int g = t.wv; g < M.ng; g += t.W) {
+  struct Slots {
+    i32x4 r[2], c[2];
+    f64x2 v[4];
+  };
+  auto load = [&](int g, Slots& L) {
     const int base = g * kGroupSlots + lane * kGroupPer;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      L.r[h] = *(gptr<const i32x4>)(grow + base + 4 * h);
+      L.c[h] = *(gptr<const i32x4>)(gci + base + 4 * h);
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) L.v[h] = *(gptr<const f64x2>)(gval + base + 2 * h);
+  };
+  auto reduce = [&](const Slots& L) {
     int rr[kGroupPer], cc[kGroupPer];
     double vv[kGroupPer];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const i32x4 r4 = *(const i32x4*)(grow + base + 4 * h);
-      const i32x4 c4 = *(const i32x4*)(gci + base + 4 * h);
-      rr[4 * h + 0] = r4.x;
-      rr[4 * h + 1] = r4.y;
-      rr[4 * h + 2] = r4.z;
-      rr[4 * h + 3] = r4.w;
-      cc[4 * h + 0] = c4.x;
-      cc[4 * h + 1] = c4.y;
-      cc[4 * h + 2] = c4.z;
-      cc[4 * h + 3] = c4.w;
+      rr[4 * h + 0] = L.r[h].x;
+      rr[4 * h + 1] = L.r[h].y;
+      rr[4 * h + 2] = L.r[h].z;
+      rr[4 * h + 3] = L.r[h].w;
+      cc[4 * h + 0] = L.c[h].x;
+      cc[4 * h + 1] = L.c[h].y;
+      cc[4 * h + 2] = L.c[h].z;
+      cc[4 * h + 3] = L.c[h].w;
     }
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-      const f64x2 v2 = *(const f64x2*)(gval + base + 2 * h);
-      vv[2 * h + 0] = v2.x;
-      vv[2 * h + 1] = v2.y;
+      vv[2 * h + 0] = L.v[h].x;
+      vv[2 * h + 1] = L.v[h].y;
     }
     TX v[kGroupPer];
 #pragma unroll
@@ -372,22 +428,21 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
     }
     if (!multi) hs = ts;
     // lane l's head continues lane l-1's tail
-    const int prev_tr = __shfl_up(tr, 1, kWave);
-    const TX prev_ts = __shfl_up(ts, 1, kWave);
-    const int prev_multi = __shfl_up((int)multi, 1, kWave);
+    const int prev_tr = dpp_move<kDppWaveShr1>(-2, tr);
+    const TX prev_ts = dpp_move<kDppWaveShr1>((TX)0, ts);
+    const int prev_multi = dpp_move<kDppWaveShr1>(0, (int)multi);
     const bool cont = lane > 0 && hr >= 0 && hr == prev_tr;
     int a = (cont && !prev_multi) ? 1 : 0;
     TX b = cont ? prev_ts : (TX)0;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const TX bp = __shfl_up(b, d, kWave);
-      const int ap = __shfl_up(a, d, kWave);
-      if (lane >= d) {
-        if (a) b = b + bp;
-        a = a & ap;
-      }
-    }
-    const int next_cont = __shfl_down((int)cont, 1, kWave);
+    // inclusive scan: within rows of 16 lanes, then the end of rows 0 / 2
+    // into rows 1 / 3, then the end of row 1 into rows 2 and 3
+    scan_step<kDppRowShr1>(a, b);
+    scan_step<kDppRowShr2>(a, b);
+    scan_step<kDppRowShr4>(a, b);
+    scan_step<kDppRowShr8>(a, b);
+    scan_step<kDppBcast15, 0xa>(a, b);
+    scan_step<kDppBcast31, 0xc>(a, b);
+    const int next_cont = dpp_move<kDppWaveShl1>(0, (int)cont);
     const bool ends_here = lane == kWave - 1 || !next_cont;
     if (hr >= 0) {
       if (multi) {
@@ -397,32 +452,48 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
         epi(hr, b + ts);
       }
     }
+  };
+  if (t.wv < M.ng) {
+    Slots cur;
+    load(t.wv, cur);
+    for (int g = t.wv;;) {
+      const int gn = g + t.W;
+      Slots nxt;
+      if (gn < M.ng) load(gn, nxt);
+      reduce(cur);
+      if (gn >= M.ng) break;
+      cur = nxt;
+      g = gn;
+    }
   }
-  for (int j = t.tid; j < M.nempty; j += t.T) epi(M.empty[j], (TX)0);
+  const gptr<const int> empty = (gptr<const int>)M.empty;
+  for (int j = t.tid; j < M.nempty; j += t.T) epi(empty[j], (TX)0);
   // long rows: lane l adds entries s + l, s + l + 64, ... in order; the loads
-  // of 8 consecutive strides are issued before their products are added
-  const int* __restrict__ rp = M.rp;
-  const int* __restrict__ ci = M.ci;
+  // of kLongPer consecutive strides are issued together, the last round's
+  // past the row's end at the row's first entry (in bounds, not added)
+  const gptr<const int> rp = (gptr<const int>)M.rp;
+  const gptr<const int> ci = (gptr<const int>)M.ci;
+  const gptr<const int> longs = (gptr<const int>)M.longs;
   for (int j = t.wv; j < M.nlong; j += t.W) {
-    const int r = M.longs[j];
+    const int r = longs[j];
     const int s = rp[r], e = rp[r + 1];
     TX acc = 0;
-    int k = s + t.lane;
-    for (; k + (kGroupPer - 1) * kWave < e; k += kGroupPer * kWave) {
-      double vv[kGroupPer];
-      int cc[kGroupPer];
+    for (int k0 = s + lane; k0 < e; k0 += kLongPer * kWave) {
+      double vv[kLongPer];
+      int cc[kLongPer];
 #pragma unroll
-      for (int i = 0; i < kGroupPer; ++i) {
-        vv[i] = val[k + i * kWave];
-        cc[i] = ci[k + i * kWave];
+      for (int i = 0; i < kLongPer; ++i) {
+        const int k = k0 + i * kWave < e ? k0 + i * kWave : s;
+        vv[i] = val[k];
+        cc[i] = ci[k];
       }
-      TX pr[kGroupPer];
+      TX pr[kLongPer];
 #pragma unroll
-      for (int i = 0; i < kGroupPer; ++i) pr[i] = (TX)elem(vv[i], cc[i]);
+      for (int i = 0; i < kLongPer; ++i) pr[i] = (TX)elem(vv[i], cc[i]);
 #pragma unroll
-      for (int i = 0; i < kGroupPer; ++i) acc += pr[i];
+      for (int i = 0; i < kLongPer; ++i)
+        if (k0 + i * kWave < e) acc += pr[i];
     }
-    for (; k < e; k += kWave) acc += elem(val[k], ci[k]);
     acc = osqp_hip::wave_sum(acc);
     if (t.lane == 0) epi(r, acc);
   }
@@ -646,27 +717,15 @@ struct Solver {
     const int n = p.n, m = p.m;
     const double sg = p.set.sigma;
     const int cg_max = p.set.cg_max_iter;
-    // a small problem keeps its CG vectors in the workgroup's dynamic LDS
-    // (d, p, Kp, r, y, M^-1 over n; w over m); a larger one in its arena block
-    V* lv = (size_t)(6 * n + m) * sizeof(V) <= (size_t)t.lds_bytes ? (V*)t.lds : nullptr;
-    V* __restrict__ d = lv ? lv : (V*)p.cd;
-    V* __restrict__ pp = lv ? lv + n : (V*)p.cp;
-    V* __restrict__ Kp = lv ? lv + 2 * n : (V*)p.cKp;
-    V* __restrict__ r = lv ? lv + 3 * n : (V*)p.cr;
-    V* __restrict__ y = lv ? lv + 4 * n : (V*)p.cy;
-    V* __restrict__ w = lv ? lv + 6 * n : (V*)p.cw;
-    const V* __restrict__ dinv = (const V*)p.cdinv;
-    if (lv) {
-      V* dl = lv + 5 * n;
-      for (int i = t.tid; i < n; i += t.T) dl[i] = dinv[i];
-      dinv = dl;  // made visible by the barriers before its first use
-    }
-    const double* __restrict__ rho = p.rho;
-    double* __restrict__ x64 = p.x64;
-    double* __restrict__ r64 = p.r64;
-    double* __restrict__ rhs64 = p.rhs64;
-    double* __restrict__ w64 = p.w64;
-    double* __restrict__ ax64 = p.ax64;
+    // arena arrays as global pointers (global loads: their waits are not
+    // tied to the LDS accesses' counter)
+    const gptr<const double> rho = (gptr<const double>)p.rho;
+    const gptr<double> x64 = (gptr<double>)p.x64;
+    const gptr<double> r64 = (gptr<double>)p.r64;
+    const gptr<double> rhs64 = (gptr<double>)p.rhs64;
+    const gptr<double> w64 = (gptr<double>)p.w64;
+    const gptr<double> ax64 = (gptr<double>)p.ax64;
+    const gptr<double> gb1 = (gptr<double>)b1, gb2 = (gptr<double>)b2;
     const BCsr A = p.A, P = p.P, At = p.At;
     const bool guard = sizeof(V) == 4;
     // zero-iteration bookkeeping of the previous solve (cuda_pcg_interface.cu:266-268)
@@ -679,10 +738,10 @@ struct Solver {
     if (!cached) {
       // rhs64 = b1 + A'(rho o b2); ||rhs64||_inf
       team_rows<double>(
-          t, At, [=](double v, int c) { return v * (rho[c] * b2[c]); },
-          [=](int row, double s) { rhs64[row] = b1[row] + s; });
+          t, At, [=](double v, int c) { return v * (rho[c] * gb2[c]); },
+          [=](int row, double s) { rhs64[row] = gb1[row] + s; });
       t.sync();
-      const double rhs_norm = norm_inf(rhs64, n);
+      const double rhs_norm = norm_inf(p.rhs64, n);
       if (admm_iter == 1) {
         S.cg_red_factor = p.set.cg_tol_fraction;
         S.cg_eps_prev = (rhs_norm < kCgTolMin) ? 1.0 : rhs_norm * S.cg_red_factor;
@@ -696,96 +755,116 @@ struct Solver {
             t, A, [=](double v, int c) { return v * x64[c]; }, [=](int row, double s) { w64[row] = rho[row] * s; });
         t.sync();
       }
-      residual64(rhs64);
+      residual64(p.rhs64);
     } else {
       eps = next_tolerance();
-      for (int i = t.tid; i < m; i += t.T) w64[i] = rho[i] * (ax64[i] - b2[i]);
+      for (int i = t.tid; i < m; i += t.T) w64[i] = rho[i] * (ax64[i] - gb2[i]);
       t.sync();
       residual64(b1);
     }
-    const double rn = norm_inf(r64, n);
+    const double rn = norm_inf(p.r64, n);
     S.cg_solves++;
     const double scale = (rn > 0.0 && isfinite(rn)) ? rn : 1.0;
     const double eps_cmp = eps / scale;
     const double inv = 1.0 / scale;
-    // CG on K d = -r64 / scale (init fused with r'y and max|r|)
-    double rty, rnorm;
-    team_dot_max<V>(
-        t, n,
-        [=](int i, V& a, double& c) {
-          const V ri = (V)(r64[i] * inv);
-          const V yi = dinv[i] * ri;
-          r[i] = ri;
-          y[i] = yi;
-          pp[i] = -yi;
-          d[i] = (V)0;
-          a = yi * ri;
-          c = fabs((double)ri);
-        },
-        &rty, &rnorm);
-    int it = 0;
-    bool go = rnorm > eps_cmp && it < cg_max;
-    if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
-    while (go) {
-      if (m) {
-        team_rows<V>(
-            t, A, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { w[row] = (V)rho[row] * s; });
-        t.sync();
-      }
-      // Kp = sigma p + P p + A' w
-      team_rows<V>(
-          t, P, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { Kp[row] = s; });
-      t.sync();
-      if (m)
-        team_rows<V>(
-            t, At, [=](double v, int c) { return (V)v * w[c]; },
-            [=](int row, V s) { Kp[row] = (V)sg * pp[row] + Kp[row] + s; });
-      else
-        for (int i = t.tid; i < n; i += t.T) Kp[i] = (V)sg * pp[i] + Kp[i];
-      t.sync();
-      const double pkp = team_dot<V>(t, n, [=](int i) { return pp[i] * Kp[i]; });
-      if (guard && !(pkp > 0.0 && isfinite(pkp))) break;
-      const V alpha = (V)(rty / pkp);
-      double rty_new;
+    // CG on K d = -r64 / scale (init fused with r'y and max|r|), over the CG
+    // vectors d, p, Kp, r, y, M^-1 (n) and w (m) as LDS or global pointers
+    auto cg = [&](auto d, auto pp, auto Kp, auto r, auto y, auto w, auto dinv) -> int {
+      double rty, rnorm;
       team_dot_max<V>(
           t, n,
           [=](int i, V& a, double& c) {
-            d[i] += alpha * pp[i];
-            const V ri = r[i] + alpha * Kp[i];
+            const V ri = (V)(r64[i] * inv);
             const V yi = dinv[i] * ri;
             r[i] = ri;
             y[i] = yi;
+            pp[i] = -yi;
+            d[i] = (V)0;
             a = yi * ri;
             c = fabs((double)ri);
           },
-          &rty_new, &rnorm);
-      const double beta = rty_new / rty;
-      it++;
-      rty = rty_new;
-      go = rnorm > eps_cmp && it < cg_max;
+          &rty, &rnorm);
+      int it = 0;
+      bool go = rnorm > eps_cmp && it < cg_max;
       if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
-      if (go) {
-        const V vb = (V)beta;
-        for (int i = t.tid; i < n; i += t.T) pp[i] = vb * pp[i] - y[i];
+      while (go) {
+        if (m) {
+          team_rows<V>(
+              t, A, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { w[row] = (V)rho[row] * s; });
+          t.sync();
+        }
+        // Kp = sigma p + P p + A' w
+        team_rows<V>(
+            t, P, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { Kp[row] = s; });
         t.sync();
+        if (m)
+          team_rows<V>(
+              t, At, [=](double v, int c) { return (V)v * w[c]; },
+              [=](int row, V s) { Kp[row] = (V)sg * pp[row] + Kp[row] + s; });
+        else
+          for (int i = t.tid; i < n; i += t.T) Kp[i] = (V)sg * pp[i] + Kp[i];
+        t.sync();
+        const double pkp = team_dot<V>(t, n, [=](int i) { return pp[i] * Kp[i]; });
+        if (guard && !(pkp > 0.0 && isfinite(pkp))) break;
+        const V alpha = (V)(rty / pkp);
+        double rty_new;
+        team_dot_max<V>(
+            t, n,
+            [=](int i, V& a, double& c) {
+              d[i] += alpha * pp[i];
+              const V ri = r[i] + alpha * Kp[i];
+              const V yi = dinv[i] * ri;
+              r[i] = ri;
+              y[i] = yi;
+              a = yi * ri;
+              c = fabs((double)ri);
+            },
+            &rty_new, &rnorm);
+        const double beta = rty_new / rty;
+        it++;
+        rty = rty_new;
+        go = rnorm > eps_cmp && it < cg_max;
+        if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
+        if (go) {
+          const V vb = (V)beta;
+          for (int i = t.tid; i < n; i += t.T) pp[i] = vb * pp[i] - y[i];
+          t.sync();
+        }
       }
+      // x64 += scale d; b1 = x64
+      for (int i = t.tid; i < n; i += t.T) {
+        double xi = x64[i];
+        if (it > 0) xi += scale * (double)d[i];
+        x64[i] = xi;
+        gb1[i] = xi;
+      }
+      return it;
+    };
+    // a small problem keeps its CG vectors in the workgroup's dynamic LDS
+    // (d, p, Kp, r, y, M^-1 over n; w over m); a larger one in its arena block.
+    // Same arithmetic either way.
+    int it;
+    if ((size_t)(6 * n + m) * sizeof(V) <= (size_t)t.lds_bytes) {
+      const lptr<V> lv = (lptr<V>)(V*)t.lds;
+      const gptr<const V> gdinv = (gptr<const V>)(const V*)p.cdinv;
+      const lptr<V> dl = lv + 5 * n;
+      for (int i = t.tid; i < n; i += t.T) dl[i] = gdinv[i];
+      t.sync();
+      it = cg(lv, lv + n, lv + 2 * n, lv + 3 * n, lv + 4 * n, lv + 6 * n, (lptr<const V>)dl);
+    } else {
+      it = cg((gptr<V>)(V*)p.cd, (gptr<V>)(V*)p.cp, (gptr<V>)(V*)p.cKp, (gptr<V>)(V*)p.cr, (gptr<V>)(V*)p.cy,
+              (gptr<V>)(V*)p.cw, (gptr<const V>)(const V*)p.cdinv);
     }
     S.cg_iter = it;
     S.cg_total += it;
-    // x64 += scale d; b1 = x64; b2 = A x64 (the A x cache)
-    for (int i = t.tid; i < n; i += t.T) {
-      double xi = x64[i];
-      if (it > 0) xi += scale * (double)d[i];
-      x64[i] = xi;
-      b1[i] = xi;
-    }
     t.sync();
+    // b2 = A x64 (the A x cache)
     if (m) {
       team_rows<double>(
           t, A, [=](double v, int c) { return v * x64[c]; },
           [=](int row, double s) {
             ax64[row] = s;
-            b2[row] = s;
+            gb2[row] = s;
           });
       t.sync();
       S.ax_valid = 1;
