@@ -394,6 +394,10 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
             issued[pid[k]] = (j, time.perf_counter())
     t_log = time.perf_counter()
     capped = False
+    # steady state: until the first problem has no request left to take (from
+    # then on fewer than n_problems solves are in flight, and the stream
+    # drains: its end waits on the largest problems' last solves)
+    steady = None
     while issued:
         if time.perf_counter() - t_log > 5.0:
             log(f"[serve] portfolio: {len(lat)}/{n_req} requests done")
@@ -414,6 +418,8 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
                 jn = queue[k].pop(0)
                 eng.resolve(i, reqs[jn][1])
                 issued[i] = (jn, time.perf_counter())
+            elif steady is None:
+                steady = (len(lat), time.perf_counter() - t_start)
     elapsed = time.perf_counter() - t_start
     eng.close()
     n_done = len(lat)
@@ -452,6 +458,8 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
                        f"with warm-started re-solves, eps 1e-4, closed loop",
            "qps": qps, "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                                       "p99": float(np.percentile(lat, 99) * 1e3)},
+           "steady_state": {"qps": steady[0] / steady[1], "requests": steady[0], "seconds": steady[1],
+                            "note": f"while all {n_problems} problems had requests to take"} if steady else None,
            "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
            "vs_cpu_oracle_direct": vs_oracle,
            "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps, "plain_sample": k_plain}
