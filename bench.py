@@ -106,6 +106,51 @@ def _cpu_mpc_chunk(job):
     return out
 
 
+# the FP64 optimum of a request (objective error reference for both the GPU
+# engine and the oracle's eps-1e-4 answers): the oracle's direct path at
+# eps_abs = eps_rel = 1e-8; None where it does not reach that accuracy
+OPT_SETTINGS = dict(linsys_solver=1, eps_abs=1e-8, eps_rel=1e-8, max_iter=200000, verbose=0)
+
+
+def _cpu_mpc_optimum(job):
+    from oracle import OracleSolver
+    from osqp_amd import problems as pr
+    Ns, seeds = job
+    out = []
+    for N, sd in zip(Ns, seeds):
+        p = pr.mpc(int(N), seed=int(sd))
+        o = OracleSolver()
+        o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **OPT_SETTINGS)
+        o.solve()
+        out.append(float(o.info.obj_val) if o.info.status_val == 1 else None)
+    return out
+
+
+def _cpu_portfolio_optimum(job):
+    """A resident portfolio's requests in stream order at the optimum settings
+    (each warm-started from the previous optimum)."""
+    from oracle import OracleSolver
+    p, reqs = job
+    o = OracleSolver()
+    o.setup(p["P"], p["q"], p["A"], p["l"], p["u"], **OPT_SETTINGS)
+    o.solve()
+    out = []
+    for j, q in reqs:
+        o.update_data_vec(q=q)
+        o.solve()
+        out.append((j, float(o.info.obj_val) if o.info.status_val == 1 else None))
+    return out
+
+
+def obj_errors(objs, ref, keys):
+    """Relative objective errors |f - f*| / max(1, |f*|) over the keys where
+    both are present: max, p99, p50."""
+    e = [abs(objs[k] - ref[k]) / max(1.0, abs(ref[k])) for k in keys
+         if objs.get(k) is not None and ref[k] is not None]
+    return {"requests": len(e), "max": float(np.max(e)) if e else None, "p99": percentile(e, 99),
+            "p50": percentile(e, 50)}
+
+
 def _cpu_portfolio_problem(job):
     """One resident portfolio on the oracle's direct path: setup + first solve
     (untimed), then its requests in stream order, each update_data_vec(q) +
@@ -148,6 +193,15 @@ def cpu_legs(seed, mpc_warm, mpc_req, pf_problems, pf_req, large_qp):
                   "solve_ms": {"p50": percentile(svc, 50) * 1e3, "p99": percentile(svc, 99) * 1e3},
                   "single_core_qps": mpc_req / float(np.sum(svc))}
     log(f"[cpu] MPC: {mpc_req} requests on {W} processes in {wall:.1f}s ({mpc_req / wall:.0f} QPs/s)")
+    t0 = time.perf_counter()
+    with ctx.Pool(W) as pool:
+        res = pool.map(_cpu_mpc_optimum, chunks)
+    opt = [None] * mpc_req
+    for i, rr in enumerate(res):
+        for j, ob in enumerate(rr):
+            opt[i + j * W] = ob
+    out["mpc"]["opt"] = opt
+    log(f"[cpu] MPC optima (eps 1e-8) in {time.perf_counter() - t0:.1f}s")
     # portfolio stream: one problem's requests in order per job
     probs, reqs = portfolio_stream(seed, pf_problems, pf_req)
     jobs = [(probs[k], [(j, q) for j, (kk, q) in enumerate(reqs) if kk == k]) for k in range(pf_problems)]
@@ -166,6 +220,15 @@ def cpu_legs(seed, mpc_warm, mpc_req, pf_problems, pf_req, large_qp):
                         "solve_ms": {"p50": percentile(psv, 50) * 1e3, "p99": percentile(psv, 99) * 1e3},
                         "wall_s_with_setups": wall}
     log(f"[cpu] portfolio: {pf_req} re-solves, {busy:.1f}s of solves on {W} processes")
+    t0 = time.perf_counter()
+    with ctx.Pool(W) as pool:
+        res = pool.map(_cpu_portfolio_optimum, jobs)
+    popt = {}
+    for rr in res:
+        for j, ob in rr:
+            popt[j] = ob
+    out["portfolio"]["opt"] = [popt[j] for j in range(pf_req)]
+    log(f"[cpu] portfolio optima (eps 1e-8) in {time.perf_counter() - t0:.1f}s")
     # large QP: the oracle's FP64 PCG path over one termination-check interval
     if large_qp:
         n, m = large_qp
@@ -328,7 +391,10 @@ def serve_mpc(n_req, n_warm, seed, cpu):
         errs = [abs(objs[k] - cob[k]) / max(1.0, abs(cob[k])) for k in range(n_req) if cst[k] == 1 and status[k] == 1]
         vs_oracle = {"requests": n_req, "statuses_identical": int(sum(int(status[k] == cst[k]) for k in range(n_req))),
                      "obj_rel_err_vs_direct": {"max": float(np.max(errs)) if errs else None,
-                                               "p99": percentile(errs, 99), "p50": percentile(errs, 50)}}
+                                               "p99": percentile(errs, 99), "p50": percentile(errs, 50)},
+                     "obj_rel_err_vs_fp64_optimum": {
+                         "gpu_batched_fp32": obj_errors(objs, cpu["mpc"]["opt"], range(n_req)),
+                         "oracle_direct_eps_1e-4": obj_errors(dict(enumerate(cob)), cpu["mpc"]["opt"], range(n_req))}}
     log(f"[serve] batched FP32: {qps:.0f} QPs/s, p50 {pct(lat, 50):.1f} ms, p99 {pct(lat, 99):.1f} ms; "
         f"at {rate70:.0f}/s: p50 {pct(lat70, 50):.1f} ms, p99 {pct(lat70, 99):.1f} ms; "
         f"plain FP64: {plain_qps:.1f} QPs/s; vs oracle: {vs_oracle and vs_oracle['statuses_identical']}/{n_req}")
@@ -394,10 +460,11 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
             issued[pid[k]] = (j, time.perf_counter())
     t_log = time.perf_counter()
     capped = False
-    # steady state: until the first problem has no request left to take (from
-    # then on fewer than n_problems solves are in flight, and the stream
-    # drains: its end waits on the largest problems' last solves)
-    steady = None
+    # completion times: the stream's end waits on the largest problems' last
+    # solves (each problem takes its requests one after another), so the
+    # rate up to half and nine tenths of the requests is stated beside the
+    # whole stream's
+    t_done = []
     while issued:
         if time.perf_counter() - t_log > 5.0:
             log(f"[serve] portfolio: {len(lat)}/{n_req} requests done")
@@ -411,6 +478,7 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
             r = eng.poll(i)
             j, t_iss = issued.pop(i)
             lat.append(time.perf_counter() - t_iss)
+            t_done.append(time.perf_counter() - t_start)
             status[j] = r.info.status_val
             objs[j] = r.info.obj_val
             k = pid.index(i)
@@ -418,8 +486,6 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
                 jn = queue[k].pop(0)
                 eng.resolve(i, reqs[jn][1])
                 issued[i] = (jn, time.perf_counter())
-            elif steady is None:
-                steady = (len(lat), time.perf_counter() - t_start)
     elapsed = time.perf_counter() - t_start
     eng.close()
     n_done = len(lat)
@@ -451,15 +517,20 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
         errs = [abs(objs[j] - cob[j]) / max(1.0, abs(cob[j])) for j in done if cst[j] == 1 and status[j] == 1]
         vs_oracle = {"requests": len(done), "statuses_identical": int(sum(int(status[j] == cst[j]) for j in done)),
                      "obj_rel_err_vs_direct": {"max": float(np.max(errs)) if errs else None,
-                                               "p50": percentile(errs, 50)}}
+                                               "p50": percentile(errs, 50)},
+                     "obj_rel_err_vs_fp64_optimum": {
+                         "gpu_batched_fp32": obj_errors(objs, cpu["portfolio"]["opt"], done),
+                         "oracle_direct_eps_1e-4": obj_errors(dict(enumerate(cob)), cpu["portfolio"]["opt"], done)}}
     log(f"[serve] portfolio batched FP32: {qps:.1f} QPs/s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
         f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms; plain FP64: {plain_qps:.1f} QPs/s")
     out = {"workload": f"portfolio stream, {n_problems} resident problems n~U{{500..5000}}, {n_done} mu updates "
                        f"with warm-started re-solves, eps 1e-4, closed loop",
            "qps": qps, "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                                       "p99": float(np.percentile(lat, 99) * 1e3)},
-           "steady_state": {"qps": steady[0] / steady[1], "requests": steady[0], "seconds": steady[1],
-                            "note": f"while all {n_problems} problems had requests to take"} if steady else None,
+           "rate_to_fraction_done": {f"{int(f * 100)}%": {"requests": int(f * n_done),
+                                                          "seconds": t_done[int(f * n_done) - 1],
+                                                          "qps": int(f * n_done) / t_done[int(f * n_done) - 1]}
+                                     for f in (0.5, 0.9) if int(f * n_done) >= 1},
            "statuses_solved": int(sum(1 for v in status.values() if v == 1)),
            "vs_cpu_oracle_direct": vs_oracle,
            "plain_fp64_qps": plain_qps, "ratio_vs_plain_fp64": qps / plain_qps, "plain_sample": k_plain}
