@@ -60,6 +60,9 @@ def cpu_workers():
     return max(1, min(16, host_cpu()["affinity"] or 1))
 
 
+CHECK_INTERVAL = 25  # check_termination of the default settings (include/osqp_amd.h)
+
+
 def lib_sync():
     """Waits for the library's queued GPU work (its own stream)."""
     from osqp_amd import load_library
@@ -721,10 +724,15 @@ def main():
     # ---- whole ADMM iteration: algorithmic bytes / measured time ----------------
     # PCG bytes from the library (same formulas as the kernels' accounting);
     # host-core vector passes per ADMM iteration: compute_rhs 3n + 5m,
-    # update_x 6n, update_z 11m, update_y 10m element touches of 8 bytes
+    # update_x 6n, update_z 11m, update_y 10m element touches of 8 bytes;
+    # the termination check (A x, P x, A' y and the vector passes of
+    # update_info, from the library) once per check_termination = 25
+    # iterations, as the solve runs it
     pb = eng.pcg_bytes()
+    check_b = eng.check_bytes()
     cg_per_admm = cg_iters / max(admm_iters, 1)
-    admm_bytes = pb["per_solve"] + cg_per_admm * pb["per_cg_iter"] + 8.0 * (9 * n + 26 * m)
+    admm_bytes = (pb["per_solve"] + cg_per_admm * pb["per_cg_iter"] + 8.0 * (9 * n + 26 * m) +
+                  check_b / CHECK_INTERVAL)
     admm_gbs = admm_bytes / (ms_per_admm * 1e-3) / 1e9
 
     # ---- dominant kernels: CG operator, timed with HIP events ------------------
@@ -815,7 +823,8 @@ def main():
         "kkt_certificate": {"prim": cert32["prim"], "dual": cert32["dual"], "obj": cert32["obj"]},
         "admm_iter": {"bytes": admm_bytes, "ms": ms_per_admm, "achieved_GBs": admm_gbs,
                       "frac_of_peak": admm_gbs / HBM_PEAK_GBS, "bytes_per_cg_iter": pb["per_cg_iter"],
-                      "bytes_per_linsys_outer": pb["per_solve"]},
+                      "bytes_per_linsys_outer": pb["per_solve"], "bytes_per_termination_check": check_b,
+                      "check_interval": CHECK_INTERVAL},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
                      "ms_per_launch": dom_ms, "bytes_per_launch": dom_bytes,

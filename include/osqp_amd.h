@@ -302,6 +302,11 @@ OSQPInt osqp_amd_time_fp64_kernels(OSQPSolver* solver, OSQPInt reps, double* ms_
  * path (see osqp_amd_linsys_bytes in osqp_amd_algebra.h). */
 OSQPInt osqp_amd_pcg_bytes(OSQPSolver* solver, double* per_cg_iter, double* per_solve);
 
+/* Algorithmic HBM bytes of one termination check (update_info: the products
+ * A x, P x, A' y and the vector passes of the residuals, objective terms and
+ * tolerance norms).  No counterpart in the reference: a measurement helper. */
+OSQPInt osqp_amd_check_bytes(OSQPSolver* solver, double* bytes);
+
 /* Number of roctx ranges pushed and popped and of events marked by the
  * profiler annotations (settings->profiler_level 1 or 2; the sections of
  * src/profilers.c:4-28).  For tests of the annotation backend. */

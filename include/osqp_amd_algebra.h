@@ -179,6 +179,11 @@ OSQPInt    osqp_amd_linsys_apply(LinSysSolver* s, OSQPInt which, const OSQPFloat
  * With the incremental z of the FP32 engine, both are averages over its
  * refresh cycle (one exact z = A x64 product in eight solves). */
 OSQPInt    osqp_amd_linsys_bytes(LinSysSolver* s, double* per_cg_iter, double* per_solve);
+/* Algorithmic bytes of one product y = M x (transpose 0) or y = M' x
+ * (transpose 1) on the kernel path the matrix takes now (FP64 panels or
+ * row-block CSR), y written, not read.  No counterpart in the reference: a
+ * measurement helper. */
+double     osqp_amd_matrix_product_bytes(const OSQPMatrix* M, OSQPInt transpose);
 /* Fused ADMM steps for the rho-vector case (no counterpart in the reference,
  * which issues the same arithmetic as separate vector calls; results are
  * bit-identical): compute_rhs (src/auxil.c:136-158) and update_x / update_z /

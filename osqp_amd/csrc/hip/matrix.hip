@@ -538,6 +538,22 @@ void OSQPMatrix_Atxpy(const OSQPMatrix* A, const OSQPVectorf* x, OSQPVectorf* y,
   }
 }
 
+double osqp_amd_matrix_product_bytes(const OSQPMatrix* M, OSQPInt transpose) {
+  if (!M) return 0.0;
+  const int t = (transpose && !M->is_triu) ? 1 : 0;
+  const double rows = M->is_triu ? M->n : (t ? M->n : M->m), cols = M->is_triu ? M->n : (t ? M->m : M->n);
+  // the panel path of panel_axpy: the panel SpMV, its partials read once by
+  // the combine, y written
+  if (!M->is_triu && M->nnz >= kPanelAxpyMinNnz && M->pan_state[t] == 1 && M->pan64[t] &&
+      M->pan64[t]->ntask > 0) {
+    const PanelMatrix& P = *M->pan64[t];
+    return panel_spmv_bytes(P, 8) + (double)P.np * P.R * 8.0 + rows * 8.0;
+  }
+  // row-block CSR: values, 32-bit indices, row pointers, x once, y written
+  const double nnz = t ? (double)M->nnz_in : (double)M->nnz;
+  return nnz * 12.0 + (rows + 1.0) * 4.0 + cols * 8.0 + rows * 8.0;
+}
+
 void OSQPMatrix_col_norm_inf(const OSQPMatrix* M, OSQPVectorf* E) {
   if (M->is_triu) launch_rows<RowMaxK>(M->lanes, M->n, M->n, (const int*)M->rp, (const int*)M->ci, (const double*)M->val, E->d, 1);
   else launch_rows<RowMaxK>(M->tlanes, M->n, M->n, (const int*)M->trp, (const int*)M->tci, (const double*)M->tval, E->d, 0);

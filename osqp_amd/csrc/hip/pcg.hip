@@ -514,8 +514,7 @@ __global__ void k_tol_w(PcgState* st, int admm_iter, double spr, double sdr, dou
 template <typename V>
 __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int n, const double* r64, const V* dinv,
                              V* r, V* y, V* p, V* d, double* part_rty, double* part_rn, int settol, int admm_iter,
-                             double spr, double sdr, double tol_fraction, int reduction_threshold, int m,
-                             double* u) {
+                             double spr, double sdr, double tol_fraction, int reduction_threshold) {
   __shared__ double lds2[2 * kBlock / kWave];
   double* lds = lds2;
   const double rn0 = parts_max(part_res, nres, lds);
@@ -540,27 +539,11 @@ __global__ void k_dc_init_cg(PcgState* st, const double* part_res, int nres, int
     double a = fabs((double)ri);
     mx = a > mx ? a : mx;
   }
-  if (u) GRID_STRIDE(i, m) u[i] = 0.0;
   double t1 = dot.value(), t2 = mx;
   block_sum_max(t1, t2, lds2);
   if (threadIdx.x == 0) {
     part_rty[blockIdx.x] = t1;
     part_rn[blockIdx.x] = t2;
-  }
-}
-
-// The incremental z = A x64 (FP32 mode): z = ax + scale u, u = A d summed
-// over the CG iterations from their own products (u_i += alpha (A p)_i);
-// ax = z; b[n:n+m] = z.  With when: only once the loop has stopped.
-__global__ void k_z_incr(const PcgState* st, const int* when, int m, int n, const double* u, double* ax, double* b) {
-  if (when && !*when) return;
-  const double s = st->scale;
-  const int ran = st->iter > 0;
-  GRID_STRIDE(i, m) {
-    double z = ax[i];
-    if (ran) z += s * u[i];
-    ax[i] = z;
-    b[n + i] = z;
   }
 }
 
@@ -575,6 +558,25 @@ __global__ void k_dc_apply(const PcgState* st, int when_done, int n, const V* d,
     if (ran) xi += s * (double)d[i];
     x64[i] = xi;
     b[i] = xi;
+  }
+}
+
+// The same, with the incremental z = A x64 of the FP32 engine in one launch:
+// rows i < n as k_dc_apply; rows n + i: z = ax + scale u (u = A d summed over
+// the CG iterations from their own A p products), ax = z, b[n + i] = z.
+__global__ void k_dc_apply_incr(const PcgState* st, int when_done, int n, int m, const float* d, const float* u,
+                                double* x64, double* ax, double* b) {
+  if (when_done && !st->done) return;
+  const double s = st->scale;
+  const int ran = st->iter > 0;
+  GRID_STRIDE(k, (long long)n + m) {
+    const bool xrow = k < n;
+    const long long i = xrow ? k : k - n;
+    double* t = xrow ? x64 : ax;
+    double v = t[i];
+    if (ran) v += s * (double)(xrow ? d[i] : u[i]);
+    t[i] = v;
+    b[k] = v;
   }
 }
 
@@ -605,7 +607,7 @@ template <typename V>
 __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, int guard, int n, V* x, V* r, V* y,
                          const V* p, const V* Kp, const V* dinv, double* part_rty, double* part_rn,
                          const double* init_rty, const double* init_rn, int ninit, int max_iter, int* flags, int m,
-                         const V* ap, double* u) {
+                         const V* ap, float* u) {
   if (!st->active) return;
   __shared__ double lds2[2 * kBlock / kWave];
   double* lds = lds2;
@@ -675,9 +677,14 @@ __global__ void k_update(PcgState* st, const double* part_pkp, int npkp, int j, 
     double a = fabs((double)ri);
     mx = a > mx ? a : mx;
   }
-  // u += alpha A p, the running A d (same rounded alpha as d's update)
-  if (u)
-    for (long long i = i0; i < m; i += stride) u[i] += (double)alpha * (double)ap[i];
+  // u += alpha A p, the running A d (same rounded alpha as d's update;
+  // iteration 0 assigns it)
+  if (u) {
+    if (j == 0)
+      for (long long i = i0; i < m; i += stride) u[i] = (float)alpha * (float)ap[i];
+    else
+      for (long long i = i0; i < m; i += stride) u[i] += (float)alpha * (float)ap[i];
+  }
   double t1 = dot.value(), t2 = mx;
   block_sum_max(t1, t2, lds2);
   if (threadIdx.x == 0) {
@@ -833,10 +840,10 @@ struct HipPcg {
   // defect correction (with FP32 copies of their values in FP32 mode)
   HybridOp hA32, hK32, hA64, hK64;
   // FP32 mode with panelled or hybrid A: z = A x64 from the CG's own products
-  // (u = A d accumulated as d is; k_z_incr) on all but every kZRefresh-th
+  // (u = A d accumulated as d is, in FP32; k_dc_apply_incr) on all but every kZRefresh-th
   // solve, which recomputes it exactly from x64.  ap: the last A p (FP32).
   float* ap;
-  double* u;
+  float* u;
   int z_incr, z_count, z_use;
 };
 
@@ -1031,7 +1038,7 @@ int build_panels(HipPcg* s) {
   s->dcA_ver = s->dcK_ver = -1;
   if (s->bits == 32 && m > 0 && (panels_A<float>(s) || hyb_A<float>(s))) {
     s->ap = (float*)dev_alloc(sizeof(float) * (size_t)m);
-    s->u = (double*)dev_alloc(sizeof(double) * (size_t)m);
+    s->u = (float*)dev_alloc(sizeof(float) * (size_t)m);
     if (!s->ap || !s->u) return 1;
     s->z_incr = 1;
   }
@@ -1393,7 +1400,7 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   }
   hipLaunchKernelGGL(k_dc_init_cg<V>, dim3(gE), dim3(kBlock), 0, S(), s->st, (const double*)part2, nres, n,
                      (const double*)s->r64, dinv, r, y, p, d, part3, part4, w_given ? 1 : 0, admm_iter, spr, sdr,
-                     s->tol_fraction, s->reduction_threshold, m, s->z_use ? s->u : nullptr);
+                     s->tol_fraction, s->reduction_threshold);
   // one CG iteration: the operator (two SpMVs with their combines), the
   // update, and the next loop head fused with the search-direction update.
   // With lookahead 0 the first launch of iteration it + 1 (A's panel SpMV) is
@@ -1413,13 +1420,15 @@ void solve_impl(HipPcg* s, OSQPVectorf* b, int admm_iter) {
   const bool spec = ahead && s->h_flags && (panels_A<double>(s) != nullptr || hyb_A<double>(s) != nullptr);
   const int* done = (const int*)((const char*)s->st + offsetof(PcgState, done));
   auto tail = [&](bool guarded) {
+    if (m && s->z_use) {
+      hipLaunchKernelGGL(k_dc_apply_incr, dim3(grid_for((long long)n + m, kBlock, 8192)), dim3(kBlock), 0, S(),
+                         (const PcgState*)s->st, guarded ? 1 : 0, n, m, (const float*)d, (const float*)s->u, s->x64,
+                         s->ax64, b->d);
+      return;
+    }
     hipLaunchKernelGGL(k_dc_apply<V>, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st,
                        guarded ? 1 : 0, n, (const V*)d, s->x64, b->d);
-    if (m && s->z_use)
-      hipLaunchKernelGGL(k_z_incr, dim3(grid_for(m, kBlock, 4096)), dim3(kBlock), 0, S(), (const PcgState*)s->st,
-                         guarded ? done : nullptr, m, n, (const double*)s->u, s->ax64, b->d);
-    else if (m)
-      op_finish_z(s, b->d, guarded ? done : nullptr);
+    if (m) op_finish_z(s, b->d, guarded ? done : nullptr);
   };
   int spec_at = -1, stop_at = s->max_iter;
   if (ahead) av_spmv<V>(s, (const V*)p, w);
@@ -1867,9 +1876,9 @@ OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per
   // p update (p, y read; p written)
   double cg = (m ? av(A32, vb) : 0.0) + kv(K32, vb) + 8.0 * n * vb + 3.0 * n * vb;
   // incremental z on kZRefresh - 1 of every kZRefresh solves: A p written by
-  // the combine, read by the update with u (u read and written)
+  // the combine, read by the update with u (FP32; read and written)
   const double f_incr = s->z_incr ? (kZRefresh - 1.0) / kZRefresh : 0.0;
-  cg += f_incr * m * (4.0 + 4.0 + 16.0);
+  cg += f_incr * m * (4.0 + 4.0 + 8.0);
   // per solve: w64 from the cached A x (ax, b2, rho read; w written), FP64
   // residual, CG init (r64, dinv read; r, y, p, d written), correction (d,
   // x64 read; x64, b1 written), z = A x64 (A panels; b2 and ax written: the
@@ -1883,9 +1892,9 @@ OSQPInt osqp_amd_linsys_bytes(LinSysSolver* ls, double* per_cg_iter, double* per
     if (P) return dc_panel_bytes(s, *P) + (double)P->np * n * 8.0 + 3.0 * n * 8.0;
     return kv(P, 8.0);
   };
-  // z: exactly (av64) on the refresh solves, else u zeroed at the CG init
-  // and z = ax + scale u (ax, u read; ax, b2 written)
-  const double z = m ? (1.0 - f_incr) * av64(A64) + f_incr * m * (8.0 + 32.0) : 0.0;
+  // z: exactly (av64) on the refresh solves, else z = ax + scale u in the
+  // tail (ax, u read; ax, b2 written)
+  const double z = m ? (1.0 - f_incr) * av64(A64) + f_incr * m * (8.0 + 4.0 + 16.0) : 0.0;
   const double outer = 4.0 * m * 8.0 + kv64(K64) + n * (8.0 + vb) + 4.0 * n * vb + n * (vb + 8.0) +
                        2.0 * n * 8.0 + z;
   if (per_cg_iter) *per_cg_iter = cg;

@@ -836,6 +836,22 @@ OSQPInt osqp_amd_pcg_bytes(OSQPSolver* solver, double* per_cg_iter, double* per_
   return osqp_amd_linsys_bytes(solver->work->linsys_solver, per_cg_iter, per_solve);
 }
 
+/* The vector passes of update_info (auxil.c), in 8-byte element reads and
+ * writes per element of n and of m: residuals, objective terms and the
+ * tolerance norms (the scaled norms of unscaled termination read two
+ * vectors); the products A x, P x and A' y are counted separately. */
+OSQPInt osqp_amd_check_bytes(OSQPSolver* solver, double* bytes) {
+  if (!solver || !solver->work || !solver->work->data || !bytes) return 1;
+  const OSQPWorkspace* work = solver->work;
+  const double n = work->data->n, m = work->data->m;
+  const int uns = solver->settings->scaling && !solver->settings->scaled_termination;
+  double b = osqp_amd_matrix_product_bytes(work->data->P, 0);
+  if (m > 0) b += osqp_amd_matrix_product_bytes(work->data->A, 0) + osqp_amd_matrix_product_bytes(work->data->A, 1);
+  b += 8.0 * (uns ? 21.0 * n + 22.0 * m : 16.0 * n + 18.0 * m);
+  *bytes = b;
+  return 0;
+}
+
 OSQPInt osqp_amd_solver_matrices(OSQPSolver* solver, const OSQPMatrix** P, const OSQPMatrix** A) {
   if (!solver || !solver->work || !solver->work->data) return 1;
   if (P) *P = solver->work->data->P;

@@ -70,6 +70,8 @@ def load_library():
         lib.osqp_amd_time_fp64_kernels.restype = C.c_int
         lib.osqp_amd_pcg_bytes.argtypes = [C.POINTER(T.OSQPSolver), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.osqp_amd_pcg_bytes.restype = C.c_int
+        lib.osqp_amd_check_bytes.argtypes = [C.POINTER(T.OSQPSolver), C.POINTER(C.c_double)]
+        lib.osqp_amd_check_bytes.restype = C.c_int
         lib.osqp_algebra_device_name.argtypes = [C.c_char_p, C.c_int]
         lib.osqp_amd_last_error.restype = C.c_int
         lib.osqp_amd_profiler_counts.argtypes = [C.POINTER(C.c_longlong)] * 3
@@ -270,6 +272,14 @@ class OSQP:
         if self.lib.osqp_amd_pcg_bytes(self._solver, C.byref(a), C.byref(b)):
             raise RuntimeError("osqp_amd_pcg_bytes failed")
         return dict(per_cg_iter=a.value, per_solve=b.value)
+
+    def check_bytes(self):
+        """Algorithmic HBM bytes of one termination check (the products A x,
+        P x, A' y and the vector passes of update_info)."""
+        b = C.c_double()
+        if self.lib.osqp_amd_check_bytes(self._solver, C.byref(b)):
+            raise RuntimeError("osqp_amd_check_bytes failed")
+        return b.value
 
     def scaling(self):
         """Ruiz scaling after setup: D, E, c and the scaled q, l, u (None
