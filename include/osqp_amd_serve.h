@@ -10,13 +10,16 @@
  *     as soon as a slot and device memory are free, then scaled, set up and
  *     solved on the GPU by one workgroup, with the host core's ADMM semantics
  *     and the PCG linear solver (FP32 CG with FP64 defect correction, or FP64);
- *   - osqp_batch_step runs one epoch: every active problem advances up to a
- *     fixed number of ADMM iterations in one persistent kernel per size bucket
- *     (one hipGraph each); a workgroup that finishes a problem takes the next
- *     one in the same launch; problems added meanwhile join the next epoch.
- *     Steps are pipelined: a step launches its epoch and returns, and the next
- *     step finishes it, so the caller's work between steps overlaps the GPU
- *     and a solve is reported by the step after the epoch it ended in;
+ *   - problems run in epochs: in one epoch every active problem of a size
+ *     bucket advances up to a fixed number of ADMM iterations in one
+ *     persistent kernel (one hipGraph per bucket, each bucket on its own
+ *     stream); a workgroup that finishes a problem takes the next one in the
+ *     same launch; problems added meanwhile join their bucket's next epoch.
+ *     Steps are pipelined: a step finishes the buckets whose epochs have
+ *     ended (waiting for at least one when any is in flight), launches the
+ *     next epoch of every idle bucket with work and returns, so the caller's
+ *     work between steps overlaps the GPU and a solve is reported by the step
+ *     that finishes the epoch it ended in;
  *   - a resident problem keeps its iterates and rho after a solve; a new q
  *     (osqp_update_data_vec semantics) and a warm-started re-solve are
  *     requested with osqp_batch_resolve.
@@ -119,9 +122,11 @@ OSQPInt osqp_batch_update_data_mat(OSQPBatch* batch, OSQPInt id, const OSQPFloat
                                    const OSQPInt* Px_new_idx, OSQPInt P_new_n, const OSQPFloat* Ax_new,
                                    const OSQPInt* Ax_new_idx, OSQPInt A_new_n);
 
-/* Admits queued problems, finishes the epoch launched by the previous step
- * and launches the next one without waiting for it.  *completed (may be
- * null) = solves that ended in the finished epoch.  Returns 0, or
+/* Admits queued problems, waits until at least one bucket's epoch in flight
+ * has ended (no wait when none is in flight), finishes every bucket whose
+ * epoch has ended, and launches the next epoch of every idle bucket with work
+ * without waiting for it.  *completed (may be null) = solves that ended in
+ * the finished epochs.  Returns 0, or
  * OSQP_MEM_ALLOC_ERROR when a HIP call failed.  Loop with
  * `while (osqp_batch_pending(b)) osqp_batch_step(b, &done);`. */
 OSQPInt osqp_batch_step(OSQPBatch* batch, OSQPInt* completed);
@@ -153,7 +158,8 @@ OSQPInt osqp_batch_tuning(const OSQPBatch* batch, OSQPInt bucket, OSQPInt* team,
  * Returns 0 or -1. */
 OSQPInt osqp_batch_set_team(OSQPBatch* batch, OSQPInt bucket, OSQPInt team, OSQPInt teams_per_cu);
 
-/* Engine counters: epochs run, ADMM iterations, GPU time of the epochs (s). */
+/* Engine counters: epochs run, ADMM iterations, GPU time of the epochs (s:
+ * the time during which at least one bucket's epoch ran). */
 OSQPInt osqp_batch_counters(const OSQPBatch* batch, long long* epochs, long long* admm_iters, OSQPFloat* gpu_s);
 
 OSQPInt osqp_batch_destroy(OSQPBatch* batch);
