@@ -243,15 +243,19 @@ class RangeAlloc {
 // next one (the rest is padding), so no row crosses a group.
 struct GroupDims {
   int ng = 0, nlong = 0, nempty = 0;
+  long long long_slots = 0;  // long-row entries of the compact copy (rows padded to kLongAlign)
+  int cols = 0;              // columns of the matrix
 };
 
-GroupDims count_groups(const int* rp, int rows) {
+GroupDims count_groups(const int* rp, int rows, int cols) {
   GroupDims d;
+  d.cols = cols;
   int fill = 0;
   for (int r = 0; r < rows; ++r) {
     const int len = rp[r + 1] - rp[r];
     if (len > kLongRow) {
       d.nlong++;
+      d.long_slots += (len + kLongAlign - 1) / kLongAlign * kLongAlign;
     } else if (len == 0) {
       d.nempty++;
     } else {
@@ -291,6 +295,9 @@ struct Layout {
   size_t Prp, Pci, Pval, Pdiag, Arp, Aci, Aval, Trp, Tci, Tval, q, l, u;
   struct Groups {
     size_t longs, empty, gk, grow, gci, gval;
+    // compact copy of the long rows for the FP32 CG (offsets 0 when absent)
+    bool compact;
+    size_t loff, lci16, lsrc, lv32;
   } gP, gA, gT;
   // dynamic
   size_t ctype, D, Dinv, E, Einv, rho, rhoinv, xa, xb, za, zb, y, xt, zt, dx, dy, Ax, Px, Aty, tn, tm, tn2;
@@ -331,6 +338,13 @@ Layout make_layout(const HostData& h, int vbytes) {
     o->gk = take(4 * slots);
     o->grow = take(4 * slots);
     o->gci = take(4 * slots);
+    // the compact long rows: FP32 engine, columns in 16 bits
+    o->compact = vbytes == 4 && g.cols < 65536 && g.nlong > 0;
+    if (o->compact) {
+      o->loff = take(4 * ((size_t)g.nlong + 1));
+      o->lci16 = take(2 * (size_t)g.long_slots);
+      o->lsrc = take(4 * (size_t)g.long_slots);
+    }
   };
   groups(&L.gP, h.gP);
   groups(&L.gA, h.gA);
@@ -343,6 +357,13 @@ Layout make_layout(const HostData& h, int vbytes) {
   L.gP.gval = take(8 * (size_t)h.gP.ng * kGroupSlots);
   L.gA.gval = take(8 * (size_t)h.gA.ng * kGroupSlots);
   L.gT.gval = take(8 * (size_t)h.gT.ng * kGroupSlots);
+  auto dyn32 = [&](Layout::Groups* o, const GroupDims& g) {
+    if (!o->compact) return;
+    o->lv32 = take(4 * (size_t)g.long_slots);
+  };
+  dyn32(&L.gP, h.gP);
+  dyn32(&L.gA, h.gA);
+  dyn32(&L.gT, h.gT);
   L.ctype = take(4 * m);
   L.D = take(8 * n);
   L.Dinv = take(8 * n);
@@ -423,10 +444,25 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
   std::fill(gk0, gk0 + slots, -1);
   std::fill(gr0, gr0 + slots, -1);
   std::fill(gc0, gc0 + slots, 0);
-  int nl = 0, ne = 0, grp = -1, fill = 0;
+  int* loff = o.compact ? (int*)(img + (o.loff - base)) : nullptr;
+  unsigned short* lci16 = o.compact ? (unsigned short*)(img + (o.lci16 - base)) : nullptr;
+  int* lsrc = o.compact ? (int*)(img + (o.lsrc - base)) : nullptr;
+  if (o.compact) {
+    std::fill(lci16, lci16 + d.long_slots, (unsigned short)0);
+    std::fill(lsrc, lsrc + d.long_slots, -1);
+  }
+  int nl = 0, ne = 0, grp = -1, fill = 0, lpos = 0;
   for (int r = 0; r < rows; ++r) {
     const int s = rp[r], e = rp[r + 1], len = e - s;
     if (len > kLongRow) {
+      if (o.compact) {
+        loff[nl] = lpos;
+        for (int k = s; k < e; ++k) {
+          lci16[lpos + (k - s)] = (unsigned short)ci[k];
+          lsrc[lpos + (k - s)] = k;
+        }
+        lpos += (len + kLongAlign - 1) / kLongAlign * kLongAlign;
+      }
       longs[nl++] = r;
       continue;
     }
@@ -448,6 +484,7 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
       gc[slot] = ci[k];
     }
   }
+  if (o.compact) loff[nl] = lpos;
 }
 
 // Builds a problem's stage image (values raw, unscaled): A (CSC, m x n) ->
@@ -482,9 +519,9 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
     }
   for (int r = 0; r < n; ++r) prp[(size_t)r + 1] += prp[r];
   h->nnzP = prp[n];
-  h->gP = count_groups(prp.data(), n);
-  h->gA = count_groups(arp.data(), m);
-  h->gT = count_groups(A->p, n);
+  h->gP = count_groups(prp.data(), n, n);
+  h->gA = count_groups(arp.data(), m, n);
+  h->gT = count_groups(A->p, n, m);
   h->warm_x = x0 != nullptr;
   h->warm_y = y0 != nullptr;
   const Layout L = make_layout(*h, vbytes);
@@ -747,6 +784,10 @@ bool admit(Engine* E, HProb* p) {
     M->grow = (int*)(dev + o.grow);
     M->gci = (int*)(dev + o.gci);
     M->gval = (double*)(dev + o.gval);
+    M->loff = o.compact ? (int*)(dev + o.loff) : nullptr;
+    M->lci16 = o.compact ? (unsigned short*)(dev + o.lci16) : nullptr;
+    M->lsrc = o.compact ? (int*)(dev + o.lsrc) : nullptr;
+    M->lv32 = o.compact ? (float*)(dev + o.lv32) : nullptr;
   };
   csr(&P->P, h.n, h.gP, L.gP, L.Prp, L.Pci, L.Pval);
   csr(&P->A, h.m, h.gA, L.gA, L.Arp, L.Aci, L.Aval);

@@ -38,6 +38,9 @@ constexpr int kLongRow = 64;       // rows longer than this go to whole wavefron
 constexpr int kGroupPer = 8;       // entries per lane of a row group
 constexpr int kGroupSlots = kWave * kGroupPer;  // entries per row group
 constexpr int kLongPer = 16;     // strides of a long row whose loads are issued together
+constexpr int kLongAlign = 8;    // compact long rows start at multiples of 8 entries
+constexpr int kLongVec = 4;      // compact long-row entries per lane per load (float4 + 4 x 16 bits)
+constexpr int kLongVecRounds = 8;  // compact loads per lane issued together
 constexpr int kMaxTeamWaves = 8;   // teams of up to 512 threads
 constexpr int kTeamLdsBytes = 32768;  // dynamic LDS per team: CG vectors of problems with 6n + m small
 
@@ -95,6 +98,13 @@ struct BCsr {
   int* grow;    // its row, -1 padding
   int* gci;     // its column (0 for padding)
   double* gval; // its value, copied from val after scaling (0 for padding)
+  // compact copy of the long rows for the FP32 CG (null when absent: FP64
+  // engine, or 65536 or more columns): 6 bytes an entry instead of 12; the
+  // values are (float) of val, so the products are the same
+  int* loff;             // nlong + 1: start of each long row in the compact long-row arrays
+  unsigned short* lci16; // long-row entries (rows start at multiples of kLongAlign, zero padding)
+  int* lsrc;             // their CSR positions, -1 for padding
+  float* lv32;           // their (float) values, 0 for padding
 };
 
 // scalar state that survives between epochs (stored with the problem)
@@ -362,118 +372,175 @@ __device__ __forceinline__ void team_dot_max(const Team& t, int n, F body, doubl
 //   3. the lane where a row ends adds C_l to its part and runs the epilogue.
 // Long rows take one wavefront each; empty rows get epi(r, 0).  The matrix is
 // taken by value so its pointers live in registers.  Caller syncs.
+//
+// With a compact copy of the long rows (FP32 CG, fewer than 65536 columns) a
+// long row's lane adds 4 consecutive entries per load ((float) values, 16-bit
+// columns) in order, chunk after chunk.
+
+// steps 1-3 for one group, from each slot's row (-1: padding) and product
+template <typename TX, typename Epi>
+__device__ __forceinline__ void group_rows(const int lane, Epi& epi, const int (&rr)[kGroupPer],
+                                           const TX (&v)[kGroupPer]) {
+  const int hr = rr[0];
+  int tr = rr[0];
+  TX hs = 0, ts = 0;
+  bool multi = false;
+#pragma unroll
+  for (int e = 0; e < kGroupPer; ++e) {
+    const int r = rr[e];
+    if (r >= 0) {
+      if (r != tr) {
+        if (multi) epi(tr, ts);  // a row wholly inside this lane
+        else hs = ts;            // the head row's part
+        multi = true;
+        tr = r;
+        ts = 0;
+      }
+      ts += v[e];
+    }
+  }
+  if (!multi) hs = ts;
+  // lane l's head continues lane l-1's tail
+  const int prev_tr = dpp_move<kDppWaveShr1>(-2, tr);
+  const TX prev_ts = dpp_move<kDppWaveShr1>((TX)0, ts);
+  const int prev_multi = dpp_move<kDppWaveShr1>(0, (int)multi);
+  const bool cont = lane > 0 && hr >= 0 && hr == prev_tr;
+  int a = (cont && !prev_multi) ? 1 : 0;
+  TX b = cont ? prev_ts : (TX)0;
+  // inclusive scan: within rows of 16 lanes, then the end of rows 0 / 2
+  // into rows 1 / 3, then the end of row 1 into rows 2 and 3
+  scan_step<kDppRowShr1>(a, b);
+  scan_step<kDppRowShr2>(a, b);
+  scan_step<kDppRowShr4>(a, b);
+  scan_step<kDppRowShr8>(a, b);
+  scan_step<kDppBcast15, 0xa>(a, b);
+  scan_step<kDppBcast31, 0xc>(a, b);
+  const int next_cont = dpp_move<kDppWaveShl1>(0, (int)cont);
+  const bool ends_here = lane == kWave - 1 || !next_cont;
+  if (hr >= 0) {
+    if (multi) {
+      epi(hr, b + hs);
+      if (ends_here) epi(tr, ts);
+    } else if (ends_here) {
+      epi(hr, b + ts);
+    }
+  }
+}
+
 template <typename TX, typename Elem, typename Epi>
 __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem, Epi epi) {
-  const gptr<const int> grow = (gptr<const int>)M.grow;
-  const gptr<const int> gci = (gptr<const int>)M.gci;
-  const gptr<const double> gval = (gptr<const double>)M.gval;
-  const gptr<const double> val = (gptr<const double>)M.val;
   const int lane = t.lane;
   static_assert(kGroupPer == 8, "a lane's slots are loaded as two int4 / four double2 vectors");
   typedef int i32x4 __attribute__((ext_vector_type(4)));
   typedef double f64x2 __attribute__((ext_vector_type(2)));
-  struct Slots {
-    i32x4 r[2], c[2];
-    f64x2 v[4];
-  };
-  auto load = [&](int g, Slots& L) {
-    const int base = g * kGroupSlots + lane * kGroupPer;
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  bool compact = false;
+  if constexpr (sizeof(TX) == 4) compact = M.loff != nullptr;
+  {
+    const gptr<const int> grow = (gptr<const int>)M.grow;
+    const gptr<const int> gci = (gptr<const int>)M.gci;
+    const gptr<const double> gval = (gptr<const double>)M.gval;
+    struct Slots {
+      i32x4 r[2], c[2];
+      f64x2 v[4];
+    };
+    auto load = [&](int g, Slots& L) {
+      const int base = g * kGroupSlots + lane * kGroupPer;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      L.r[h] = *(gptr<const i32x4>)(grow + base + 4 * h);
-      L.c[h] = *(gptr<const i32x4>)(gci + base + 4 * h);
-    }
-#pragma unroll
-    for (int h = 0; h < 4; ++h) L.v[h] = *(gptr<const f64x2>)(gval + base + 2 * h);
-  };
-  auto reduce = [&](const Slots& L) {
-    int rr[kGroupPer], cc[kGroupPer];
-    double vv[kGroupPer];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      rr[4 * h + 0] = L.r[h].x;
-      rr[4 * h + 1] = L.r[h].y;
-      rr[4 * h + 2] = L.r[h].z;
-      rr[4 * h + 3] = L.r[h].w;
-      cc[4 * h + 0] = L.c[h].x;
-      cc[4 * h + 1] = L.c[h].y;
-      cc[4 * h + 2] = L.c[h].z;
-      cc[4 * h + 3] = L.c[h].w;
-    }
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      vv[2 * h + 0] = L.v[h].x;
-      vv[2 * h + 1] = L.v[h].y;
-    }
-    TX v[kGroupPer];
-#pragma unroll
-    for (int e = 0; e < kGroupPer; ++e) v[e] = rr[e] >= 0 ? (TX)elem(vv[e], cc[e]) : (TX)0;
-    const int hr = rr[0];
-    int tr = rr[0];
-    TX hs = 0, ts = 0;
-    bool multi = false;
-#pragma unroll
-    for (int e = 0; e < kGroupPer; ++e) {
-      const int r = rr[e];
-      if (r >= 0) {
-        if (r != tr) {
-          if (multi) epi(tr, ts);  // a row wholly inside this lane
-          else hs = ts;            // the head row's part
-          multi = true;
-          tr = r;
-          ts = 0;
-        }
-        ts += v[e];
+      for (int h = 0; h < 2; ++h) {
+        L.r[h] = *(gptr<const i32x4>)(grow + base + 4 * h);
+        L.c[h] = *(gptr<const i32x4>)(gci + base + 4 * h);
       }
-    }
-    if (!multi) hs = ts;
-    // lane l's head continues lane l-1's tail
-    const int prev_tr = dpp_move<kDppWaveShr1>(-2, tr);
-    const TX prev_ts = dpp_move<kDppWaveShr1>((TX)0, ts);
-    const int prev_multi = dpp_move<kDppWaveShr1>(0, (int)multi);
-    const bool cont = lane > 0 && hr >= 0 && hr == prev_tr;
-    int a = (cont && !prev_multi) ? 1 : 0;
-    TX b = cont ? prev_ts : (TX)0;
-    // inclusive scan: within rows of 16 lanes, then the end of rows 0 / 2
-    // into rows 1 / 3, then the end of row 1 into rows 2 and 3
-    scan_step<kDppRowShr1>(a, b);
-    scan_step<kDppRowShr2>(a, b);
-    scan_step<kDppRowShr4>(a, b);
-    scan_step<kDppRowShr8>(a, b);
-    scan_step<kDppBcast15, 0xa>(a, b);
-    scan_step<kDppBcast31, 0xc>(a, b);
-    const int next_cont = dpp_move<kDppWaveShl1>(0, (int)cont);
-    const bool ends_here = lane == kWave - 1 || !next_cont;
-    if (hr >= 0) {
-      if (multi) {
-        epi(hr, b + hs);
-        if (ends_here) epi(tr, ts);
-      } else if (ends_here) {
-        epi(hr, b + ts);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) L.v[h] = *(gptr<const f64x2>)(gval + base + 2 * h);
+    };
+    auto reduce = [&](const Slots& L) {
+      int rr[kGroupPer], cc[kGroupPer];
+      double vv[kGroupPer];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        rr[4 * h + 0] = L.r[h].x;
+        rr[4 * h + 1] = L.r[h].y;
+        rr[4 * h + 2] = L.r[h].z;
+        rr[4 * h + 3] = L.r[h].w;
+        cc[4 * h + 0] = L.c[h].x;
+        cc[4 * h + 1] = L.c[h].y;
+        cc[4 * h + 2] = L.c[h].z;
+        cc[4 * h + 3] = L.c[h].w;
       }
-    }
-  };
-  if (t.wv < M.ng) {
-    Slots cur;
-    load(t.wv, cur);
-    for (int g = t.wv;;) {
-      const int gn = g + t.W;
-      Slots nxt;
-      if (gn < M.ng) load(gn, nxt);
-      reduce(cur);
-      if (gn >= M.ng) break;
-      cur = nxt;
-      g = gn;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        vv[2 * h + 0] = L.v[h].x;
+        vv[2 * h + 1] = L.v[h].y;
+      }
+      TX v[kGroupPer];
+#pragma unroll
+      for (int e = 0; e < kGroupPer; ++e) v[e] = rr[e] >= 0 ? (TX)elem(vv[e], cc[e]) : (TX)0;
+      group_rows<TX>(lane, epi, rr, v);
+    };
+    if (t.wv < M.ng) {
+      Slots cur;
+      load(t.wv, cur);
+      for (int g = t.wv;;) {
+        const int gn = g + t.W;
+        Slots nxt;
+        if (gn < M.ng) load(gn, nxt);
+        reduce(cur);
+        if (gn >= M.ng) break;
+        cur = nxt;
+        g = gn;
+      }
     }
   }
   const gptr<const int> empty = (gptr<const int>)M.empty;
   for (int j = t.tid; j < M.nempty; j += t.T) epi(empty[j], (TX)0);
+  const gptr<const int> rp = (gptr<const int>)M.rp;
+  const gptr<const int> longs = (gptr<const int>)M.longs;
+  if (compact) {
+    // compact long rows: lane l adds entries s + 4 (64 i + l) + q, q = 0..3,
+    // for i = 0, 1, ... in order (kLongVecRounds loads of each kind in
+    // flight; loads past the row's end read its first chunk, not added)
+    const gptr<const int> loff = (gptr<const int>)M.loff;
+    const gptr<const float> lv32 = (gptr<const float>)M.lv32;
+    const gptr<const unsigned short> lci16 = (gptr<const unsigned short>)M.lci16;
+    constexpr int kChunk = kWave * kLongVec;
+    for (int j = t.wv; j < M.nlong; j += t.W) {
+      const int r = longs[j];
+      const int s = loff[j], e = s + (rp[r + 1] - rp[r]);
+      TX acc = 0;
+      for (int k0 = s; k0 < e; k0 += kLongVecRounds * kChunk) {
+        f32x4 vv[kLongVecRounds];
+        u32x2 cw[kLongVecRounds];
+#pragma unroll
+        for (int i = 0; i < kLongVecRounds; ++i) {
+          const int k = k0 + i * kChunk + lane * kLongVec;
+          const int kk = k < e ? k : s;
+          vv[i] = *(gptr<const f32x4>)(lv32 + kk);
+          cw[i] = *(gptr<const u32x2>)(lci16 + kk);
+        }
+#pragma unroll
+        for (int i = 0; i < kLongVecRounds; ++i) {
+          const int k = k0 + i * kChunk + lane * kLongVec;
+#pragma unroll
+          for (int q = 0; q < kLongVec; ++q) {
+            const unsigned w = cw[i][q >> 1];
+            const int c = (int)((q & 1) ? (w >> 16) : (w & 0xFFFFu));
+            const TX pr = (TX)elem((double)vv[i][q], c);
+            if (k + q < e) acc += pr;
+          }
+        }
+      }
+      acc = osqp_hip::wave_sum(acc);
+      if (t.lane == 0) epi(r, acc);
+    }
+    return;
+  }
   // long rows: lane l adds entries s + l, s + l + 64, ... in order; the loads
   // of kLongPer consecutive strides are issued together, the last round's
   // past the row's end at the row's first entry (in bounds, not added)
-  const gptr<const int> rp = (gptr<const int>)M.rp;
+  const gptr<const double> val = (gptr<const double>)M.val;
   const gptr<const int> ci = (gptr<const int>)M.ci;
-  const gptr<const int> longs = (gptr<const int>)M.longs;
   for (int j = t.wv; j < M.nlong; j += t.W) {
     const int r = longs[j];
     const int s = rp[r], e = rp[r + 1];
@@ -687,6 +754,12 @@ struct Solver {
       double* __restrict__ gval = M.gval;
       const int slots = M.ng * kGroupSlots;
       for (int i = t.tid; i < slots; i += t.T) gval[i] = gk[i] >= 0 ? val[gk[i]] : 0.0;
+      if (M.lv32) {
+        const int* __restrict__ lsrc = M.lsrc;
+        float* __restrict__ lv32 = M.lv32;
+        const int ne = M.loff[M.nlong];
+        for (int i = t.tid; i < ne; i += t.T) lv32[i] = lsrc[i] >= 0 ? (float)val[lsrc[i]] : 0.f;
+      }
     }
     t.sync();
   }
