@@ -16,6 +16,8 @@ tests/oracle_refs.py), not against the same library's single-problem engine:
     each objective is within 1e-4 of the oracle's own accuracy on that
     request (the larger of OSQP's gap-test bound and the oracle PCG path's
     error against the FP64 optimum);
+  * a QP with rows of every length class the team kernel's products treat
+    apart (short, just over 64 entries, around multiples of 4, 8 and 2048);
   * OSQP's result contract (src/auxil.c:598-674): infeasible problems return
     x = y = OSQP_NAN and the normalised certificate, equal to the
     single-problem engine's; a non-convex problem returns obj = OSQP_NAN;
@@ -28,6 +30,7 @@ import os
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 from oracle import OracleSolver
 from osqp_amd import OSQP, OSQP_NAN
@@ -139,6 +142,49 @@ def test_portfolio_updates_vs_oracle(n, precision):
               f"x {vec_err(r.x, x_opt):.2e} y {vec_err(r.y, y_opt):.2e}")
         assert r.info.status_val == ref["direct"][0] == ref["pcg"][0] == 1
         assert e_gpu <= max(ref["bound"], e_pcg) + OBJ_TOL
+
+
+def long_row_qp(seed=11):
+    """A QP whose A has rows of every length class the team kernel treats
+    apart: short rows (row groups) and rows longer than 64 entries, just
+    above it and around multiples of 4, 8, 64 and 2048 (the FP32 engine's
+    compact long rows: 4-entry chunks per lane, rows padded to 8, rounds of
+    2048 entries per wavefront), and one of 1000."""
+    rng = np.random.default_rng(seed)
+    n = 3000
+    lens = [65, 66, 67, 69, 71, 72, 73, 127, 128, 129, 255, 256, 257, 1000, 2047, 2048, 2049]
+    lens += list(rng.integers(1, 21, 500))
+    rows, cols, vals = [], [], []
+    for i, L in enumerate(lens):
+        c = rng.choice(n, int(L), replace=False)
+        rows += [i] * int(L)
+        cols += list(c)
+        vals += list(rng.standard_normal(int(L)) / np.sqrt(L))
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(len(lens), n))
+    P = sp.triu(sp.diags(rng.random(n) + 0.1), format="csc")
+    m = A.shape[0]
+    return dict(P=P, q=rng.standard_normal(n), A=A, l=-0.5 - rng.random(m), u=0.5 + rng.random(m))
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_long_row_lengths_vs_oracle(precision):
+    """Every row-length class of the team's sparse products against the
+    oracle: statuses equal both oracle paths', the objective within 1e-4 of
+    the FP64 optimum's accuracy bound, x and y within the engines' bounds of
+    the oracle's PCG path."""
+    p = long_row_qp()
+    pc, di, op = oracle(p, 2), oracle(p, 1), optimum(p, **ST)
+    eng = BatchEngine(precision=precision, **ST)
+    _, (r,) = run_all(eng, [p])
+    eng.close()
+    e_obj = rel(r.info.obj_val, op["obj"])
+    gb = gap_bound(p, op["x"], op["y"], 1e-4, 1e-4)
+    e_x, e_y = vec_err(r.x, pc.x), vec_err(r.y, pc.y)
+    print(f"long rows batch{precision}: {r.info.status} it {r.info.iter} (oracle PCG {pc.info.status} "
+          f"it {pc.info.iter}) obj vs optimum {e_obj:.1e} (gap bound {gb:.1e}) x {e_x:.1e} y {e_y:.1e}")
+    assert r.info.status_val == pc.info.status_val == di.info.status_val == 1
+    assert e_obj <= max(gb, rel(pc.info.obj_val, op["obj"])) + OBJ_TOL
+    assert e_x <= XY_TOL[precision] and e_y <= XY_TOL[precision]
 
 
 @pytest.mark.parametrize("precision", [32, 64])
