@@ -377,29 +377,27 @@ __device__ __forceinline__ void team_dot_max(const Team& t, int n, F body, doubl
 // long row's lane adds 4 consecutive entries per load ((float) values, 16-bit
 // columns) in order, chunk after chunk.
 
-// steps 1-3 for one group, from each slot's row (-1: padding) and product
+// steps 1-3 for one group, from each slot's row (-1: padding; padding slots
+// are a lane's last ones) and product.  Branch-free up to the epilogues: the
+// running sum of each slot's row within the lane (the same additions in the
+// same order as a row-by-row loop), the lane's tail, the DPP scan, then one
+// epilogue per row end.
 template <typename TX, typename Epi>
 __device__ __forceinline__ void group_rows(const int lane, Epi& epi, const int (&rr)[kGroupPer],
                                            const TX (&v)[kGroupPer]) {
-  const int hr = rr[0];
-  int tr = rr[0];
-  TX hs = 0, ts = 0;
-  bool multi = false;
+  TX seg[kGroupPer];
+  seg[0] = v[0];
 #pragma unroll
-  for (int e = 0; e < kGroupPer; ++e) {
-    const int r = rr[e];
-    if (r >= 0) {
-      if (r != tr) {
-        if (multi) epi(tr, ts);  // a row wholly inside this lane
-        else hs = ts;            // the head row's part
-        multi = true;
-        tr = r;
-        ts = 0;
-      }
-      ts += v[e];
-    }
+  for (int e = 1; e < kGroupPer; ++e) seg[e] = rr[e] == rr[e - 1] ? seg[e - 1] + v[e] : v[e];
+  const int hr = rr[0];
+  int tr = hr;
+  TX ts = seg[0];
+#pragma unroll
+  for (int e = 1; e < kGroupPer; ++e) {
+    tr = rr[e] >= 0 ? rr[e] : tr;
+    ts = rr[e] >= 0 ? seg[e] : ts;
   }
-  if (!multi) hs = ts;
+  const bool multi = hr >= 0 && tr != hr;
   // lane l's head continues lane l-1's tail
   const int prev_tr = dpp_move<kDppWaveShr1>(-2, tr);
   const TX prev_ts = dpp_move<kDppWaveShr1>((TX)0, ts);
@@ -417,13 +415,15 @@ __device__ __forceinline__ void group_rows(const int lane, Epi& epi, const int (
   scan_step<kDppBcast31, 0xc>(a, b);
   const int next_cont = dpp_move<kDppWaveShl1>(0, (int)cont);
   const bool ends_here = lane == kWave - 1 || !next_cont;
-  if (hr >= 0) {
-    if (multi) {
-      epi(hr, b + hs);
-      if (ends_here) epi(tr, ts);
-    } else if (ends_here) {
-      epi(hr, b + ts);
-    }
+  // a row ends at slot e when the next slot holds another row (or padding);
+  // the lane's tail row only when the next lane does not continue it.  The
+  // head row's value carries b (zero unless it continues from earlier lanes).
+#pragma unroll
+  for (int e = 0; e < kGroupPer; ++e) {
+    const int r = rr[e];
+    const bool last = e == kGroupPer - 1 || rr[e + 1 < kGroupPer ? e + 1 : e] != r;
+    const bool is_tail = r == tr;
+    if (r >= 0 && last && (!is_tail || ends_here)) epi(r, r == hr ? b + seg[e] : seg[e]);
   }
 }
 
@@ -474,9 +474,13 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
         vv[2 * h + 0] = L.v[h].x;
         vv[2 * h + 1] = L.v[h].y;
       }
-      TX v[kGroupPer];
+      // every slot's product (padding: column 0, in bounds), then the select,
+      // so the gathers are issued together
+      TX pr[kGroupPer], v[kGroupPer];
 #pragma unroll
-      for (int e = 0; e < kGroupPer; ++e) v[e] = rr[e] >= 0 ? (TX)elem(vv[e], cc[e]) : (TX)0;
+      for (int e = 0; e < kGroupPer; ++e) pr[e] = (TX)elem(vv[e], cc[e]);
+#pragma unroll
+      for (int e = 0; e < kGroupPer; ++e) v[e] = rr[e] >= 0 ? pr[e] : (TX)0;
       group_rows<TX>(lane, epi, rr, v);
     };
     if (t.wv < M.ng) {
