@@ -328,13 +328,14 @@ def _mpc_stream(eng, reqs, arrivals, label):
             nxt = k1
         done = eng.step()
         t_seen = time.perf_counter() - t_start
-        for pid in done:
-            r = eng.poll(pid)
+        # the results' status, iterations and objective in one call (x and y
+        # are not needed here)
+        for pid, (st, it, ob) in eng.poll_info_many(done).items():
             k, a = arr_of[pid]
             lat.append(t_seen - a)
-            status[k] = r.info.status_val
-            objs[k] = r.info.obj_val
-            iters.append(r.info.iter)
+            status[k] = st
+            objs[k] = ob
+            iters.append(it)
             pending.discard(pid)
     return time.perf_counter() - t_start, lat, status, objs, iters
 

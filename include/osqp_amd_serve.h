@@ -135,6 +135,16 @@ OSQPInt osqp_batch_step(OSQPBatch* batch, OSQPInt* completed);
  * them written to ids); returns their number. */
 OSQPInt osqp_batch_last_completed(const OSQPBatch* batch, OSQPInt* ids, OSQPInt cap);
 
+/* osqp_batch_poll of count ids in one call: states[k] = 1 when problem k's
+ * result was taken (infos[k] = its info; a non-resident problem is released),
+ * 0 while it is queued or runs, 2 for a resident problem whose result was
+ * taken before (idle), -1 for an unknown id; infos[k] is zeroed otherwise.
+ * x / y (may be null): the x (n) and y (m) of the problems taken, one after
+ * another in the order of ids.  Returns the number taken, or -1 for bad
+ * arguments.  No counterpart in the reference. */
+OSQPInt osqp_batch_poll_many(OSQPBatch* batch, OSQPInt count, const OSQPInt* ids, OSQPFloat* x, OSQPFloat* y,
+                             OSQPBatchInfo* infos, OSQPInt* states);
+
 /* 1 and the result when the problem's solve has ended; 0 while it runs; -1
  * for an unknown id.  x: n, y: m, prim_inf_cert: m, dual_inf_cert: n (any may
  * be null), as OSQPSolution holds them after osqp_solve.  A non-resident

@@ -1152,6 +1152,33 @@ OSQPInt osqp_batch_add_many(OSQPBatch* b, OSQPInt count, const OSQPCscMatrix* co
   return first_error;
 }
 
+OSQPInt osqp_batch_poll_many(OSQPBatch* b, OSQPInt count, const OSQPInt* ids, OSQPFloat* x, OSQPFloat* y,
+                             OSQPBatchInfo* infos, OSQPInt* states) {
+  if (!b || count < 0 || (count > 0 && (!ids || !infos || !states))) return -1;
+  Engine* E = &b->e;
+  long long ox = 0, oy = 0;
+  OSQPInt polled = 0;
+  for (OSQPInt k = 0; k < count; ++k) {
+    auto it = E->probs.find(ids[k]);
+    memset(&infos[k], 0, sizeof(OSQPBatchInfo));
+    if (it == E->probs.end()) {
+      states[k] = -1;
+      continue;
+    }
+    const HProb* p = it->second;
+    if (p->state != kEnded) {
+      states[k] = p->state == kIdle ? 2 : 0;
+      continue;
+    }
+    const int n = p->h.n, m = p->h.m;
+    states[k] = osqp_batch_poll(b, ids[k], x ? x + ox : nullptr, y ? y + oy : nullptr, nullptr, nullptr, &infos[k]);
+    ox += n;
+    oy += m;
+    polled++;
+  }
+  return polled;
+}
+
 OSQPInt osqp_batch_update_data_mat(OSQPBatch* b, OSQPInt id, const OSQPFloat* Px_new, const OSQPInt* Px_new_idx,
                                    OSQPInt P_new_n, const OSQPFloat* Ax_new, const OSQPInt* Ax_new_idx,
                                    OSQPInt A_new_n) {

@@ -42,6 +42,8 @@ def _bind(lib):
     lib.osqp_batch_add.restype = C.c_int
     lib.osqp_batch_add_many.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, C.c_int, vp]
     lib.osqp_batch_add_many.restype = C.c_int
+    lib.osqp_batch_poll_many.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+    lib.osqp_batch_poll_many.restype = C.c_int
     lib.osqp_batch_resolve.argtypes = [vp, C.c_int, vp]
     lib.osqp_batch_resolve.restype = C.c_int
     lib.osqp_batch_update_data_mat.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int]
@@ -142,6 +144,26 @@ class BatchEngine:
                 out.append(ids[i])
             else:
                 out.append(None)
+        return out
+
+    def poll_info_many(self, ids):
+        """Takes the results of many ended solves in one call, without x and
+        y: a dict id -> (status_val, iter, obj_val) for those taken (a
+        non-resident problem is released, as by poll)."""
+        k = len(ids)
+        if k == 0:
+            return {}
+        idv = np.ascontiguousarray(ids, dtype=np.int32)
+        infos = (BatchInfo * k)()
+        states = np.empty(k, dtype=np.int32)
+        if self.lib.osqp_batch_poll_many(self._b, k, T.addr(idv), None, None, C.addressof(infos),
+                                         T.addr(states)) < 0:
+            raise ValueError("osqp_batch_poll_many: bad arguments")
+        out = {}
+        for j in range(k):
+            if states[j] == 1:
+                inf = infos[j]
+                out[int(idv[j])] = (inf.status_val, inf.iter, inf.obj_val)
         return out
 
     def resolve(self, pid, q=None):

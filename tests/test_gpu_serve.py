@@ -149,6 +149,43 @@ def test_add_many_matches_add():
         same(a, b)
 
 
+def test_poll_many_matches_poll():
+    """osqp_batch_poll_many (status, iterations and objective of many ended
+    solves in one call) gives the bits osqp_batch_poll gives for the same
+    problems, takes each result once, and reports a resident problem whose
+    result was already taken as idle (state 2) and an unknown id as -1."""
+    from osqp_amd.serve import BatchInfo
+    import ctypes as C
+    rng = np.random.default_rng(5)
+    probs = [pr.mpc(int(N), seed=500 + k) for k, N in enumerate(rng.integers(10, 51, 24))]
+    eng = BatchEngine(precision=32, **ST)
+    one = run_all(eng, probs)
+    eng.close()
+    eng = BatchEngine(precision=32, **ST)
+    ids = eng.add_many(probs)
+    got = {}
+    while eng.pending():
+        got.update(eng.poll_info_many(eng.step()))
+    assert sorted(got) == sorted(ids)
+    for i, r in zip(ids, one):
+        assert got[i] == (r.info.status_val, r.info.iter, r.info.obj_val)
+    # taken once: a second call finds the (released) ids unknown
+    assert eng.poll_info_many(ids) == {}
+    pid = eng.add(probs[0]["P"], probs[0]["q"], probs[0]["A"], probs[0]["l"], probs[0]["u"], resident=True)
+    while eng.pending():
+        eng.step()
+    infos, states = (BatchInfo * 2)(), np.zeros(2, dtype=np.int32)
+    idv = np.array([pid, pid], dtype=np.int32)
+    assert eng.lib.osqp_batch_poll_many(eng._b, 2, idv.ctypes.data, None, None, C.addressof(infos),
+                                        states.ctypes.data) == 1
+    assert list(states) == [1, 2]
+    idv[:] = (123456, pid)
+    assert eng.lib.osqp_batch_poll_many(eng._b, 2, idv.ctypes.data, None, None, C.addressof(infos),
+                                        states.ctypes.data) == 0
+    assert list(states) == [-1, 2]
+    eng.close()
+
+
 def test_team_size_does_not_change_results():
     probs = [pr.mpc(30, seed=2), pr.random_qp(80, 400, 0.08, 0.08, seed=4), pr.portfolio(500, seed=8)]
     outs = []
