@@ -299,6 +299,9 @@ def measured_traffic(kernel):
     return None
 
 
+MPC_INTAKE = 256  # arrivals taken in between two engine steps
+
+
 def _mpc_stream(eng, reqs, arrivals, label):
     """Feeds requests to the engine at their arrival times (seconds from the
     start) and steps it until every result is seen; returns the elapsed time,
@@ -315,10 +318,10 @@ def _mpc_stream(eng, reqs, arrivals, label):
             log(f"[serve] mpc {label}: {nxt} admitted, {len(lat)} done")
             t_log = time.perf_counter()
         now = time.perf_counter() - t_start
-        # take at most 256 arrivals between two steps, so the engine works on
-        # admitted requests while the host takes in the next ones
+        # take at most MPC_INTAKE arrivals between two steps, so the engine
+        # works on admitted requests while the host takes in the next ones
         k1 = nxt
-        while k1 < n_req and arrivals[k1] <= now and len(pending) + (k1 - nxt) < 4096 and k1 - nxt < 256:
+        while k1 < n_req and arrivals[k1] <= now and len(pending) + (k1 - nxt) < 4096 and k1 - nxt < MPC_INTAKE:
             k1 += 1
         if k1 > nxt:
             # the arrivals since the last step, queued in one call
