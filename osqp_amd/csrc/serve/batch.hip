@@ -8,10 +8,11 @@
 //
 // Host side: problems are copied at osqp_batch_add, converted to CSR (A, A',
 // P with both triangles), and admitted when a slot, an HBM block of the device
-// arena and pinned staging memory are free.  Each epoch the host writes every
-// bucket's table into pinned memory, replays the bucket's hipGraph (one kernel
-// node; grid and team size fixed per graph), waits, and collects the results
-// the kernel wrote to pinned memory.  An online tuner tries team sizes and
+// arena and pinned staging memory are free.  Each size bucket runs its epochs
+// on its own stream: when a bucket's epoch has ended the host collects the
+// results the kernel wrote to pinned memory, writes the bucket's next table
+// into pinned memory and replays the bucket's hipGraph (one kernel node; grid,
+// team size and team LDS fixed per graph).  An online tuner tries team sizes and
 // teams per CU on the first epochs of every bucket and keeps the fastest
 // (ADMM iterations per second); results do not depend on that choice.
 #include <hip/hip_runtime.h>

@@ -42,7 +42,7 @@ constexpr int kLongAlign = 8;    // compact long rows start at multiples of 8 en
 constexpr int kLongVec = 4;      // compact long-row entries per lane per load (float4 + 4 x 16 bits)
 constexpr int kLongVecRounds = 8;  // compact loads per lane issued together
 constexpr int kMaxTeamWaves = 8;   // teams of up to 512 threads
-constexpr int kTeamLdsBytes = 32768;  // dynamic LDS per team: CG vectors of problems with 6n + m small
+constexpr int kTeamLdsBytes = 32768;  // least dynamic LDS of a team (more where fewer teams share a CU)
 
 // phases of a resident problem
 enum : int { kPhaseIdle = 0, kPhaseSetup = 1, kPhaseUpdate = 2, kPhaseRun = 3, kPhaseDone = 4 };
