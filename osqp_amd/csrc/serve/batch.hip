@@ -692,6 +692,8 @@ struct Engine {
   size_t lds_per_cu = 65536;
   int lds_T[2] = {0, 0};  // team size and dynamic LDS bytes of the last lookup
   int lds_B[2] = {0, 0};
+  int lds_max[2] = {0, 0};  // the kernel's dynamic-LDS limit as set (only ever raised: the
+                            // graphs of other buckets may launch with the larger size)
   char* arena = nullptr;
   size_t arena_size = 0;
   RangeAlloc arena_alloc;
@@ -897,7 +899,13 @@ int team_lds_bytes(Engine* E, int T) {
     const long long share = (long long)(E->lds_per_cu / (size_t)per_cu) - (long long)fa.sharedSizeBytes - 1024;
     const int cand = (int)(std::min<long long>(share, 1 << 20) / 1024 * 1024);
     // a refused attribute leaves the default size (not an engine failure)
-    if (cand > bytes && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, cand) == hipSuccess) {
+    bool allowed = cand <= E->lds_max[w];
+    if (cand > bytes && !allowed &&
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, cand) == hipSuccess) {
+      E->lds_max[w] = cand;
+      allowed = true;
+    }
+    if (cand > bytes && allowed) {
       int per_cu2 = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, fn, T, cand) == hipSuccess && per_cu2 >= per_cu)
         bytes = cand;
