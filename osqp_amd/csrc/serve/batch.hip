@@ -241,17 +241,52 @@ class RangeAlloc {
 // Sizes of the row groups of one CSR matrix (the BCsr layout).  Rows of
 // 1..kLongRow entries are packed in row order into groups of kGroupSlots
 // entries; a row that does not fit the rest of the current group starts the
-// next one (the rest is padding), so no row crosses a group.
+// next one (the rest is padding), so no row crosses a group.  With the FP32
+// CG's slices (ell), the short rows (under kEllMin entries) are packed first,
+// into groups [0, ng_short), and the medium rows (kEllMin..kLongRow) after
+// them: the FP64 products walk every group, the FP32 CG only the short ones
+// and takes the medium rows from the slices.
 struct GroupDims {
   int ng = 0, nlong = 0, nempty = 0;
   long long long_slots = 0;  // long-row entries of the compact copy (rows padded to kLongAlign)
   int cols = 0;              // columns of the matrix
+  bool ell = false;          // medium rows also in slices (FP32 engine, fewer than 65536 columns)
+  int ng_short = 0;          // groups of the short rows (all groups without slices)
+  int nslice = 0;            // slices of kWave medium rows
+  long long ell_slots = 0;   // slice entries: kWave x the slice's width, summed
 };
 
-GroupDims count_groups(const int* rp, int rows, int cols) {
+// medium rows in slice order: longest first, row order among equals
+void ell_order(const int* rp, int rows, std::vector<int>* order) {
+  order->clear();
+  for (int r = 0; r < rows; ++r) {
+    const int len = rp[r + 1] - rp[r];
+    if (len >= kEllMin && len <= kLongRow) order->push_back(r);
+  }
+  std::stable_sort(order->begin(), order->end(),
+                   [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+}
+
+// a slice's width: its longest row rounded up to whole kEllVec chunks
+inline int ell_width(int len) { return (len + kEllVec - 1) / kEllVec * kEllVec; }
+
+GroupDims count_groups(const int* rp, int rows, int cols, bool fp32) {
   GroupDims d;
   d.cols = cols;
+  int nmed = 0;
+  for (int r = 0; r < rows; ++r) {
+    const int len = rp[r + 1] - rp[r];
+    if (len >= kEllMin && len <= kLongRow) nmed++;
+  }
+  d.ell = fp32 && cols < 65536 && nmed >= kEllMinRows;
   int fill = 0;
+  auto pack = [&](int len) {
+    if (d.ng == 0 || fill + len > kGroupSlots) {
+      d.ng++;
+      fill = 0;
+    }
+    fill += len;
+  };
   for (int r = 0; r < rows; ++r) {
     const int len = rp[r + 1] - rp[r];
     if (len > kLongRow) {
@@ -259,12 +294,24 @@ GroupDims count_groups(const int* rp, int rows, int cols) {
       d.long_slots += (len + kLongAlign - 1) / kLongAlign * kLongAlign;
     } else if (len == 0) {
       d.nempty++;
-    } else {
-      if (d.ng == 0 || fill + len > kGroupSlots) {
-        d.ng++;
-        fill = 0;
-      }
-      fill += len;
+    } else if (!d.ell || len < kEllMin) {
+      pack(len);
+    }
+  }
+  d.ng_short = d.ng;
+  if (d.ell) {
+    // the medium rows' groups start a new group
+    fill = kGroupSlots;
+    for (int r = 0; r < rows; ++r) {
+      const int len = rp[r + 1] - rp[r];
+      if (len >= kEllMin && len <= kLongRow) pack(len);
+    }
+    thread_local std::vector<int> order;
+    ell_order(rp, rows, &order);
+    d.nslice = (nmed + kWave - 1) / kWave;
+    for (int s = 0; s < d.nslice; ++s) {
+      const int r = order[(size_t)s * kWave];
+      d.ell_slots += (long long)kWave * ell_width(rp[r + 1] - rp[r]);
     }
   }
   return d;
@@ -299,6 +346,8 @@ struct Layout {
     // compact copy of the long rows for the FP32 CG (offsets 0 when absent)
     bool compact;
     size_t loff, lci16, lsrc, lv32;
+    // the FP32 CG's slices of the medium rows (offsets 0 when absent)
+    size_t eoff, erl, eci16, esrc, ev32;
   } gP, gA, gT;
   // dynamic
   size_t ctype, D, Dinv, E, Einv, rho, rhoinv, xa, xb, za, zb, y, xt, zt, dx, dy, Ax, Px, Aty, tn, tm, tn2;
@@ -346,6 +395,12 @@ Layout make_layout(const HostData& h, int vbytes) {
       o->lci16 = take(2 * (size_t)g.long_slots);
       o->lsrc = take(4 * (size_t)g.long_slots);
     }
+    if (g.ell) {
+      o->eoff = take(4 * ((size_t)g.nslice + 1));
+      o->erl = take(8 * (size_t)g.nslice * kWave);
+      o->eci16 = take(2 * (size_t)g.ell_slots);
+      o->esrc = take(4 * (size_t)g.ell_slots);
+    }
   };
   groups(&L.gP, h.gP);
   groups(&L.gA, h.gA);
@@ -359,8 +414,8 @@ Layout make_layout(const HostData& h, int vbytes) {
   L.gA.gval = take(8 * (size_t)h.gA.ng * kGroupSlots);
   L.gT.gval = take(8 * (size_t)h.gT.ng * kGroupSlots);
   auto dyn32 = [&](Layout::Groups* o, const GroupDims& g) {
-    if (!o->compact) return;
-    o->lv32 = take(4 * (size_t)g.long_slots);
+    if (o->compact) o->lv32 = take(4 * (size_t)g.long_slots);
+    if (g.ell) o->ev32 = take(4 * (size_t)g.ell_slots);
   };
   dyn32(&L.gP, h.gP);
   dyn32(&L.gA, h.gA);
@@ -453,6 +508,22 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
     std::fill(lsrc, lsrc + d.long_slots, -1);
   }
   int nl = 0, ne = 0, grp = -1, fill = 0, lpos = 0;
+  auto pack = [&](int r) {
+    const int s = rp[r], e = rp[r + 1], len = e - s;
+    if (grp < 0 || fill + len > kGroupSlots) {
+      grp++;
+      fill = 0;
+    }
+    int* gk = gk0 + (size_t)grp * kGroupSlots;
+    int* gr = gr0 + (size_t)grp * kGroupSlots;
+    int* gc = gc0 + (size_t)grp * kGroupSlots;
+    for (int k = s; k < e; ++k) {
+      const int slot = fill++;
+      gk[slot] = k;
+      gr[slot] = r;
+      gc[slot] = ci[k];
+    }
+  };
   for (int r = 0; r < rows; ++r) {
     const int s = rp[r], e = rp[r + 1], len = e - s;
     if (len > kLongRow) {
@@ -471,21 +542,49 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
       empty[ne++] = r;
       continue;
     }
-    if (grp < 0 || fill + len > kGroupSlots) {
-      grp++;
-      fill = 0;
-    }
-    int* gk = gk0 + (size_t)grp * kGroupSlots;
-    int* gr = gr0 + (size_t)grp * kGroupSlots;
-    int* gc = gc0 + (size_t)grp * kGroupSlots;
-    for (int k = s; k < e; ++k) {
-      const int slot = fill++;
-      gk[slot] = k;
-      gr[slot] = r;
-      gc[slot] = ci[k];
-    }
+    if (!d.ell || len < kEllMin) pack(r);
   }
   if (o.compact) loff[nl] = lpos;
+  if (!d.ell) return;
+  // the medium rows: their groups (a new group first), then the slices.
+  // Slice s holds rows order[kWave s + lane] (row -1, length 0 past the
+  // last), as (row, length) pairs; its width
+  // K is its first (longest) row's rounded up to kEllVec; entry k of the
+  // lane's row sits at eoff[s] + (k / kEllVec) kWave kEllVec + lane kEllVec
+  // + k % kEllVec, so a lane loads kEllVec consecutive entries at a time.
+  fill = kGroupSlots;
+  for (int r = 0; r < rows; ++r) {
+    const int len = rp[r + 1] - rp[r];
+    if (len >= kEllMin && len <= kLongRow) pack(r);
+  }
+  int* eoff = (int*)(img + (o.eoff - base));
+  int* erl = (int*)(img + (o.erl - base));
+  unsigned short* eci16 = (unsigned short*)(img + (o.eci16 - base));
+  int* esrc = (int*)(img + (o.esrc - base));
+  std::fill(eci16, eci16 + d.ell_slots, (unsigned short)0);
+  std::fill(esrc, esrc + d.ell_slots, -1);
+  thread_local std::vector<int> order;
+  ell_order(rp, rows, &order);
+  long long pos = 0;
+  for (int sl = 0; sl < d.nslice; ++sl) {
+    eoff[sl] = (int)pos;
+    const int r0 = order[(size_t)sl * kWave];
+    const int K = ell_width(rp[r0 + 1] - rp[r0]);
+    for (int lane = 0; lane < kWave; ++lane) {
+      const size_t at = (size_t)sl * kWave + lane;
+      const int r = at < order.size() ? order[at] : -1;
+      erl[2 * at] = r;
+      erl[2 * at + 1] = r >= 0 ? rp[r + 1] - rp[r] : 0;
+      if (r < 0) continue;
+      for (int k = 0; k < rp[r + 1] - rp[r]; ++k) {
+        const long long idx = pos + (long long)(k / kEllVec) * kWave * kEllVec + lane * kEllVec + k % kEllVec;
+        eci16[idx] = (unsigned short)ci[rp[r] + k];
+        esrc[idx] = rp[r] + k;
+      }
+    }
+    pos += (long long)kWave * K;
+  }
+  eoff[d.nslice] = (int)pos;
 }
 
 // Builds a problem's stage image (values raw, unscaled): A (CSC, m x n) ->
@@ -520,9 +619,9 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
     }
   for (int r = 0; r < n; ++r) prp[(size_t)r + 1] += prp[r];
   h->nnzP = prp[n];
-  h->gP = count_groups(prp.data(), n, n);
-  h->gA = count_groups(arp.data(), m, n);
-  h->gT = count_groups(A->p, n, m);
+  h->gP = count_groups(prp.data(), n, n, vbytes == 4);
+  h->gA = count_groups(arp.data(), m, n, vbytes == 4);
+  h->gT = count_groups(A->p, n, m, vbytes == 4);
   h->warm_x = x0 != nullptr;
   h->warm_y = y0 != nullptr;
   const Layout L = make_layout(*h, vbytes);
@@ -791,6 +890,13 @@ bool admit(Engine* E, HProb* p) {
     M->lci16 = o.compact ? (unsigned short*)(dev + o.lci16) : nullptr;
     M->lsrc = o.compact ? (int*)(dev + o.lsrc) : nullptr;
     M->lv32 = o.compact ? (float*)(dev + o.lv32) : nullptr;
+    M->ng32 = g.ell ? g.ng_short : g.ng;
+    M->nslice = g.ell ? g.nslice : 0;
+    M->eoff = g.ell ? (int*)(dev + o.eoff) : nullptr;
+    M->erl = g.ell ? (int*)(dev + o.erl) : nullptr;
+    M->eci16 = g.ell ? (unsigned short*)(dev + o.eci16) : nullptr;
+    M->esrc = g.ell ? (int*)(dev + o.esrc) : nullptr;
+    M->ev32 = g.ell ? (float*)(dev + o.ev32) : nullptr;
   };
   csr(&P->P, h.n, h.gP, L.gP, L.Prp, L.Pci, L.Pval);
   csr(&P->A, h.m, h.gA, L.gA, L.Arp, L.Aci, L.Aval);

@@ -40,7 +40,11 @@ constexpr int kGroupSlots = kWave * kGroupPer;  // entries per row group
 constexpr int kLongPer = 16;     // strides of a long row whose loads are issued together
 constexpr int kLongAlign = 8;    // compact long rows start at multiples of 8 entries
 constexpr int kLongVec = 4;      // compact long-row entries per lane per load (float4 + 4 x 16 bits)
-constexpr int kLongVecRounds = 8;  // compact loads per lane issued together
+constexpr int kLongVecRounds = 16;  // compact loads per lane issued together
+constexpr int kEllMin = 8;         // FP32 CG: rows of kEllMin..kLongRow entries go to slices
+constexpr int kEllMinRows = 64;    // ... when a matrix has at least this many of them
+constexpr int kEllVec = 4;         // slice entries per lane per load (float4 + 4 x 16 bits)
+constexpr int kEllRounds = 8;      // slice loads per lane issued together
 constexpr int kMaxTeamWaves = 8;   // teams of up to 512 threads
 constexpr int kTeamLdsBytes = 32768;  // least dynamic LDS of a team (more where fewer teams share a CU)
 
@@ -105,6 +109,21 @@ struct BCsr {
   unsigned short* lci16; // long-row entries (rows start at multiples of kLongAlign, zero padding)
   int* lsrc;             // their CSR positions, -1 for padding
   float* lv32;           // their (float) values, 0 for padding
+  // slices of the medium rows for the FP32 CG (null when absent; same
+  // conditions, and a matrix with fewer than kEllMinRows medium rows has
+  // none): the FP32 CG walks groups [0, ng32), which then hold only the
+  // short rows, and takes the medium rows from the slices, one row per lane
+  // summed in CSR order.  Slice s: (row, length) of lane l's row at
+  // erl[2 (kWave s + l)] (row -1 for padding), entries [eoff[s], eoff[s + 1]);
+  // entry k of the lane's row at
+  // eoff[s] + (k / kEllVec) kWave kEllVec + lane kEllVec + k % kEllVec.
+  int ng32;              // groups the FP32 CG walks (ng without slices)
+  int nslice;
+  int* eoff;             // nslice + 1
+  int* erl;              // nslice * kWave (row, length) pairs
+  unsigned short* eci16; // columns (0 for padding)
+  int* esrc;             // CSR positions, -1 for padding
+  float* ev32;           // (float) values, 0 for padding
 };
 
 // scalar state that survives between epochs (stored with the problem)
@@ -435,8 +454,12 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
   typedef double f64x2 __attribute__((ext_vector_type(2)));
   typedef float f32x4 __attribute__((ext_vector_type(4)));
   typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-  bool compact = false;
-  if constexpr (sizeof(TX) == 4) compact = M.loff != nullptr;
+  bool compact = false, sliced = false;
+  if constexpr (sizeof(TX) == 4) {
+    compact = M.loff != nullptr;
+    sliced = M.eoff != nullptr;
+  }
+  const int ng = sliced ? M.ng32 : M.ng;
   {
     const gptr<const int> grow = (gptr<const int>)M.grow;
     const gptr<const int> gci = (gptr<const int>)M.gci;
@@ -483,15 +506,15 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
       for (int e = 0; e < kGroupPer; ++e) v[e] = rr[e] >= 0 ? pr[e] : (TX)0;
       group_rows<TX>(lane, epi, rr, v);
     };
-    if (t.wv < M.ng) {
+    if (t.wv < ng) {
       Slots cur;
       load(t.wv, cur);
       for (int g = t.wv;;) {
         const int gn = g + t.W;
         Slots nxt;
-        if (gn < M.ng) load(gn, nxt);
+        if (gn < ng) load(gn, nxt);
         reduce(cur);
-        if (gn >= M.ng) break;
+        if (gn >= ng) break;
         cur = nxt;
         g = gn;
       }
@@ -500,11 +523,84 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
   const gptr<const int> empty = (gptr<const int>)M.empty;
   for (int j = t.tid; j < M.nempty; j += t.T) epi(empty[j], (TX)0);
   const gptr<const int> rp = (gptr<const int>)M.rp;
+  if (sliced) {
+    // slices: lane l sums its row's entries in order, kEllVec per load.  A
+    // wave takes slices wv, wv + W, ...; lane i holds the offsets of the
+    // i-th of them (windows of kWave slices), and the next slice's first
+    // kEllRounds loads of each kind are in flight while this one is added
+    // (loads past the slice's width read its first chunk, not added; entries
+    // past the row's length are padding, not added).  Wider slices load the
+    // rest chunk by chunk.
+    const gptr<const int> eoff = (gptr<const int>)M.eoff;
+    const gptr<const int> erl = (gptr<const int>)M.erl;
+    const gptr<const float> ev32 = (gptr<const float>)M.ev32;
+    const gptr<const unsigned short> eci16 = (gptr<const unsigned short>)M.eci16;
+    typedef int i32x2 __attribute__((ext_vector_type(2)));
+    constexpr int kChunk = kWave * kEllVec;
+    struct Slice {
+      f32x4 v[kEllRounds];
+      u32x2 c[kEllRounds];
+      i32x2 rl;
+      int b0, nch;
+    };
+    auto sload = [&](int sl, int b0, int b1, Slice& L) {
+      L.b0 = b0;
+      L.nch = (b1 - b0) / kChunk;
+      L.rl = *(gptr<const i32x2>)(erl + 2 * (sl * kWave + lane));
+#pragma unroll
+      for (int i = 0; i < kEllRounds; ++i) {
+        const int k = b0 + (i < L.nch ? i : 0) * kChunk + lane * kEllVec;
+        L.v[i] = *(gptr<const f32x4>)(ev32 + k);
+        L.c[i] = *(gptr<const u32x2>)(eci16 + k);
+      }
+    };
+    auto add4 = [&](TX& acc, const f32x4& v, const u32x2& c, int e0, int len) {
+#pragma unroll
+      for (int q = 0; q < kEllVec; ++q) {
+        const unsigned w = c[q >> 1];
+        const int col = (int)((q & 1) ? (w >> 16) : (w & 0xFFFFu));
+        const TX pr = (TX)elem((double)v[q], col);
+        if (e0 + q < len) acc += pr;
+      }
+    };
+    auto sreduce = [&](const Slice& L) {
+      TX acc = 0;
+      const int len = L.rl.y;
+#pragma unroll
+      for (int i = 0; i < kEllRounds; ++i) add4(acc, L.v[i], L.c[i], i * kEllVec, len);
+      for (int j = kEllRounds; j < L.nch; ++j) {
+        const int k = L.b0 + j * kChunk + lane * kEllVec;
+        const f32x4 v = *(gptr<const f32x4>)(ev32 + k);
+        const u32x2 c = *(gptr<const u32x2>)(eci16 + k);
+        add4(acc, v, c, j * kEllVec, len);
+      }
+      if (L.rl.x >= 0) epi(L.rl.x, acc);
+    };
+    for (int base = t.wv; base < M.nslice; base += kWave * t.W) {
+      const int mine = base + lane * t.W;
+      const int o0 = mine < M.nslice ? eoff[mine] : 0;
+      const int o1 = mine < M.nslice ? eoff[mine + 1] : 0;
+      const int cnt = (M.nslice - base + t.W - 1) / t.W < kWave ? (M.nslice - base + t.W - 1) / t.W : kWave;
+      Slice cur;
+      sload(base, __builtin_amdgcn_readfirstlane(o0), __builtin_amdgcn_readfirstlane(o1), cur);
+      for (int i = 0;;) {
+        const int in = i + 1;
+        Slice nxt;
+        if (in < cnt) sload(base + in * t.W, __builtin_amdgcn_readlane(o0, in), __builtin_amdgcn_readlane(o1, in), nxt);
+        sreduce(cur);
+        if (in >= cnt) break;
+        cur = nxt;
+        i = in;
+      }
+    }
+  }
   const gptr<const int> longs = (gptr<const int>)M.longs;
   if (compact) {
     // compact long rows: lane l adds entries s + 4 (64 i + l) + q, q = 0..3,
     // for i = 0, 1, ... in order (kLongVecRounds loads of each kind in
-    // flight; loads past the row's end read its first chunk, not added)
+    // flight; chunks that start past the row's end are not loaded, and a
+    // lane past the end of the last chunk reads the row's first entries, not
+    // added)
     const gptr<const int> loff = (gptr<const int>)M.loff;
     const gptr<const float> lv32 = (gptr<const float>)M.lv32;
     const gptr<const unsigned short> lci16 = (gptr<const unsigned short>)M.lci16;
@@ -520,8 +616,12 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
         for (int i = 0; i < kLongVecRounds; ++i) {
           const int k = k0 + i * kChunk + lane * kLongVec;
           const int kk = k < e ? k : s;
-          vv[i] = *(gptr<const f32x4>)(lv32 + kk);
-          cw[i] = *(gptr<const u32x2>)(lci16 + kk);
+          vv[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+          cw[i] = u32x2{0u, 0u};
+          if (k0 + i * kChunk < e) {
+            vv[i] = *(gptr<const f32x4>)(lv32 + kk);
+            cw[i] = *(gptr<const u32x2>)(lci16 + kk);
+          }
         }
 #pragma unroll
         for (int i = 0; i < kLongVecRounds; ++i) {
@@ -763,6 +863,12 @@ struct Solver {
         float* __restrict__ lv32 = M.lv32;
         const int ne = M.loff[M.nlong];
         for (int i = t.tid; i < ne; i += t.T) lv32[i] = lsrc[i] >= 0 ? (float)val[lsrc[i]] : 0.f;
+      }
+      if (M.ev32) {
+        const int* __restrict__ esrc = M.esrc;
+        float* __restrict__ ev32 = M.ev32;
+        const int ne = M.eoff[M.nslice];
+        for (int i = t.tid; i < ne; i += t.T) ev32[i] = esrc[i] >= 0 ? (float)val[esrc[i]] : 0.f;
       }
     }
     t.sync();
