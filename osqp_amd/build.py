@@ -2,7 +2,8 @@
 
   python -m osqp_amd.build          (or osqp_amd.build.build())
 
-C host core: gcc.  HIP kernels: hipcc --offload-arch=gfx950.  Every HIP file is
+C host core: gcc.  HIP kernels: hipcc --offload-arch=gfx950.  The Python front
+end's marshalling helper (osqp_amd/_marshal.so, CPython C API): gcc.  Every HIP file is
 compiled with -ffp-contract=off (elementwise results must equal the CPU's
 correctly rounded values) and with packed-FP32 instructions disabled.
 """
@@ -17,6 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libosqp_amd.so")
+MARSHAL = os.path.join(HERE, "_marshal.so")
 
 HIP_SOURCES = ["hip/runtime.hip", "hip/vector.hip", "hip/matrix.hip", "hip/panel.hip", "hip/pcg.hip", "serve/batch.hip"]
 C_SOURCES = ["host/auxil.c", "host/osqp_api.c", "host/polish.c", "host/profilers.c"]
@@ -59,7 +61,20 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocprofiler-sdk-roctx"])
     shutil.move(tmp, LIB)
+    build_marshal()
     return LIB
+
+
+def build_marshal():
+    """osqp_amd/_marshal.so: packs lists of QPs into osqp_batch_add_many's
+    arguments without per-array Python work (osqp_amd/serve.py uses it when
+    it imports; it links nothing but the interpreter's C API)."""
+    import sysconfig
+    tmp = MARSHAL + ".tmp"
+    _run(["gcc", "-O3", "-fPIC", "-shared", "-Wall", "-I", sysconfig.get_paths()["include"], "-I",
+          os.path.join(ROOT, "include"), os.path.join(CSRC, "py", "marshal.c"), "-o", tmp])
+    shutil.move(tmp, MARSHAL)
+    return MARSHAL
 
 
 if __name__ == "__main__":

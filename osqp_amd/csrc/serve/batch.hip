@@ -157,8 +157,19 @@ struct HostInputs {
   std::vector<OSQPFloat> px, ax, vq, vl, vu, vx0, vy0;
 };
 
+// whether any of a problem's arrays is device memory (hipPointerGetAttributes
+// serialises concurrent callers: call it from one thread)
+bool any_device(const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A, const OSQPFloat* l,
+                const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0) {
+  const void* ptrs[] = {P->p, P->i, P->x, A->p, A->i, A->x, q, l, u, x0, y0};
+  for (const void* v : ptrs)
+    if (osqp_amd_ptr_is_device(v)) return true;
+  return false;
+}
+
+// dev: any_device(...) of these arrays, found by the caller
 bool host_inputs(const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A, const OSQPFloat* l,
-                 const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, HostInputs* H) {
+                 const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, bool dev, HostInputs* H) {
   H->P = *P;
   H->A = *A;
   H->q = q;
@@ -166,9 +177,6 @@ bool host_inputs(const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix
   H->u = u;
   H->x0 = x0;
   H->y0 = y0;
-  const void* ptrs[] = {P->p, P->i, P->x, A->p, A->i, A->x, q, l, u, x0, y0};
-  bool dev = false;
-  for (const void* v : ptrs) dev = dev || osqp_amd_ptr_is_device(v);
   if (!dev) return true;
   const size_t n = (size_t)P->n, m = (size_t)A->m;
   bool good = true;
@@ -342,12 +350,14 @@ struct Layout {
   size_t s_begin, s_end;
   size_t Prp, Pci, Pval, Pdiag, Arp, Aci, Aval, Trp, Tci, Tval, q, l, u;
   struct Groups {
-    size_t longs, empty, gk, grow, gci, gval;
+    // static (in the stage image): longs, empty, goff, grow, loff, eoff, erl;
+    // dynamic (derived on the GPU at setup): gci, gval, lci16, lv32, eci16, ev32
+    size_t longs, empty, goff, grow, gci, gval;
     // compact copy of the long rows for the FP32 CG (offsets 0 when absent)
     bool compact;
-    size_t loff, lci16, lsrc, lv32;
+    size_t loff, lci16, lv32;
     // the FP32 CG's slices of the medium rows (offsets 0 when absent)
-    size_t eoff, erl, eci16, esrc, ev32;
+    size_t eoff, erl, eci16, ev32;
   } gP, gA, gT;
   // dynamic
   size_t ctype, D, Dinv, E, Einv, rho, rhoinv, xa, xb, za, zb, y, xt, zt, dx, dy, Ax, Px, Aty, tn, tm, tn2;
@@ -385,21 +395,14 @@ Layout make_layout(const HostData& h, int vbytes) {
     const size_t slots = (size_t)g.ng * kGroupSlots;
     o->longs = take(4 * (size_t)g.nlong);
     o->empty = take(4 * (size_t)g.nempty);
-    o->gk = take(4 * slots);
+    o->goff = take(slots);
     o->grow = take(4 * slots);
-    o->gci = take(4 * slots);
     // the compact long rows: FP32 engine, columns in 16 bits
     o->compact = vbytes == 4 && g.cols < 65536 && g.nlong > 0;
-    if (o->compact) {
-      o->loff = take(4 * ((size_t)g.nlong + 1));
-      o->lci16 = take(2 * (size_t)g.long_slots);
-      o->lsrc = take(4 * (size_t)g.long_slots);
-    }
+    if (o->compact) o->loff = take(4 * ((size_t)g.nlong + 1));
     if (g.ell) {
       o->eoff = take(4 * ((size_t)g.nslice + 1));
       o->erl = take(8 * (size_t)g.nslice * kWave);
-      o->eci16 = take(2 * (size_t)g.ell_slots);
-      o->esrc = take(4 * (size_t)g.ell_slots);
     }
   };
   groups(&L.gP, h.gP);
@@ -410,16 +413,21 @@ Layout make_layout(const HostData& h, int vbytes) {
   L.u = take(8 * m);
   o = align_up(o, 64);
   L.s_end = o;
-  L.gP.gval = take(8 * (size_t)h.gP.ng * kGroupSlots);
-  L.gA.gval = take(8 * (size_t)h.gA.ng * kGroupSlots);
-  L.gT.gval = take(8 * (size_t)h.gT.ng * kGroupSlots);
-  auto dyn32 = [&](Layout::Groups* o, const GroupDims& g) {
-    if (o->compact) o->lv32 = take(4 * (size_t)g.long_slots);
-    if (g.ell) o->ev32 = take(4 * (size_t)g.ell_slots);
+  auto dyn = [&](Layout::Groups* o, const GroupDims& g) {
+    o->gci = take(4 * (size_t)g.ng * kGroupSlots);
+    o->gval = take(8 * (size_t)g.ng * kGroupSlots);
+    if (o->compact) {
+      o->lci16 = take(2 * (size_t)g.long_slots);
+      o->lv32 = take(4 * (size_t)g.long_slots);
+    }
+    if (g.ell) {
+      o->eci16 = take(2 * (size_t)g.ell_slots);
+      o->ev32 = take(4 * (size_t)g.ell_slots);
+    }
   };
-  dyn32(&L.gP, h.gP);
-  dyn32(&L.gA, h.gA);
-  dyn32(&L.gT, h.gT);
+  dyn(&L.gP, h.gP);
+  dyn(&L.gA, h.gA);
+  dyn(&L.gT, h.gT);
   L.ctype = take(4 * m);
   L.D = take(8 * n);
   L.Dinv = take(8 * n);
@@ -485,54 +493,42 @@ Layout make_layout(const HostData& h, int vbytes) {
   return L;
 }
 
-// Pass 2 of the packing: the long and empty row lists and the slot arrays
-// (a group's entry number e sits in lane e / kGroupPer, element e % kGroupPer,
-// at slot e: a lane's slots are contiguous), written into the image img whose
-// first byte is block offset base.
-void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, const Layout::Groups& o,
-                  char* img, size_t base) {
+// Pass 2 of the packing: the long and empty row lists, each group slot's row
+// and its entry's offset within the row (a group's entry number e sits in
+// lane e / kGroupPer, element e % kGroupPer, at slot e: a lane's slots are
+// contiguous), the long rows' offsets in the compact copy and the slices'
+// offsets and (row, length) pairs, written into the image img whose first
+// byte is block offset base.  Columns and values are derived from these and
+// the CSR arrays on the GPU at setup (Solver::fill_group_values).
+void write_groups(const int* rp, int rows, const GroupDims& d, const Layout::Groups& o, char* img, size_t base) {
   int* longs = (int*)(img + (o.longs - base));
   int* empty = (int*)(img + (o.empty - base));
-  int* gk0 = (int*)(img + (o.gk - base));
+  unsigned char* go0 = (unsigned char*)(img + (o.goff - base));
   int* gr0 = (int*)(img + (o.grow - base));
-  int* gc0 = (int*)(img + (o.gci - base));
   const size_t slots = (size_t)d.ng * kGroupSlots;
-  std::fill(gk0, gk0 + slots, -1);
+  std::fill(go0, go0 + slots, (unsigned char)0);
   std::fill(gr0, gr0 + slots, -1);
-  std::fill(gc0, gc0 + slots, 0);
   int* loff = o.compact ? (int*)(img + (o.loff - base)) : nullptr;
-  unsigned short* lci16 = o.compact ? (unsigned short*)(img + (o.lci16 - base)) : nullptr;
-  int* lsrc = o.compact ? (int*)(img + (o.lsrc - base)) : nullptr;
-  if (o.compact) {
-    std::fill(lci16, lci16 + d.long_slots, (unsigned short)0);
-    std::fill(lsrc, lsrc + d.long_slots, -1);
-  }
   int nl = 0, ne = 0, grp = -1, fill = 0, lpos = 0;
   auto pack = [&](int r) {
-    const int s = rp[r], e = rp[r + 1], len = e - s;
+    const int len = rp[r + 1] - rp[r];
     if (grp < 0 || fill + len > kGroupSlots) {
       grp++;
       fill = 0;
     }
-    int* gk = gk0 + (size_t)grp * kGroupSlots;
+    unsigned char* go = go0 + (size_t)grp * kGroupSlots;
     int* gr = gr0 + (size_t)grp * kGroupSlots;
-    int* gc = gc0 + (size_t)grp * kGroupSlots;
-    for (int k = s; k < e; ++k) {
+    for (int k = 0; k < len; ++k) {
       const int slot = fill++;
-      gk[slot] = k;
+      go[slot] = (unsigned char)k;
       gr[slot] = r;
-      gc[slot] = ci[k];
     }
   };
   for (int r = 0; r < rows; ++r) {
-    const int s = rp[r], e = rp[r + 1], len = e - s;
+    const int len = rp[r + 1] - rp[r];
     if (len > kLongRow) {
       if (o.compact) {
         loff[nl] = lpos;
-        for (int k = s; k < e; ++k) {
-          lci16[lpos + (k - s)] = (unsigned short)ci[k];
-          lsrc[lpos + (k - s)] = k;
-        }
         lpos += (len + kLongAlign - 1) / kLongAlign * kLongAlign;
       }
       longs[nl++] = r;
@@ -548,10 +544,10 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
   if (!d.ell) return;
   // the medium rows: their groups (a new group first), then the slices.
   // Slice s holds rows order[kWave s + lane] (row -1, length 0 past the
-  // last), as (row, length) pairs; its width
-  // K is its first (longest) row's rounded up to kEllVec; entry k of the
-  // lane's row sits at eoff[s] + (k / kEllVec) kWave kEllVec + lane kEllVec
-  // + k % kEllVec, so a lane loads kEllVec consecutive entries at a time.
+  // last), as (row, length) pairs; its width K is its first (longest) row's
+  // rounded up to kEllVec; entry k of the lane's row sits at eoff[s] +
+  // (k / kEllVec) kWave kEllVec + lane kEllVec + k % kEllVec, so a lane
+  // loads kEllVec consecutive entries at a time.
   fill = kGroupSlots;
   for (int r = 0; r < rows; ++r) {
     const int len = rp[r + 1] - rp[r];
@@ -559,10 +555,6 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
   }
   int* eoff = (int*)(img + (o.eoff - base));
   int* erl = (int*)(img + (o.erl - base));
-  unsigned short* eci16 = (unsigned short*)(img + (o.eci16 - base));
-  int* esrc = (int*)(img + (o.esrc - base));
-  std::fill(eci16, eci16 + d.ell_slots, (unsigned short)0);
-  std::fill(esrc, esrc + d.ell_slots, -1);
   thread_local std::vector<int> order;
   ell_order(rp, rows, &order);
   long long pos = 0;
@@ -575,12 +567,6 @@ void write_groups(const int* rp, const int* ci, int rows, const GroupDims& d, co
       const int r = at < order.size() ? order[at] : -1;
       erl[2 * at] = r;
       erl[2 * at + 1] = r >= 0 ? rp[r + 1] - rp[r] : 0;
-      if (r < 0) continue;
-      for (int k = 0; k < rp[r + 1] - rp[r]; ++k) {
-        const long long idx = pos + (long long)(k / kEllVec) * kWave * kEllVec + lane * kEllVec + k % kEllVec;
-        eci16[idx] = (unsigned short)ci[rp[r] + k];
-        esrc[idx] = rp[r] + k;
-      }
     }
     pos += (long long)kWave * K;
   }
@@ -709,9 +695,9 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
     lo[i] = l[i] < -kInfty ? -kInfty : (l[i] > kInfty ? kInfty : l[i]);
     up[i] = u[i] < -kInfty ? -kInfty : (u[i] > kInfty ? kInfty : u[i]);
   }
-  write_groups(Prp, Pci, n, h->gP, L.gP, img, base);
-  write_groups(Arp, Aci, m, h->gA, L.gA, img, base);
-  write_groups(A->p, A->i, n, h->gT, L.gT, img, base);
+  write_groups(Prp, n, h->gP, L.gP, img, base);
+  write_groups(Arp, m, h->gA, L.gA, img, base);
+  write_groups(A->p, n, h->gT, L.gT, img, base);
   if (x0) memcpy(img + (L.st_x0 - L.st_static), x0, 8 * (size_t)n);
   if (y0) memcpy(img + (L.st_y0 - L.st_static), y0, 8 * (size_t)m);
   return L;
@@ -760,6 +746,21 @@ struct Cand {
 };
 const Cand kCands[] = {{64, 16}, {128, 8}, {256, 4}, {256, 8}, {512, 2}, {512, 4}};
 constexpr int kNumCands = sizeof(kCands) / sizeof(kCands[0]);
+// The tuner times every candidate of a bucket on two epochs, sweeping the
+// list forward and then backward, and ranks them by the sum of their two
+// rates: a bucket's load drifts while it is tuned (a burst drains, a stream
+// ramps up), and a drift that is linear in time then weighs every candidate
+// alike.  Buckets 0 and 1 (n + m + nnz(A) + nnz(P) <= 16384: at most 32 row
+// groups a product) leave out the 512-thread teams, whose eight wavefronts
+// such problems cannot keep busy; their epochs' times also move with the
+// other buckets' epochs running beside them, which can rank such a team first.
+constexpr int kSmallBuckets = 2;
+inline int cands_of(int bucket) { return bucket < kSmallBuckets ? 4 : kNumCands; }
+inline int trials_of(int bucket) { return 2 * cands_of(bucket); }
+inline int cand_of_trial(int bucket, int t) {
+  const int nc = cands_of(bucket);
+  return t < nc ? t : 2 * nc - 1 - t;
+}
 
 struct Bucket {
   std::vector<int> active;
@@ -775,7 +776,7 @@ struct Bucket {
   std::vector<int> incoming;     // admitted or re-solved while it was in flight
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // tuner
-  int trial = 0;  // next candidate to try; kNumCands when tuned
+  int trial = 0;  // next trial (cand_of_trial gives its candidate); trials_of(bucket) when tuned
   double rate[kNumCands] = {0};
   int best = 2;   // 256 x 4 until tuned
   int fixT = 0, fixTpc = 0;  // pinned configuration (osqp_batch_set_team)
@@ -802,6 +803,14 @@ struct Engine {
   std::map<int, HProb*> probs;
   std::vector<int> queue;  // waiting for admission, in arrival order
   std::vector<int> last_done;  // solves that ended in the last epoch
+  // images of the problems admitted in this step, copied into their stages
+  // by flush_stage_copies (on several threads when large)
+  struct StageCopy {
+    char* dst;
+    std::unique_ptr<char[]> src;
+    size_t bytes;
+  };
+  std::vector<StageCopy> stage_copies;
   int next_id = 1;
   int n_active = 0;
   Bucket buckets[kBuckets];
@@ -859,10 +868,10 @@ bool admit(Engine* E, HProb* p) {
   char* dev = E->arena + d;
   char* st = E->pool + s;
   char* rs = E->pool + r;
-  // static image (laid out as in the HBM block) and warm start, built at add
+  // static image (laid out as in the HBM block) and warm start, built at add:
+  // copied into the stage by flush_stage_copies before the epoch is launched
   char* img = st + L.st_static;
-  memcpy(img, h.img.get(), L.st_total - L.st_static);
-  h.img.reset();
+  E->stage_copies.push_back({img, std::move(h.img), L.st_total - L.st_static});
   // descriptor
   BProb* P = (BProb*)st;
   memset(P, 0, sizeof(BProb));
@@ -882,20 +891,18 @@ bool admit(Engine* E, HProb* p) {
     M->val = (double*)(dev + val);
     M->longs = (int*)(dev + o.longs);
     M->empty = (int*)(dev + o.empty);
-    M->gk = (int*)(dev + o.gk);
+    M->goff = (unsigned char*)(dev + o.goff);
     M->grow = (int*)(dev + o.grow);
     M->gci = (int*)(dev + o.gci);
     M->gval = (double*)(dev + o.gval);
     M->loff = o.compact ? (int*)(dev + o.loff) : nullptr;
     M->lci16 = o.compact ? (unsigned short*)(dev + o.lci16) : nullptr;
-    M->lsrc = o.compact ? (int*)(dev + o.lsrc) : nullptr;
     M->lv32 = o.compact ? (float*)(dev + o.lv32) : nullptr;
     M->ng32 = g.ell ? g.ng_short : g.ng;
     M->nslice = g.ell ? g.nslice : 0;
     M->eoff = g.ell ? (int*)(dev + o.eoff) : nullptr;
     M->erl = g.ell ? (int*)(dev + o.erl) : nullptr;
     M->eci16 = g.ell ? (unsigned short*)(dev + o.eci16) : nullptr;
-    M->esrc = g.ell ? (int*)(dev + o.esrc) : nullptr;
     M->ev32 = g.ell ? (float*)(dev + o.ev32) : nullptr;
   };
   csr(&P->P, h.n, h.gP, L.gP, L.Prp, L.Pci, L.Pval);
@@ -960,6 +967,30 @@ bool admit(Engine* E, HProb* p) {
   memset(P->out, 0, sizeof(BResult));
   p->cmd = 1;
   return true;
+}
+
+// The stage copies of this step's admissions (hundreds of KB a problem, a
+// few hundred problems a step: split over threads when over 2 MB).
+void flush_stage_copies(Engine* E) {
+  std::vector<Engine::StageCopy>& cs = E->stage_copies;
+  if (cs.empty()) return;
+  size_t total = 0;
+  for (const auto& c : cs) total += c.bytes;
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nthreads = total < ((size_t)2 << 20) ? 1 : (int)std::min<size_t>(std::min<size_t>(8, hw ? hw : 1), cs.size());
+  auto work = [&](int t) {
+    for (size_t k = (size_t)t; k < cs.size(); k += (size_t)nthreads) memcpy(cs[k].dst, cs[k].src.get(), cs[k].bytes);
+  };
+  std::vector<std::thread> pool;
+  try {
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+  } catch (...) {
+    // no more threads: this one copies what the missing threads would have
+    for (int t = 1 + (int)pool.size(); t < nthreads; ++t) work(t);
+  }
+  work(0);
+  for (auto& th : pool) th.join();
+  cs.clear();
 }
 
 bool ensure_table(Engine* E, Bucket& B, size_t count) {
@@ -1048,12 +1079,14 @@ constexpr int kAddThreads = 8;  // worker threads of osqp_batch_add_many
 // Validates a problem (src/auxil.c validate_data) and builds its host image.
 // Reads the engine's fixed configuration only, so problems may be prepared
 // on several threads at once.  Returns 0 with *out set, or an error code.
+// dev: any_device(...) of the problem's arrays
 int prepare(const Engine* E, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A, const OSQPFloat* l,
-            const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, OSQPInt resident, HProb** out) {
+            const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, OSQPInt resident, bool dev,
+            HProb** out) {
   *out = nullptr;
   if (!P || !A || !q) return OSQP_DATA_VALIDATION_ERROR;
   HostInputs H;
-  if (!host_inputs(P, q, A, l, u, x0, y0, &H)) return OSQP_DATA_VALIDATION_ERROR;
+  if (!host_inputs(P, q, A, l, u, x0, y0, dev, &H)) return OSQP_DATA_VALIDATION_ERROR;
   P = &H.P;
   A = &H.A;
   q = H.q;
@@ -1223,7 +1256,8 @@ OSQPInt osqp_batch_add(OSQPBatch* b, const OSQPCscMatrix* P, const OSQPFloat* q,
                        OSQPInt resident, OSQPInt* id) {
   if (!b || !id) return OSQP_DATA_VALIDATION_ERROR;
   HProb* p = nullptr;
-  const int rc = osqp_serve::prepare(&b->e, P, q, A, l, u, x0, y0, resident, &p);
+  const bool dev = P && A && osqp_serve::any_device(P, q, A, l, u, x0, y0);
+  const int rc = osqp_serve::prepare(&b->e, P, q, A, l, u, x0, y0, resident, dev, &p);
   if (rc) return rc;
   *id = osqp_serve::enqueue(&b->e, p);
   return 0;
@@ -1236,6 +1270,13 @@ OSQPInt osqp_batch_add_many(OSQPBatch* b, OSQPInt count, const OSQPCscMatrix* co
   if (!b || count < 0 || (count > 0 && (!P || !q || !A || !l || !u || !ids))) return OSQP_DATA_VALIDATION_ERROR;
   std::vector<HProb*> ready((size_t)count, nullptr);
   std::vector<int> codes((size_t)count, 0);
+  // which problems bring device arrays: on this thread (the runtime's pointer
+  // queries serialise concurrent callers)
+  std::vector<char> dev((size_t)count, 0);
+  for (int k = 0; k < count; ++k)
+    if (P[k] && A[k])
+      dev[(size_t)k] = osqp_serve::any_device(P[k], q[k], A[k], l[k], u[k], x0 ? x0[k] : nullptr,
+                                              y0 ? y0[k] : nullptr);
   // the host-side preparation (validation, CSR / row-group image) of each
   // problem touches only its own data: spread it over worker threads, then
   // queue in order on this thread
@@ -1243,7 +1284,8 @@ OSQPInt osqp_batch_add_many(OSQPBatch* b, OSQPInt count, const OSQPCscMatrix* co
   auto work = [&]() {
     for (int k = next.fetch_add(1); k < count; k = next.fetch_add(1))
       codes[(size_t)k] = osqp_serve::prepare(&b->e, P[k], q[k], A[k], l[k], u[k], x0 ? x0[k] : nullptr,
-                                             y0 ? y0[k] : nullptr, resident, &ready[(size_t)k]);
+                                             y0 ? y0[k] : nullptr, resident, dev[(size_t)k] != 0,
+                                             &ready[(size_t)k]);
   };
   const unsigned hw = std::thread::hardware_concurrency();
   const int nthreads = (int)std::min<long>(std::min<long>(kAddThreads, hw ? (long)hw : 1L), count / 8 + 1);
@@ -1458,12 +1500,12 @@ int finish_bucket(Engine* E, int k) {
   E->admm_iters += its;
   E->epochs++;
   // online tuner: iterations per second of each candidate on a full epoch
-  if (B.trial < kNumCands) {
-    B.rate[B.trial] = ms > 0 ? its / (double)ms : 0.0;
+  if (B.trial < trials_of(k)) {
+    B.rate[cand_of_trial(k, B.trial)] += ms > 0 ? its / (double)ms : 0.0;
     B.trial++;
-    if (B.trial == kNumCands) {
+    if (B.trial == trials_of(k)) {
       int best = 0;
-      for (int c = 1; c < kNumCands; ++c)
+      for (int c = 1; c < cands_of(k); ++c)
         if (B.rate[c] > B.rate[best]) best = c;
       B.best = best;
     }
@@ -1535,6 +1577,9 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
     }
   }
   E->queue.resize(kept);
+  // the admitted problems' images into their stages (while the epochs in
+  // flight run), before any of them is launched
+  flush_stage_copies(E);
   const int done = finish_ready(E);
   if (done < 0) return OSQP_MEM_ALLOC_ERROR;
   for (int id : fresh) {
@@ -1568,7 +1613,7 @@ OSQPInt osqp_batch_step(OSQPBatch* b, OSQPInt* completed) {
       if ((p->cmd == 2 || p->cmd == 3) && p->qnew) en.upd_q = (const double*)(E->pool + p->res_off + p->L.rs_q);
       en.upd_mat = p->cmd == 3 ? (const BMatUpd*)(E->pool + p->upd_off) : nullptr;
     }
-    const int c = B.trial < kNumCands ? B.trial : B.best;
+    const int c = B.trial < trials_of(k) ? cand_of_trial(k, B.trial) : B.best;
     const int T = B.fixT ? B.fixT : kCands[c].T, tpc = B.fixT ? B.fixTpc : kCands[c].tpc;
     if (!ensure_graph(E, B, T, tpc)) return OSQP_MEM_ALLOC_ERROR;
     if (!ok(E, hipEventRecord(B.e0, B.stream), "event record")) return OSQP_MEM_ALLOC_ERROR;
@@ -1649,7 +1694,7 @@ OSQPInt osqp_batch_pending(const OSQPBatch* b) {
 OSQPInt osqp_batch_tuning(const OSQPBatch* b, OSQPInt bucket, OSQPInt* team, OSQPInt* tpc) {
   if (!b || bucket < 0 || bucket >= kBuckets) return -1;
   const Bucket& B = b->e.buckets[bucket];
-  if (B.trial < kNumCands) return -1;
+  if (B.trial < trials_of(bucket)) return -1;
   if (team) *team = B.fixT ? B.fixT : kCands[B.best].T;
   if (tpc) *tpc = B.fixT ? B.fixTpc : kCands[B.best].tpc;
   return 0;
@@ -1661,7 +1706,7 @@ OSQPInt osqp_batch_set_team(OSQPBatch* b, OSQPInt bucket, OSQPInt team, OSQPInt 
   Bucket& B = b->e.buckets[bucket];
   B.fixT = (int)team;
   B.fixTpc = (int)tpc;
-  B.trial = kNumCands;
+  B.trial = trials_of(bucket);
   return 0;
 }
 

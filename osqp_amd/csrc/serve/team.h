@@ -86,8 +86,10 @@ enum : int { kAdaptOff = 0, kAdaptIters = 1, kAdaptTime = 2, kAdaptKkt = 3 };
 // empty-row lists.  Slot (g, lane, e) of the groups sits at
 // g * kGroupSlots + lane * kGroupPer + e and holds the group's entry
 // lane * kGroupPer + e (row order, then column order), or padding
-// (gk = grow = -1) after the group's last entry.  A lane's slots are
-// contiguous, so it loads them with 16-byte vector loads.
+// (grow = -1) after the group's last entry.  A lane's slots are
+// contiguous, so it loads them with 16-byte vector loads.  The host writes
+// the rows and the entries' offsets within them; fill_group_values derives
+// the columns and values from the CSR arrays.
 struct BCsr {
   int rows;
   int ng;       // row groups
@@ -98,7 +100,7 @@ struct BCsr {
   double* val;
   int* longs;
   int* empty;
-  int* gk;      // ng * kGroupSlots: CSR position of the slot's entry, -1 padding
+  unsigned char* goff;  // ng * kGroupSlots: the slot's entry's offset within its row
   int* grow;    // its row, -1 padding
   int* gci;     // its column (0 for padding)
   double* gval; // its value, copied from val after scaling (0 for padding)
@@ -107,7 +109,6 @@ struct BCsr {
   // values are (float) of val, so the products are the same
   int* loff;             // nlong + 1: start of each long row in the compact long-row arrays
   unsigned short* lci16; // long-row entries (rows start at multiples of kLongAlign, zero padding)
-  int* lsrc;             // their CSR positions, -1 for padding
   float* lv32;           // their (float) values, 0 for padding
   // slices of the medium rows for the FP32 CG (null when absent; same
   // conditions, and a matrix with fewer than kEllMinRows medium rows has
@@ -122,7 +123,6 @@ struct BCsr {
   int* eoff;             // nslice + 1
   int* erl;              // nslice * kWave (row, length) pairs
   unsigned short* eci16; // columns (0 for padding)
-  int* esrc;             // CSR positions, -1 for padding
   float* ev32;           // (float) values, 0 for padding
 };
 
@@ -848,27 +848,53 @@ struct Solver {
     update_precond();
   }
 
-  // the row groups' values in slot order, from the (scaled) CSR values
+  // the row groups' columns and values in slot order, and those of the
+  // compact long rows and of the slices, from the CSR arrays (values scaled)
   __device__ __forceinline__ void fill_group_values() {
     const BCsr* Ms[3] = {&p.P, &p.A, &p.At};
     for (int k = 0; k < 3; ++k) {
       const BCsr& M = *Ms[k];
-      const int* __restrict__ gk = M.gk;
+      const int* __restrict__ rp = M.rp;
+      const int* __restrict__ ci = M.ci;
       const double* __restrict__ val = M.val;
+      const int* __restrict__ grow = M.grow;
+      const unsigned char* __restrict__ goff = M.goff;
+      int* __restrict__ gci = M.gci;
       double* __restrict__ gval = M.gval;
       const int slots = M.ng * kGroupSlots;
-      for (int i = t.tid; i < slots; i += t.T) gval[i] = gk[i] >= 0 ? val[gk[i]] : 0.0;
+      for (int i = t.tid; i < slots; i += t.T) {
+        const int r = grow[i];
+        const int e = r >= 0 ? rp[r] + goff[i] : 0;
+        gci[i] = r >= 0 ? ci[e] : 0;
+        gval[i] = r >= 0 ? val[e] : 0.0;
+      }
       if (M.lv32) {
-        const int* __restrict__ lsrc = M.lsrc;
+        // long row j: entries loff[j] + q, q below its padded length
+        unsigned short* __restrict__ lci16 = M.lci16;
         float* __restrict__ lv32 = M.lv32;
-        const int ne = M.loff[M.nlong];
-        for (int i = t.tid; i < ne; i += t.T) lv32[i] = lsrc[i] >= 0 ? (float)val[lsrc[i]] : 0.f;
+        for (int j = 0; j < M.nlong; ++j) {
+          const int r = M.longs[j], s = M.loff[j], len = rp[r + 1] - rp[r], pad = M.loff[j + 1] - s;
+          for (int q = t.tid; q < pad; q += t.T) {
+            lci16[s + q] = q < len ? (unsigned short)ci[rp[r] + q] : (unsigned short)0;
+            lv32[s + q] = q < len ? (float)val[rp[r] + q] : 0.f;
+          }
+        }
       }
       if (M.ev32) {
-        const int* __restrict__ esrc = M.esrc;
+        // slice s, lane l: entries q < the slice's width at
+        // eoff[s] + (q / kEllVec) kWave kEllVec + l kEllVec + q % kEllVec
+        unsigned short* __restrict__ eci16 = M.eci16;
         float* __restrict__ ev32 = M.ev32;
-        const int ne = M.eoff[M.nslice];
-        for (int i = t.tid; i < ne; i += t.T) ev32[i] = esrc[i] >= 0 ? (float)val[esrc[i]] : 0.f;
+        for (int w = t.tid; w < M.nslice * kWave; w += t.T) {
+          const int sl = w / kWave, lane = w % kWave;
+          const int r = M.erl[2 * w], len = M.erl[2 * w + 1];
+          const int b0 = M.eoff[sl], width = (M.eoff[sl + 1] - b0) / kWave;
+          for (int q = 0; q < width; ++q) {
+            const int at = b0 + (q / kEllVec) * kWave * kEllVec + lane * kEllVec + q % kEllVec;
+            eci16[at] = q < len ? (unsigned short)ci[rp[r] + q] : (unsigned short)0;
+            ev32[at] = q < len ? (float)val[rp[r] + q] : 0.f;
+          }
+        }
       }
     }
     t.sync();

@@ -18,6 +18,11 @@ import numpy as np
 from . import _types as T
 from .solver import load_library
 
+try:  # the marshalling helper built beside libosqp_amd.so (osqp_amd/build.py)
+    from . import _marshal
+except ImportError:  # arrays are then converted one by one in Python
+    _marshal = None
+
 _bound = False
 
 
@@ -124,6 +129,22 @@ class BatchEngine:
         k = len(problems)
         if k == 0:
             return []
+        packed = _marshal.csc_batch(problems) if _marshal is not None else None
+        if packed is not None:
+            # every problem already in the engine's form (CSC, int32 / float64,
+            # contiguous, exact lengths): its arrays are passed as they are
+            buf, base = packed
+            ps, ts = C.sizeof(T.OSQPCscMatrix), C.sizeof(C.c_void_p)
+            tab = base + 2 * k * ps
+            ids = (C.c_int * k)()
+            self.lib.osqp_batch_add_many(self._b, k, tab, tab + k * ts, tab + 2 * k * ts, tab + 3 * k * ts,
+                                         tab + 4 * k * ts, tab + 5 * k * ts, tab + 6 * k * ts,
+                                         1 if resident else 0, ids)
+            dims = np.frombuffer(buf, dtype=np.int32, count=2 * k, offset=2 * k * ps + 7 * k * ts)
+            idl, nl, ml = list(ids), dims[:k].tolist(), dims[k:].tolist()
+            del buf
+            self._dims.update((i, (nn, mm)) for i, nn, mm in zip(idl, nl, ml) if i >= 0)
+            return [i if i >= 0 else None for i in idl]
         arr = lambda: (C.c_void_p * k)()
         Pp, qp, Ap, lp, up, xp, yp = arr(), arr(), arr(), arr(), arr(), arr(), arr()
         keep = []
