@@ -484,7 +484,7 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
     for j, (k, q) in enumerate(reqs):
         queue[k].append(j)
     t_start = time.perf_counter()
-    issued, lat, status, objs = {}, [], {}, {}
+    issued, lat, status, objs, its_of = {}, [], {}, {}, {}
     for k in range(n_problems):
         if queue[k]:
             j = queue[k].pop(0)
@@ -513,6 +513,7 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
             t_done.append(time.perf_counter() - t_start)
             status[j] = r.info.status_val
             objs[j] = r.info.obj_val
+            its_of[j] = r.info.iter
             k = pid.index(i)
             if queue[k]:
                 jn = queue[k].pop(0)
@@ -547,7 +548,12 @@ def serve_portfolio(n_problems, n_req, seed, cpu):
         cst, cob = cpu["portfolio"]["status"], cpu["portfolio"]["obj"]
         done = sorted(status)
         errs = [abs(objs[j] - cob[j]) / max(1.0, abs(cob[j])) for j in done if cst[j] == 1 and status[j] == 1]
+        # the requests whose status differs from the oracle's (at most 10)
+        diff = [{"request": j, "assets": int(probs[reqs[j][0]]["P"].shape[0] - probs[reqs[j][0]]["k"]),
+                 "gpu_status": int(status[j]), "gpu_iter": int(its_of[j]), "oracle_status": int(cst[j])}
+                for j in done if status[j] != cst[j]][:10]
         vs_oracle = {"requests": len(done), "statuses_identical": int(sum(int(status[j] == cst[j]) for j in done)),
+                     "status_differences": diff,
                      "obj_rel_err_vs_direct": {"max": float(np.max(errs)) if errs else None,
                                                "p50": percentile(errs, 50)},
                      "obj_rel_err_vs_fp64_optimum": {
