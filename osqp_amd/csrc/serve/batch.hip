@@ -55,9 +55,9 @@ struct BTable {
 
 constexpr int kCounterRing = 1024;
 
-template <typename V>
+template <typename V, bool SL>
 __device__ void run_entry(const Team& t, BProb& P, const BEntry& E, int iters, int* done_iters) {
-  Solver<V> sv(t, P);
+  Solver<V, SL> sv(t, P);
   if (E.cmd == 1) {
     sv.setup();
   } else if (E.cmd == 2) {
@@ -87,7 +87,8 @@ __device__ void run_entry(const Team& t, BProb& P, const BEntry& E, int iters, i
 // working set in registers (about 250 VGPRs, two waves per SIMD) instead of
 // spilling it at 128 (a 1024-thread bound); on the MPC stream that cut the GPU
 // time of the same work by 28 %.
-template <typename V>
+// SL: with the FP32 CG's slices (size buckets kSlicedBucket and up)
+template <typename V, bool SL>
 __global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters, int lds_bytes) {
   __shared__ double red[kMaxRedBlocks];
   __shared__ double red2[kMaxRedBlocks];
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(512) void k_epoch(BTable* tab, int* counters, int l
     int* dw = (int*)&P;
     for (int i = threadIdx.x; i < (int)(sizeof(BProb) / 4); i += blockDim.x) dw[i] = sw[i];
     __syncthreads();
-    run_entry<V>(t, P, E, iters, &done_iters);
+    run_entry<V, SL>(t, P, E, iters, &done_iters);
     __syncthreads();
     int* gw = (int*)E.dev;
     for (int i = threadIdx.x; i < (int)(sizeof(BProb) / 4); i += blockDim.x) gw[i] = dw[i];
@@ -263,6 +264,24 @@ struct GroupDims {
   int nslice = 0;            // slices of kWave medium rows
   long long ell_slots = 0;   // slice entries: kWave x the slice's width, summed
 };
+
+// Size bucket of a problem (one hipGraph and stream each): by
+// w = n + m + nnz(A) + nnz(P), 4096 / 16384 / 65536 / 262144 / 2^20 / more.
+inline int size_bucket(long long n, long long m, long long nnzA, long long nnzP) {
+  const long long w = n + m + nnzA + nnzP;
+  if (w <= (1 << 12)) return 0;
+  if (w <= (1 << 14)) return 1;
+  if (w <= (1 << 16)) return 2;
+  if (w <= (1 << 18)) return 3;
+  if (w <= (1 << 20)) return 4;
+  return 5;
+}
+// Buckets from kSlicedBucket on run the kernel with the FP32 CG's slices,
+// and only their problems get slices: a 5000-asset portfolio's CG iteration
+// 93.5 -> 81 us with them, while a 2000-asset one's (bucket 2) went from
+// 25 to 34 us, and MPC problems' epochs took 40 % longer in the kernel that
+// holds the slice path at all.
+constexpr int kSlicedBucket = 3;
 
 // medium rows in slice order: longest first, row order among equals
 void ell_order(const int* rp, int rows, std::vector<int>* order) {
@@ -605,9 +624,10 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
     }
   for (int r = 0; r < n; ++r) prp[(size_t)r + 1] += prp[r];
   h->nnzP = prp[n];
-  h->gP = count_groups(prp.data(), n, n, vbytes == 4);
-  h->gA = count_groups(arp.data(), m, n, vbytes == 4);
-  h->gT = count_groups(A->p, n, m, vbytes == 4);
+  const bool sliced = vbytes == 4 && size_bucket(n, m, h->nnzA, h->nnzP) >= kSlicedBucket;
+  h->gP = count_groups(prp.data(), n, n, sliced);
+  h->gA = count_groups(arp.data(), m, n, sliced);
+  h->gT = count_groups(A->p, n, m, sliced);
   h->warm_x = x0 != nullptr;
   h->warm_y = y0 != nullptr;
   const Layout L = make_layout(*h, vbytes);
@@ -731,15 +751,7 @@ constexpr int kBuckets = 6;
 constexpr int kEpochIters = 25;
 constexpr int kEpochItersLight = 15;
 
-int bucket_of(const HostData& h) {
-  const long long w = (long long)h.n + h.m + (long long)h.nnzA + (long long)h.nnzP;
-  if (w <= (1 << 12)) return 0;
-  if (w <= (1 << 14)) return 1;
-  if (w <= (1 << 16)) return 2;
-  if (w <= (1 << 18)) return 3;
-  if (w <= (1 << 20)) return 4;
-  return 5;
-}
+int bucket_of(const HostData& h) { return size_bucket(h.n, h.m, h.nnzA, h.nnzP); }
 
 struct Cand {
   int T, tpc;
@@ -780,6 +792,7 @@ struct Bucket {
   double rate[kNumCands] = {0};
   int best = 2;   // 256 x 4 until tuned
   int fixT = 0, fixTpc = 0;  // pinned configuration (osqp_batch_set_team)
+  bool sliced = false;       // runs the kernel with slices (bucket >= kSlicedBucket)
 };
 
 }  // namespace
@@ -790,10 +803,12 @@ struct Engine {
   int max_active = 4096;
   int cus = 256;
   size_t lds_per_cu = 65536;
-  int lds_T[2] = {0, 0};  // team size and dynamic LDS bytes of the last lookup
-  int lds_B[2] = {0, 0};
-  int lds_max[2] = {0, 0};  // the kernel's dynamic-LDS limit as set (only ever raised: the
-                            // graphs of other buckets may launch with the larger size)
+  // per kernel (FP32, FP32 with slices, FP64): team size and dynamic LDS
+  // bytes of the last lookup, and the kernel's dynamic-LDS limit as set (only
+  // ever raised: the graphs of other buckets may launch with the larger size)
+  int lds_T[3] = {0, 0, 0};
+  int lds_B[3] = {0, 0, 0};
+  int lds_max[3] = {0, 0, 0};
   char* arena = nullptr;
   size_t arena_size = 0;
   RangeAlloc arena_alloc;
@@ -1022,10 +1037,15 @@ bool ensure_table(Engine* E, Bucket& B, size_t count) {
 // kTeamLdsBytes and at most what keeps that team count resident.  A problem
 // whose CG vectors fit keeps them there (same arithmetic either way: the
 // bits do not depend on it).  -1 on a HIP error.
-int team_lds_bytes(Engine* E, int T) {
-  const int w = E->bits == 32 ? 0 : 1;
+const void* epoch_fn(const Engine* E, bool sliced) {
+  if (E->bits != 32) return (const void*)k_epoch<double, false>;
+  return sliced ? (const void*)k_epoch<float, true> : (const void*)k_epoch<float, false>;
+}
+
+int team_lds_bytes(Engine* E, int T, bool sliced) {
+  const int w = E->bits == 32 ? (sliced ? 1 : 0) : 2;
   if (E->lds_T[w] == T) return E->lds_B[w];
-  const void* fn = E->bits == 32 ? (const void*)k_epoch<float> : (const void*)k_epoch<double>;
+  const void* fn = epoch_fn(E, sliced);
   hipFuncAttributes fa;
   int per_cu = 0;
   if (!ok(E, hipFuncGetAttributes(&fa, fn), "hipFuncGetAttributes") ||
@@ -1059,14 +1079,17 @@ bool ensure_graph(Engine* E, Bucket& B, int T, int tpc) {
   if (B.graph) hipGraphDestroy(B.graph);
   B.exec = nullptr;
   B.graph = nullptr;
-  const int lds = team_lds_bytes(E, T);
+  const bool sliced = E->bits == 32 && B.sliced;
+  const int lds = team_lds_bytes(E, T, sliced);
   if (lds < 0) return false;
   if (!ok(E, hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal), "begin capture")) return false;
   const dim3 grid((unsigned)(tpc * E->cus)), block((unsigned)T);
-  if (E->bits == 32)
-    hipLaunchKernelGGL(k_epoch<float>, grid, block, (unsigned)lds, B.stream, B.tab, B.counters, lds);
+  if (E->bits != 32)
+    hipLaunchKernelGGL((k_epoch<double, false>), grid, block, (unsigned)lds, B.stream, B.tab, B.counters, lds);
+  else if (sliced)
+    hipLaunchKernelGGL((k_epoch<float, true>), grid, block, (unsigned)lds, B.stream, B.tab, B.counters, lds);
   else
-    hipLaunchKernelGGL(k_epoch<double>, grid, block, (unsigned)lds, B.stream, B.tab, B.counters, lds);
+    hipLaunchKernelGGL((k_epoch<float, false>), grid, block, (unsigned)lds, B.stream, B.tab, B.counters, lds);
   if (!ok(E, hipStreamEndCapture(B.stream, &B.graph), "end capture")) return false;
   if (!ok(E, hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "graph instantiate")) return false;
   B.g_T = T;
@@ -1230,6 +1253,7 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
   good = good && ok(E, hipHostMalloc((void**)&E->pool, E->pool_size, hipHostMallocDefault), "hipHostMalloc pool");
   for (int k = 0; k < kBuckets && good; ++k) {
     Bucket& Bk = E->buckets[k];
+    Bk.sliced = k >= kSlicedBucket;
     good = ok(E, hipStreamCreateWithFlags(&Bk.stream, hipStreamNonBlocking), "stream") &&
            ok(E, hipMalloc((void**)&Bk.counters, sizeof(int) * kCounterRing), "hipMalloc counters") &&
            ok(E, hipMemset(Bk.counters, 0, sizeof(int) * kCounterRing), "hipMemset counters") &&

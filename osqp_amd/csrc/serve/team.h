@@ -40,7 +40,8 @@ constexpr int kGroupSlots = kWave * kGroupPer;  // entries per row group
 constexpr int kLongPer = 16;     // strides of a long row whose loads are issued together
 constexpr int kLongAlign = 8;    // compact long rows start at multiples of 8 entries
 constexpr int kLongVec = 4;      // compact long-row entries per lane per load (float4 + 4 x 16 bits)
-constexpr int kLongVecRounds = 16;  // compact loads per lane issued together
+constexpr int kLongVecRounds = 16;  // compact loads per lane issued together (sliced kernel)
+constexpr int kLongVecRoundsLean = 8;  // the same in the kernel without slices
 constexpr int kEllMin = 8;         // FP32 CG: rows of kEllMin..kLongRow entries go to slices
 constexpr int kEllMinRows = 64;    // ... when a matrix has at least this many of them
 constexpr int kEllVec = 4;         // slice entries per lane per load (float4 + 4 x 16 bits)
@@ -446,7 +447,10 @@ __device__ __forceinline__ void group_rows(const int lane, Epi& epi, const int (
   }
 }
 
-template <typename TX, typename Elem, typename Epi>
+// SL: the kernel with slices (the large problems' buckets).  Without them the
+// slice path is not compiled at all: its registers, present even when unused,
+// cost the small problems' group loops 40 % more time.
+template <typename TX, bool SL = false, typename Elem, typename Epi>
 __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem, Epi epi) {
   const int lane = t.lane;
   static_assert(kGroupPer == 8, "a lane's slots are loaded as two int4 / four double2 vectors");
@@ -457,7 +461,7 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
   bool compact = false, sliced = false;
   if constexpr (sizeof(TX) == 4) {
     compact = M.loff != nullptr;
-    sliced = M.eoff != nullptr;
+    if constexpr (SL) sliced = M.eoff != nullptr;
   }
   const int ng = sliced ? M.ng32 : M.ng;
   {
@@ -523,7 +527,7 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
   const gptr<const int> empty = (gptr<const int>)M.empty;
   for (int j = t.tid; j < M.nempty; j += t.T) epi(empty[j], (TX)0);
   const gptr<const int> rp = (gptr<const int>)M.rp;
-  if (sliced) {
+  if constexpr (SL) if (sliced) {
     // slices: lane l sums its row's entries in order, kEllVec per load.  A
     // wave takes slices wv, wv + W, ...; lane i holds the offsets of the
     // i-th of them (windows of kWave slices), and the next slice's first
@@ -609,11 +613,12 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
       const int r = longs[j];
       const int s = loff[j], e = s + (rp[r + 1] - rp[r]);
       TX acc = 0;
-      for (int k0 = s; k0 < e; k0 += kLongVecRounds * kChunk) {
-        f32x4 vv[kLongVecRounds];
-        u32x2 cw[kLongVecRounds];
+      constexpr int kRounds = SL ? kLongVecRounds : kLongVecRoundsLean;
+      for (int k0 = s; k0 < e; k0 += kRounds * kChunk) {
+        f32x4 vv[kRounds];
+        u32x2 cw[kRounds];
 #pragma unroll
-        for (int i = 0; i < kLongVecRounds; ++i) {
+        for (int i = 0; i < kRounds; ++i) {
           const int k = k0 + i * kChunk + lane * kLongVec;
           const int kk = k < e ? k : s;
           vv[i] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -624,7 +629,7 @@ __device__ __forceinline__ void team_rows(const Team& t, const BCsr M, Elem elem
           }
         }
 #pragma unroll
-        for (int i = 0; i < kLongVecRounds; ++i) {
+        for (int i = 0; i < kRounds; ++i) {
           const int k = k0 + i * kChunk + lane * kLongVec;
 #pragma unroll
           for (int q = 0; q < kLongVec; ++q) {
@@ -681,7 +686,7 @@ __device__ __forceinline__ void team_spmv(const Team& t, const BCsr M, const dou
 // The problem descriptor is shared by the team (LDS, read-only here); the
 // scalar state S is a private copy in every thread, updated identically by all
 // of them from team-wide reductions (no shared read-modify-write).
-template <typename V>
+template <typename V, bool SL = false>
 struct Solver {
   const Team& t;
   const BProb& p;
@@ -998,16 +1003,16 @@ struct Solver {
       if (guard && !(rty > 0.0 && isfinite(rty))) go = false;
       while (go) {
         if (m) {
-          team_rows<V>(
+          team_rows<V, SL>(
               t, A, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { w[row] = (V)rho[row] * s; });
           t.sync();
         }
         // Kp = sigma p + P p + A' w
-        team_rows<V>(
+        team_rows<V, SL>(
             t, P, [=](double v, int c) { return (V)v * pp[c]; }, [=](int row, V s) { Kp[row] = s; });
         t.sync();
         if (m)
-          team_rows<V>(
+          team_rows<V, SL>(
               t, At, [=](double v, int c) { return (V)v * w[c]; },
               [=](int row, V s) { Kp[row] = (V)sg * pp[row] + Kp[row] + s; });
         else
