@@ -24,6 +24,7 @@
 #include <chrono>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -247,6 +248,28 @@ class RangeAlloc {
   std::map<size_t, size_t> free_, used_;
 };
 
+// Pinned memory for the stage images of problems not yet set up: the
+// preparation (worker threads) builds a problem's image there, so admission
+// does not copy it, and the block is released with the stage after the
+// problem's setup epoch.  When it is full an image is built on the heap and
+// copied into the stage at admission.  Separate from the staging pool, so
+// images waiting in the queue never hold the pool space admission needs.
+struct ImageArena {
+  char* base = nullptr;
+  size_t size = 0;
+  RangeAlloc alloc;
+  std::mutex mu;
+  char* take(size_t bytes, size_t* off) {
+    if (!base) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    return alloc.alloc(bytes, off) ? base + *off : nullptr;
+  }
+  void give(size_t off) {
+    std::lock_guard<std::mutex> lk(mu);
+    alloc.release(off);
+  }
+};
+
 // Sizes of the row groups of one CSR matrix (the BCsr layout).  Rows of
 // 1..kLongRow entries are packed in row order into groups of kGroupSlots
 // entries; a row that does not fit the rest of the current group starts the
@@ -353,6 +376,9 @@ struct HostData {
   GroupDims gP, gA, gT;
   bool warm_x = false, warm_y = false;
   std::unique_ptr<char[]> img;  // stage bytes [st_static, st_total): static image, x0, y0
+  bool img_pinned = false;      // the image is at img_off in the engine's image arena instead
+  size_t img_off = 0;
+  char* img_ptr = nullptr;      // the image's first byte, wherever it lives
   // resident problems: where each entry of the input CSC of P (triangle) and
   // of A lives in the HBM copies, for value updates (osqp_update_data_mat):
   // P entry k at row-of-column slot pslot_lo[k] and, off the diagonal, at
@@ -598,7 +624,7 @@ void write_groups(const int* rp, int rows, const GroupDims& d, const Layout::Gro
 // +-OSQP_INFTY as osqp_setup does; the row groups of all three; the warm start.
 Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A,
                   const OSQPFloat* l, const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0,
-                  bool keep_maps) {
+                  bool keep_maps, ImageArena* ia = nullptr) {
   const int n = P->n, m = A->m;
   h->n = n;
   h->m = m;
@@ -631,8 +657,14 @@ Layout build_host(HostData* h, int vbytes, const OSQPCscMatrix* P, const OSQPFlo
   h->warm_x = x0 != nullptr;
   h->warm_y = y0 != nullptr;
   const Layout L = make_layout(*h, vbytes);
-  h->img.reset(new char[L.st_total - L.st_static]);
-  char* img = h->img.get();
+  // the image in the pinned arena when it has room, else on the heap
+  char* img = ia ? ia->take(L.st_total - L.st_static, &h->img_off) : nullptr;
+  h->img_pinned = img != nullptr;
+  if (!img) {
+    h->img.reset(new char[L.st_total - L.st_static]);
+    img = h->img.get();
+  }
+  h->img_ptr = img;
   const size_t base = L.s_begin;
   auto at = [&](size_t off) { return img + (off - base); };
   const size_t nnzA = (size_t)h->nnzA, nnzP = (size_t)h->nnzP;
@@ -815,6 +847,7 @@ struct Engine {
   char* pool = nullptr;  // pinned
   size_t pool_size = 0;
   RangeAlloc pool_alloc;
+  ImageArena images;  // pinned; the stage images of problems not yet set up
   std::map<int, HProb*> probs;
   std::vector<int> queue;  // waiting for admission, in arrival order
   std::vector<int> last_done;  // solves that ended in the last epoch
@@ -855,6 +888,11 @@ void free_problem_memory(Engine* E, HProb* p, bool dev, bool stage, bool res) {
     E->pool_alloc.release(p->stage_off);
     p->has_stage = false;
   }
+  if (stage && p->h.img_pinned) {
+    E->images.give(p->h.img_off);
+    p->h.img_pinned = false;
+    p->h.img_ptr = nullptr;
+  }
   if (res && p->has_res) {
     E->pool_alloc.release(p->res_off);
     p->has_res = false;
@@ -867,7 +905,8 @@ bool admit(Engine* E, HProb* p) {
   const Layout& L = p->L;
   size_t d, s, r;
   if (!E->arena_alloc.alloc(L.total, &d)) return false;
-  if (!E->pool_alloc.alloc(L.st_total, &s)) {
+  // the stage: descriptor, then the image unless it is already pinned
+  if (!E->pool_alloc.alloc(h.img_pinned ? L.st_static : L.st_total, &s)) {
     E->arena_alloc.release(d);
     return false;
   }
@@ -884,9 +923,10 @@ bool admit(Engine* E, HProb* p) {
   char* st = E->pool + s;
   char* rs = E->pool + r;
   // static image (laid out as in the HBM block) and warm start, built at add:
-  // copied into the stage by flush_stage_copies before the epoch is launched
-  char* img = st + L.st_static;
-  E->stage_copies.push_back({img, std::move(h.img), L.st_total - L.st_static});
+  // read where it is when pinned, else copied into the stage by
+  // flush_stage_copies before the epoch is launched
+  char* img = h.img_pinned ? h.img_ptr : st + L.st_static;
+  if (!h.img_pinned) E->stage_copies.push_back({img, std::move(h.img), L.st_total - L.st_static});
   // descriptor
   BProb* P = (BProb*)st;
   memset(P, 0, sizeof(BProb));
@@ -968,8 +1008,8 @@ bool admit(Engine* E, HProb* p) {
   P->stage_bytes = (long long)(L.s_end - L.s_begin);
   P->arena_static = dev + L.s_begin;
   P->upd_q = nullptr;
-  P->warm_x = h.warm_x ? (const double*)(st + L.st_x0) : nullptr;
-  P->warm_y = h.warm_y ? (const double*)(st + L.st_y0) : nullptr;
+  P->warm_x = h.warm_x ? (const double*)(img + (L.st_x0 - L.st_static)) : nullptr;
+  P->warm_y = h.warm_y ? (const double*)(img + (L.st_y0 - L.st_static)) : nullptr;
   P->out_x = (double*)(rs + L.rs_x);
   P->out_y = (double*)(rs + L.rs_y);
   P->out_pcert = (double*)(rs + L.rs_pc);
@@ -1103,7 +1143,7 @@ constexpr int kAddThreads = 8;  // worker threads of osqp_batch_add_many
 // Reads the engine's fixed configuration only, so problems may be prepared
 // on several threads at once.  Returns 0 with *out set, or an error code.
 // dev: any_device(...) of the problem's arrays
-int prepare(const Engine* E, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A, const OSQPFloat* l,
+int prepare(Engine* E, const OSQPCscMatrix* P, const OSQPFloat* q, const OSQPCscMatrix* A, const OSQPFloat* l,
             const OSQPFloat* u, const OSQPFloat* x0, const OSQPFloat* y0, OSQPInt resident, bool dev,
             HProb** out) {
   *out = nullptr;
@@ -1134,11 +1174,12 @@ int prepare(const Engine* E, const OSQPCscMatrix* P, const OSQPFloat* q, const O
     if (l[i] > u[i]) return OSQP_DATA_VALIDATION_ERROR;
   HProb* p = new HProb();
   p->resident = resident ? 1 : 0;
-  p->L = build_host(&p->h, E->bits / 8, P, q, A, l, u, x0, y0, resident != 0);
+  p->L = build_host(&p->h, E->bits / 8, P, q, A, l, u, x0, y0, resident != 0, &E->images);
   if (p->L.total > E->arena_size || p->L.st_total + p->L.rs_total > E->pool_size) {
     // could never be admitted: larger than the whole arena or staging pool
     fprintf(stderr, "osqp_batch: problem needs %zu bytes of device arena (of %zu) and %zu of staging (of %zu)\n",
             p->L.total, E->arena_size, p->L.st_total + p->L.rs_total, E->pool_size);
+    if (p->h.img_pinned) E->images.give(p->h.img_off);
     delete p;
     return OSQP_DATA_VALIDATION_ERROR;
   }
@@ -1251,6 +1292,16 @@ OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* s, OSQPInt prec
   E->pool_size = std::max<size_t>(E->arena_size / 4, (size_t)64 << 20);
   E->pool_size = std::min<size_t>(E->pool_size, (size_t)1 << 30);
   good = good && ok(E, hipHostMalloc((void**)&E->pool, E->pool_size, hipHostMallocDefault), "hipHostMalloc pool");
+  // the image arena: half the pool's size; without it images go through the heap
+  if (good) {
+    E->images.size = E->pool_size / 2;
+    if (hipHostMalloc((void**)&E->images.base, E->images.size, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      E->images.base = nullptr;
+      E->images.size = 0;
+    }
+    E->images.alloc.reset(E->images.size);
+  }
   for (int k = 0; k < kBuckets && good; ++k) {
     Bucket& Bk = E->buckets[k];
     Bk.sliced = k >= kSlicedBucket;
@@ -1762,6 +1813,7 @@ OSQPInt osqp_batch_destroy(OSQPBatch* b) {
   if (E->ref) hipEventDestroy(E->ref);
   if (E->arena) hipFree(E->arena);
   if (E->pool) hipHostFree(E->pool);
+  if (E->images.base) hipHostFree(E->images.base);
   delete b;
   osqp_algebra_free_libs();
   return 0;
