@@ -6,9 +6,11 @@
 // up to `iters` ADMM iterations (team.h), writes the problem's state back to
 // its HBM block and takes the next problem, until the table is exhausted.
 //
-// Host side: problems are copied at osqp_batch_add, converted to CSR (A, A',
-// P with both triangles), and admitted when a slot, an HBM block of the device
-// arena and pinned staging memory are free.  Each size bucket runs its epochs
+// Host side: at osqp_batch_add a problem's stage image is built (CSR of A, A'
+// and P with both triangles, row-group rows and offsets; in a pinned image
+// arena when it has room), and the problem is admitted when a slot, an HBM
+// block of the device arena and pinned staging memory are free; its setup
+// epoch copies the image into its HBM block and derives the rest there.  Each size bucket runs its epochs
 // on its own stream: when a bucket's epoch has ended the host collects the
 // results the kernel wrote to pinned memory, writes the bucket's next table
 // into pinned memory and replays the bucket's hipGraph (one kernel node; grid,
@@ -367,8 +369,9 @@ GroupDims count_groups(const int* rp, int rows, int cols, bool fp32) {
   return d;
 }
 
-// Dimensions of a problem and its pinned stage image, built once at add time
-// in its final byte layout (so admission is one copy into the stage).
+// Dimensions of a problem and its stage image, built once at add time in its
+// final byte layout, in pinned memory when the image arena has room (then
+// admission copies nothing), else on the heap (then one copy into the stage).
 struct HostData {
   int n = 0, m = 0;
   int nnzP = 0;  // the full symmetric P
