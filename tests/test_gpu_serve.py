@@ -149,6 +149,32 @@ def test_add_many_matches_add():
         same(a, b)
 
 
+def test_image_arena_overflow_matches():
+    """Stage images are built in a pinned image arena (half the staging pool)
+    while it has room, else on the heap and copied into the stage at
+    admission.  A small engine (64 MB pool: 32 MB of image arena; 256 MB of
+    device arena, so admission also waits for memory) overflows on 240 MPC
+    problems of N = 40..50 (0.35-0.44 MB of image each, about 96 MB) and must give the
+    same bits as a default-sized engine, whose arena holds them all."""
+    rng = np.random.default_rng(11)
+    probs = [pr.mpc(int(N), seed=900 + k) for k, N in enumerate(rng.integers(40, 51, 240))]
+    eng = BatchEngine(precision=32, **ST)
+    ref = run_all(eng, probs)
+    eng.close()
+    eng = BatchEngine(precision=32, arena_bytes=256 << 20, **ST)
+    ids = eng.add_many(probs)
+    assert all(i is not None for i in ids)
+    got = {}
+    for _ in range(100000):
+        if not eng.pending():
+            break
+        for i in eng.step():
+            got[i] = eng.poll(i)
+    eng.close()
+    for a, i in zip(ref, ids):
+        same(a, got[i])
+
+
 def test_poll_many_matches_poll():
     """osqp_batch_poll_many (status, iterations and objective of many ended
     solves in one call) gives the bits osqp_batch_poll gives for the same
