@@ -19,7 +19,9 @@ library) on the same data:
   * resident problems with q updates and warm-started re-solves follow
     osqp_update_data_vec + osqp_solve;
   * admission is limited by slots and arena memory, and problems added while
-    others run are admitted as capacity frees up.
+    others run are admitted as capacity frees up;
+  * stage images built in the pinned image arena and those that overflow it
+    to the heap give the same bits.
 """
 import numpy as np
 import pytest
