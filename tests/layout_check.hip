@@ -11,11 +11,17 @@
 //     slot is written once, the slice sums equal the CSR row sums;
 //   * group slots hold consecutive CSR entries of one row; long-row padding
 //     and chunk loads stay inside the compact copy;
-//   * only problems of the sliced buckets get slices.
+//   * only problems of the sliced buckets get slices;
+//   * images built in the image arena (here over ordinary memory) equal the
+//     heap-built ones, a full arena falls back to the heap, freed blocks
+//     coalesce, and concurrent take / give never hand out overlapping blocks.
 // usage: layout_check <problems file> [plant]   ("plant" corrupts one slice
 // length in each sliced image, which must then fail)
 #include "serve/batch.hip"
 #include <set>
+#include <array>
+#include <random>
+#include <thread>
 using namespace osqp_serve;
 
 static int fails = 0;
@@ -194,6 +200,114 @@ static void check_matrix(const char* name, const char* blk, size_t total, int ro
          d.nlong, d.nslice, d.ell_slots, d.ell ? "" : " (no slices)");
 }
 
+// the image arena over ordinary memory: arena-built images equal heap-built
+// ones; once full, images go to the heap; giving every block back leaves one
+// free range; eight threads taking and giving blocks never overlap
+static void check_image_arena(const std::vector<std::array<OSQPCscMatrix, 2>>& probs, const std::vector<int>& ns,
+                              const std::vector<int>& ms) {
+  size_t total = 0;
+  std::vector<double> q, l, u;
+  for (size_t t = 0; t < probs.size(); ++t) {
+    HostData h;
+    q.assign(ns[t], 0.0), l.assign(ms[t], -1.0), u.assign(ms[t], 1.0);
+    const Layout L = build_host(&h, 4, &probs[t][0], q.data(), &probs[t][1], l.data(), u.data(), nullptr, nullptr, true);
+    total += align_up(L.st_total - L.st_static, 256);
+  }
+  std::vector<char> mem(total / 2);
+  ImageArena ia;
+  ia.base = mem.data();
+  ia.size = mem.size();
+  ia.alloc.reset(ia.size);
+  std::vector<std::unique_ptr<HostData>> keep;
+  int pinned = 0, heap = 0;
+  for (size_t t = 0; t < probs.size(); ++t) {
+    q.assign(ns[t], 0.0), l.assign(ms[t], -1.0), u.assign(ms[t], 1.0);
+    keep.emplace_back(new HostData());
+    HostData& h1 = *keep.back();
+    HostData h2;
+    const Layout L1 = build_host(&h1, 4, &probs[t][0], q.data(), &probs[t][1], l.data(), u.data(), nullptr, nullptr,
+                                 true, &ia);
+    build_host(&h2, 4, &probs[t][0], q.data(), &probs[t][1], l.data(), u.data(), nullptr, nullptr, true);
+    const size_t bytes = L1.st_total - L1.st_static;
+    CHECK(!h2.img_pinned && h2.img_ptr == h2.img.get(), "heap image");
+    CHECK(h1.img_pinned == (h1.img_ptr >= mem.data() && h1.img_ptr + bytes <= mem.data() + mem.size()), "arena image");
+    // every region the builder writes (the alignment gaps between them are
+    // written by no one and read by no one)
+    const size_t b0 = L1.s_begin;
+    auto same = [&](size_t off, size_t len, const char* what) {
+      CHECK(off >= b0 && off - b0 + len <= bytes && memcmp(h1.img_ptr + (off - b0), h2.img_ptr + (off - b0), len) == 0,
+            "arena image %zu: %s differs from the heap image", t, what);
+    };
+    const size_t n = (size_t)ns[t], m = (size_t)ms[t];
+    same(L1.Prp, 4 * (n + 1), "Prp");
+    same(L1.Pci, 4 * (size_t)h1.nnzP, "Pci");
+    same(L1.Pval, 8 * (size_t)h1.nnzP, "Pval");
+    same(L1.Pdiag, 4 * n, "Pdiag");
+    same(L1.Arp, 4 * (m + 1), "Arp");
+    same(L1.Aci, 4 * (size_t)h1.nnzA, "Aci");
+    same(L1.Aval, 8 * (size_t)h1.nnzA, "Aval");
+    same(L1.Trp, 4 * (n + 1), "Trp");
+    same(L1.Tci, 4 * (size_t)h1.nnzA, "Tci");
+    same(L1.Tval, 8 * (size_t)h1.nnzA, "Tval");
+    same(L1.q, 8 * n, "q");
+    same(L1.l, 8 * m, "l");
+    same(L1.u, 8 * m, "u");
+    const GroupDims* gd[3] = {&h1.gP, &h1.gA, &h1.gT};
+    const Layout::Groups* go[3] = {&L1.gP, &L1.gA, &L1.gT};
+    for (int k = 0; k < 3; ++k) {
+      const GroupDims& g = *gd[k];
+      const Layout::Groups& o = *go[k];
+      const size_t slots = (size_t)g.ng * kGroupSlots;
+      same(o.longs, 4 * (size_t)g.nlong, "longs");
+      same(o.empty, 4 * (size_t)g.nempty, "empty");
+      same(o.goff, slots, "goff");
+      same(o.grow, 4 * slots, "grow");
+      if (o.compact) same(o.loff, 4 * ((size_t)g.nlong + 1), "loff");
+      if (g.ell) {
+        same(o.eoff, 4 * ((size_t)g.nslice + 1), "eoff");
+        same(o.erl, 8 * (size_t)g.nslice * kWave, "erl");
+      }
+    }
+    (h1.img_pinned ? pinned : heap)++;
+  }
+  CHECK(pinned > 0 && heap > 0, "arena of half the images: %d pinned, %d on the heap", pinned, heap);
+  for (auto& h : keep)
+    if (h->img_pinned) ia.give(h->img_off);
+  size_t off = 0;
+  CHECK(ia.alloc.alloc(ia.size / 256 * 256, &off) && off == 0, "freed blocks did not coalesce");
+  ia.give(off);
+  std::atomic<int> overlaps{0};
+  auto worker = [&](int id) {
+    std::mt19937 rng(id);
+    std::vector<std::pair<size_t, size_t>> mine;
+    for (int it = 0; it < 2000; ++it) {
+      if (mine.size() < 8 && rng() % 2) {
+        const size_t b = 256 + rng() % 8192;
+        size_t o;
+        if (char* pblk = ia.take(b, &o)) {
+          memset(pblk, id + 1, b);
+          mine.push_back({o, b});
+        }
+      } else if (!mine.empty()) {
+        const auto [o, b] = mine.back();
+        mine.pop_back();
+        for (size_t k = 0; k < b; ++k)
+          if (mem[o + k] != (char)(id + 1)) {
+            overlaps++;
+            break;
+          }
+        ia.give(o);
+      }
+    }
+    for (auto [o, b] : mine) ia.give(o);
+  };
+  std::vector<std::thread> th;
+  for (int id = 0; id < 8; ++id) th.emplace_back(worker, id);
+  for (auto& x : th) x.join();
+  CHECK(overlaps == 0, "concurrent arena blocks overlapped %d times", overlaps.load());
+  printf("image arena: %d images pinned, %d on the heap\n", pinned, heap);
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   FILE* f = fopen(argv[1], "rb");
@@ -201,6 +315,10 @@ int main(int argc, char** argv) {
   const bool plant = argc > 2 && strcmp(argv[2], "plant") == 0;
   int np = 0;
   if (fread(&np, 4, 1, f) != 1) return 2;
+  std::vector<std::array<OSQPCscMatrix, 2>> all;
+  std::vector<int> all_n, all_m;
+  std::vector<std::vector<int>> store_i;
+  std::vector<std::vector<double>> store_x;
   for (int t = 0; t < np; ++t) {
     int n, m;
     if (fread(&n, 4, 1, f) != 1 || fread(&m, 4, 1, f) != 1) return 2;
@@ -225,6 +343,21 @@ int main(int argc, char** argv) {
       M[k].owned = 0;
     }
     std::vector<double> q(n, 0.0), l(m, -1.0), u(m, 1.0);
+    {
+      // kept for the arena check
+      std::array<OSQPCscMatrix, 2> c = {M[0], M[1]};
+      for (int k = 0; k < 2; ++k) {
+        store_i.push_back(pp[k]);
+        c[k].p = store_i.back().data();
+        store_i.push_back(pi[k]);
+        c[k].i = store_i.back().data();
+        store_x.push_back(px[k]);
+        c[k].x = store_x.back().data();
+      }
+      all.push_back(c);
+      all_n.push_back(n);
+      all_m.push_back(m);
+    }
     for (int vbytes : {4, 8}) {
       HostData h;
       const Layout L = build_host(&h, vbytes, &M[0], q.data(), &M[1], l.data(), u.data(), nullptr, nullptr, true);
@@ -243,6 +376,7 @@ int main(int argc, char** argv) {
       check_matrix("At", blk.data(), L.total, n, m, h.gT, L.gT, L.Trp, L.Tci, L.Tval, vbytes == 4);
     }
   }
+  check_image_arena(all, all_n, all_m);
   printf("sliced matrices %lld, slices wider than one load round %lld, long rows %lld\n", n_sliced, n_wide, n_long);
   printf(fails ? "FAILURES: %d\n" : "all checks passed\n", fails);
   return fails ? 1 : 0;
