@@ -7,7 +7,10 @@ and the product loops that read them; see its header for the list of checks.
 This test writes the problems (MPC, portfolios on both sides of the sliced
 buckets' threshold, a random QP, a lasso and a matrix mixing every row-length
 class), builds the harness against the library's objects and runs it; a run
-with one slice length corrupted must fail, so the checks are not vacuous.
+with one slice length corrupted must fail, so the checks are not vacuous.  The
+harness also builds every image in an image arena over ordinary memory, sized
+to hold about half of them, and checks the arena's images against heap-built
+ones, the fall-back to the heap, and concurrent take / give from 8 threads.
 """
 import os
 import shutil
