@@ -69,6 +69,10 @@ typedef struct {
 /* precision: 32 (FP32 CG, FP64 residuals and ADMM state) or 64.
  * max_active: problems on the GPU at once (0: 4096).
  * arena_bytes: device memory for resident problems (0: 4 GiB).
+ * Pinned host memory: a staging pool of arena_bytes / 4 (at least 64 MiB, at
+ * most 1 GiB) for stage images, results and matrix updates, and an image arena
+ * of half the pool's size where osqp_batch_add builds stage images (when it
+ * cannot be had, images are built on the heap and copied at admission).
  * Returns 0, OSQP_SETTINGS_VALIDATION_ERROR or OSQP_MEM_ALLOC_ERROR. */
 OSQPInt osqp_batch_create(OSQPBatch** batch, const OSQPSettings* settings, OSQPInt precision, OSQPInt max_active,
                           long long arena_bytes);
