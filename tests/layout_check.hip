@@ -12,6 +12,8 @@
 //   * group slots hold consecutive CSR entries of one row; long-row padding
 //     and chunk loads stay inside the compact copy;
 //   * only problems of the sliced buckets get slices;
+//   * the team-size tuner times every candidate of a bucket twice, forward
+//     then backward, and the small buckets have no 512-thread candidates;
 //   * images built in the image arena (here over ordinary memory) equal the
 //     heap-built ones, a full arena falls back to the heap, freed blocks
 //     coalesce, and concurrent take / give never hand out overlapping blocks.
@@ -308,6 +310,23 @@ static void check_image_arena(const std::vector<std::array<OSQPCscMatrix, 2>>& p
   printf("image arena: %d images pinned, %d on the heap\n", pinned, heap);
 }
 
+static void check_tuner_schedule() {
+  for (int b = 0; b < kBuckets; ++b) {
+    const int nc = cands_of(b);
+    CHECK(nc >= 1 && nc <= kNumCands && trials_of(b) == 2 * nc, "bucket %d: %d candidates", b, nc);
+    std::vector<int> seen(kNumCands, 0);
+    for (int t = 0; t < trials_of(b); ++t) {
+      const int c = cand_of_trial(b, t);
+      CHECK(c >= 0 && c < nc, "bucket %d trial %d: candidate %d", b, t, c);
+      if (c >= 0 && c < kNumCands) seen[c]++;
+      CHECK(c == (t < nc ? t : 2 * nc - 1 - t), "bucket %d: trial %d not forward-then-backward", b, t);
+    }
+    for (int c = 0; c < nc; ++c) CHECK(seen[c] == 2, "bucket %d: candidate %d timed %d times", b, c, seen[c]);
+    if (b < kSmallBuckets)
+      for (int c = 0; c < nc; ++c) CHECK(kCands[c].T <= 256, "bucket %d tries a %d-thread team", b, kCands[c].T);
+  }
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   FILE* f = fopen(argv[1], "rb");
@@ -377,6 +396,7 @@ int main(int argc, char** argv) {
     }
   }
   check_image_arena(all, all_n, all_m);
+  check_tuner_schedule();
   printf("sliced matrices %lld, slices wider than one load round %lld, long rows %lld\n", n_sliced, n_wide, n_long);
   printf(fails ? "FAILURES: %d\n" : "all checks passed\n", fails);
   return fails ? 1 : 0;
